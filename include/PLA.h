@@ -1,0 +1,276 @@
+/*
+ * PLA.h -- drop-in PLAPACK R3.0 C API for libpla_mi355x (AMD Instinct
+ * MI355X, gfx950), restricted to the dense factor-and-solve path on a 1x1
+ * process template.
+ *
+ * Names, argument order and meaning, and constant values follow the
+ * reference headers (INCLUDE/PLA_define.h:11-102, INCLUDE/PLA_Obj.h,
+ * INCLUDE/PLA_Template.h) so that drivers such as TEST/test_lu.c compile
+ * unmodified.  PLA_Obj buffers live in device memory (HBM); host access
+ * through PLA_Obj_local_buffer goes through a pinned host mirror that is
+ * synchronised at every library entry and exit (see PLA_Obj_local_buffer).
+ *
+ * Every entry point below cites the reference routine it replaces.
+ * Entry points marked EXTENSION have no counterpart in the reference.
+ *
+ * Error behaviour: functions return PLA_SUCCESS (0) or a negative error
+ * code; PLA_Last_error() gives the message.  The library never calls
+ * exit() (the reference's PLA_Abort does, OBJ/PLA_Diagnostics.c:108-129).
+ * Not thread-safe, like the reference.
+ */
+#ifndef PLA_MI355X_H
+#define PLA_MI355X_H
+
+#ifdef PLA_NO_MPI_H
+#include "PLA_mpi_abi.h"
+#else
+#include <mpi.h>
+#endif
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants: values identical to INCLUDE/PLA_define.h:11-102 ---- */
+#define PLA_OBJTYPE_OFFSET 10
+#define PLA_VECTOR   (PLA_OBJTYPE_OFFSET + 1)
+#define PLA_MVECTOR  (PLA_OBJTYPE_OFFSET + 2)
+#define PLA_MATRIX   (PLA_OBJTYPE_OFFSET + 3)
+#define PLA_MSCALAR  (PLA_OBJTYPE_OFFSET + 4)
+#define PLA_PVECTOR  (PLA_OBJTYPE_OFFSET + 5)
+#define PLA_PMVECTOR (PLA_OBJTYPE_OFFSET + 6)
+
+#define PLA_TRANSPOSE_OFFSET 20
+#define PLA_NO_TRANSPOSE        (PLA_TRANSPOSE_OFFSET + 1)
+#define PLA_NO_TRANS            (PLA_TRANSPOSE_OFFSET + 1)
+#define PLA_TRANSPOSE           (PLA_TRANSPOSE_OFFSET + 2)
+#define PLA_TRANS               (PLA_TRANSPOSE_OFFSET + 2)
+#define PLA_CONJUGATE_TRANSPOSE (PLA_TRANSPOSE_OFFSET + 3)
+#define PLA_CONJ_TRANS          (PLA_TRANSPOSE_OFFSET + 3)
+#define PLA_CONJUGATE           (PLA_TRANSPOSE_OFFSET + 4)
+#define PLA_CONJ                (PLA_TRANSPOSE_OFFSET + 4)
+
+#define PLA_OWNER_OFFSET -30
+#define PLA_ALL_ROWS (PLA_OWNER_OFFSET - 1)
+#define PLA_ALL_COLS (PLA_OWNER_OFFSET - 2)
+
+#define PLA_SIDE_OFFSET 40
+#define PLA_SIDE_TOP    (PLA_SIDE_OFFSET + 1)
+#define PLA_SIDE_BOTTOM (PLA_SIDE_OFFSET + 2)
+#define PLA_SIDE_LEFT   (PLA_SIDE_OFFSET + 3)
+#define PLA_SIDE_RIGHT  (PLA_SIDE_OFFSET + 4)
+
+#define PLA_SHAPE_OFFSET 50
+#define PLA_UPPER_TRIANGULAR        (PLA_SHAPE_OFFSET + 1)
+#define PLA_UP_TRIAN                (PLA_SHAPE_OFFSET + 1)
+#define PLA_LOWER_TRIANGULAR        (PLA_SHAPE_OFFSET + 2)
+#define PLA_LOWER_TRIAN             (PLA_SHAPE_OFFSET + 2)
+#define PLA_SHAPE_GENERAL           (PLA_SHAPE_OFFSET + 3)
+#define PLA_SHAPE_LOWER_TRAPEZOIDAL (PLA_SHAPE_OFFSET + 4)
+#define PLA_SHAPE_LOW_TRAP          (PLA_SHAPE_OFFSET + 4)
+#define PLA_SHAPE_UPPER_TRAPEZOIDAL (PLA_SHAPE_OFFSET + 5)
+#define PLA_SHAPE_UP_TRAP           (PLA_SHAPE_OFFSET + 5)
+
+#define PLA_OP_OFFSET 80
+#define PLA_OP_MAT_PAN       (PLA_OP_OFFSET + 1)
+#define PLA_OP_PAN_MAT       (PLA_OP_OFFSET + 2)
+#define PLA_OP_PAN_PAN       (PLA_OP_OFFSET + 3)
+#define PLA_OP_PAN_TRIAN_MAT (PLA_OP_OFFSET + 4)
+#define PLA_OP_SYM_PAN_PAN   (PLA_OP_OFFSET + 5)
+#define PLA_OP_TRIAN_MAT_PAN (PLA_OP_OFFSET + 6)
+#define PLA_OP_ALL_ALG       (PLA_OP_OFFSET + 7)
+
+#define PLA_PROJ_OFFSET 90
+#define PLA_PROJ_ONTO_COL (PLA_PROJ_OFFSET + 1)
+#define PLA_PROJ_ONTO_ROW (PLA_PROJ_OFFSET + 2)
+
+#define PLA_DIAG_OFFSET 100
+#define PLA_UNIT_DIAG    (PLA_DIAG_OFFSET + 1)
+#define PLA_NONUNIT_DIAG (PLA_DIAG_OFFSET + 2)
+#define PLA_ZERO_DIAG    (PLA_DIAG_OFFSET + 3)
+
+#define PLA_DUMMY        0
+#define PLA_SUCCESS      0
+#define PLA_ALIGN_FIRST  -200
+#define PLA_DIM_ALL      -201
+#define PLA_INHERIT      -202
+#define PLA_UNDEFINED    -203
+
+#ifndef TRUE
+#define TRUE 1
+#endif
+#ifndef FALSE
+#define FALSE 0
+#endif
+#ifndef min
+#define min(x, y) ((x) < (y) ? (x) : (y))
+#endif
+
+/* Negative error codes (the reference aborts instead). */
+#define PLA_ERR_NOT_INITIALIZED -1001
+#define PLA_ERR_INVALID_OBJECT  -1002
+#define PLA_ERR_INVALID_PARAM   -1003
+#define PLA_ERR_UNSUPPORTED     -1004
+#define PLA_ERR_DIMENSIONS      -1005
+#define PLA_ERR_DEVICE          -1006
+#define PLA_ERR_NO_MEMORY       -1007
+#define PLA_ERR_MULTI_PROCESS   -1008
+
+/* ---- opaque handles (INCLUDE/PLA_Obj.h:82, INCLUDE/PLA_Template.h:31) ---- */
+typedef struct PLA_Obj_view_struct *PLA_Obj;
+typedef struct PLA_Template_struct *PLA_Template;
+
+/* Checking flags (OBJ/PLA_Diagnostics.c:16-19). */
+extern int PLA_ERROR_CHECKING;
+extern int PLA_CHECK_PARAMETERS;
+extern int PLA_CHECK_AGAINST_SEQUENTIAL;
+extern int PLA_CHECK_AGAINST_R12;
+
+/* ---- initialisation and templates ---- */
+/* TEMPLATE/PLA_Init.c:20.  Only a 1x1 mesh is supported: rows*cols != 1
+   returns PLA_ERR_MULTI_PROCESS.  comm2d receives comm1d. */
+int PLA_Comm_1D_to_2D(MPI_Comm comm1d, int rows, int cols, MPI_Comm *comm2d);
+/* TEMPLATE/PLA_Init.c:85.  Also selects and initialises the HIP device. */
+int PLA_Init(MPI_Comm comm);
+/* TEMPLATE/PLA_Init.c:132 */
+int PLA_Finalize(void);
+/* TEMPLATE/PLA_Init.c:113 */
+int PLA_Initialized(int *initialized);
+/* TEMPLATE/PLA_Temp.c:19 */
+int PLA_Temp_create(int nb, int zero_or_one, PLA_Template *templ);
+/* TEMPLATE/PLA_Temp.c:136 */
+int PLA_Temp_free(PLA_Template *templ);
+/* TEMPLATE/PLA_Temp.c:176, :269, :169, :197, :205 */
+int PLA_Temp_nb(PLA_Template templ, int *nb);
+int PLA_Temp_zero_or_one(PLA_Template templ, int *zero_or_one);
+int PLA_Temp_comm_all_size(PLA_Template templ, int *nprocs);
+int PLA_Temp_comm_all(PLA_Template templ, MPI_Comm *comm);
+int PLA_Temp_comm_all_rank(PLA_Template templ, int *rank);
+
+/* TEMPLATE/PLA_Environ.c:88 (lower-case name is the one the reference links) */
+int pla_Environ_set_nb_alg(int type, int nb_alg);
+/* TEMPLATE/PLA_Environ.c:54 */
+int PLA_Environ_nb_alg(int type, PLA_Template templ, int *nb_alg);
+
+/* OBJ/PLA_Diagnostics.c:402, :436 */
+int PLA_Set_error_checking(int error, int parameters, int sequential, int r12);
+int PLA_Get_error_checking(int *error, int *parameters, int *sequential, int *r12);
+
+/* ---- objects (OBJ/PLA_Obj.c) ---- */
+/* :323 */
+int PLA_Matrix_create(MPI_Datatype datatype, int global_length, int global_width,
+                      PLA_Template templ, int global_align_row, int global_align_col,
+                      PLA_Obj *new_obj);
+/* :149 */
+int PLA_Mvector_create(MPI_Datatype datatype, int global_length, int global_width,
+                       PLA_Template templ, int global_align, PLA_Obj *new_obj);
+/* :497 */
+int PLA_Mscalar_create(MPI_Datatype datatype, int owner_row, int owner_col, int length,
+                       int width, PLA_Template templ, PLA_Obj *new_obj);
+/* :3985, :3720, :4025 */
+int PLA_Matrix_create_conf_to(PLA_Obj obj, PLA_Obj *new_obj);
+int PLA_Mvector_create_conf_to(PLA_Obj obj, int global_width, PLA_Obj *new_obj);
+int PLA_Mscalar_create_conf_to(PLA_Obj obj, int owner_row, int owner_col, PLA_Obj *new_obj);
+/* :4189 -- duplicated 1x1 constants of obj's datatype; NULL skips one */
+int PLA_Create_constants_conf_to(PLA_Obj obj, PLA_Obj *minus_one, PLA_Obj *zero, PLA_Obj *one);
+/* :803 */
+int PLA_Obj_free(PLA_Obj *obj);
+
+/* :831-1430 queries */
+int PLA_Obj_objtype(PLA_Obj obj, int *objtype);
+int PLA_Obj_datatype(PLA_Obj obj, MPI_Datatype *datatype);
+int PLA_Obj_template(PLA_Obj obj, PLA_Template *templ);
+int PLA_Obj_global_length(PLA_Obj obj, int *global_length);
+int PLA_Obj_global_width(PLA_Obj obj, int *global_width);
+int PLA_Obj_owner_row(PLA_Obj obj, int *owner_row);
+int PLA_Obj_owner_col(PLA_Obj obj, int *owner_col);
+int PLA_Obj_local_length(PLA_Obj obj, int *local_length);
+int PLA_Obj_local_width(PLA_Obj obj, int *local_width);
+int PLA_Obj_local_ldim(PLA_Obj obj, int *local_ldim);
+int PLA_Obj_local_stride(PLA_Obj obj, int *local_stride);
+/* :1350.  Returns a HOST pointer into a pinned mirror of the object's
+   device buffer.  The mirror is filled from the device on first use, and
+   it is treated as possibly written by the caller from then on: every
+   library entry that reads the object first copies the mirror to the
+   device, and every entry that writes the object copies the result back
+   into the mirror before returning. */
+int PLA_Obj_local_buffer(PLA_Obj obj, void **local_buffer);
+/* :1437, :1545 -- host <-> device copies of the local contents */
+int PLA_Obj_get_local_contents(PLA_Obj obj, int trans, int *rows_in_buf, int *cols_in_buf,
+                               void *buf, int leading_dim_buf, int stride_buf);
+int PLA_Obj_set_local_contents(int trans, int rows_in_buf, int cols_in_buf, void *buf,
+                               int leading_dim_buf, int stride_buf, PLA_Obj obj);
+/* :1807, :1838, :1870 */
+int PLA_Obj_set_to_zero(PLA_Obj obj);
+int PLA_Obj_set_to_one(PLA_Obj obj);
+int PLA_Obj_set_to_minus_one(PLA_Obj obj);
+
+/* :1981, :2063, :3122, :3156 -- views share the base object */
+int PLA_Obj_view_all(PLA_Obj old_obj, PLA_Obj *new_obj);
+int PLA_Obj_split_4(PLA_Obj obj, int length, int width, PLA_Obj *upper_left,
+                    PLA_Obj *upper_right, PLA_Obj *lower_left, PLA_Obj *lower_right);
+int PLA_Obj_horz_split_2(PLA_Obj obj, int length, PLA_Obj *upper, PLA_Obj *lower);
+int PLA_Obj_vert_split_2(PLA_Obj obj, int width, PLA_Obj *left, PLA_Obj *right);
+
+/* COPY/PLA_Copy.c:18 -- same-shape copy between any two objects */
+int PLA_Copy(PLA_Obj from, PLA_Obj to);
+
+/* ---- factor and solve path ---- */
+/* SOLVERS/LU/PLA_LU.c:13.  A: m x n MPI_DOUBLE matrix, overwritten with L
+   (unit, below the diagonal) and U; pivots: MPI_INT mvector of length
+   min(m,n) receiving, for each row j, the 0-based offset of its pivot row
+   relative to row j.  Returns PLA_SUCCESS; like the reference, a zero pivot
+   is not flagged (the factors then hold inf/nan). */
+int PLA_LU(PLA_Obj A, PLA_Obj pivots);
+/* SOLVERS/Cholesky/PLA_Chol.c:13.  Lower only (the reference aborts on
+   upper).  Returns 0, or the 1-based global index k+i+1 of the first
+   non-positive pivot (PLA_Chol.c:181-185); the diagonal block that failed
+   is left as it was before its factorisation, as in the reference. */
+int PLA_Chol(int uplo, PLA_Obj A);
+/* SOLVERS/LU/PLA_Apply_pivots_new.c:13 -- forward sequential row swaps */
+int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots);
+/* BLAS2/PLA_Trsv.c:15 */
+int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x);
+/* BLAS3/PLA_Trsm.c:13 (side = PLA_SIDE_LEFT) */
+int PLA_Trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, PLA_Obj B);
+/* BLAS2/PLA_Gemv.c:13 */
+int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y);
+/* BLAS3/PLA_Gemm.c:13 */
+int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta,
+             PLA_Obj C);
+/* SOLVERS/LU/PLA_General.c:67 -- factor A, then solve A X = B in place */
+int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B);
+/* SOLVERS/Cholesky/PLA_Pos_def.c:44 -- factor A = L L^T, then solve in place */
+int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B);
+
+/* ---- EXTENSIONS (no counterpart in the reference) ---- */
+/* Message of the last error returned by any entry point. */
+const char *PLA_Last_error(void);
+/* Device pointer of the object's top-left element (HBM).  Drops the host
+   mirror, so later writes through the device pointer are authoritative. */
+int PLA_Obj_device_buffer(PLA_Obj obj, void **device_buffer);
+/* Fill obj with drand48()*2-1 in column-major order, starting `skip` draws
+   after srand48(seed) -- the reference's TEST/create_problem.c:31-52 input,
+   generated on the device with LCG jump-ahead (bit-identical values). */
+int PLA_Obj_fill_drand48(PLA_Obj obj, long seed, int64_t skip);
+/* Wait for all device work issued by the library. */
+int PLA_Device_synchronize(void);
+/* Block size the device LU/Cholesky use when pla_Environ_set_nb_alg has not
+   been called (0 restores the built-in choice). */
+int PLA_Set_device_nb(int nb);
+int PLA_Get_device_nb(int n, int *nb);
+/* Per-launch hipEvent timing of the trailing-update kernels on the library
+   stream (the reference keeps per-operation timers in PLA_TIMINGS,
+   OBJ/PLA_Timings.c:17-125).  role 1: PLA_LU's A22 -= A21 A12 GEMM;
+   role 2: PLA_Chol's lower(A22) -= A21 A21^T SYRK.  PLA_Kernel_timing(on)
+   clears the counters; the query waits for the recorded events and returns
+   the launch count, summed kernel milliseconds and summed algorithmic flops. */
+int PLA_Kernel_timing(int on);
+int PLA_Kernel_timing_query(int role, long *launches, double *total_ms, double *total_flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

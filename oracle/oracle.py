@@ -1,0 +1,188 @@
+"""ctypes front end of the CPU oracle (oracle/pla_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is the checker the HIP library is
+compared with; only tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline may import this module.  The product library never loads it.
+
+All arrays are numpy float64 in Fortran (column-major) order, int32 pivots in
+the reference's encoding (offset of the pivot row relative to row j,
+SOLVERS/LU/PLA_seq_lu.c:33-35).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libpla_oracle.so")
+_lib = None
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+
+
+def build():
+    """Compile the oracle with gcc (oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.ora_srand48.restype = ctypes.c_uint64
+        L.ora_srand48.argtypes = [ctypes.c_long]
+        L.ora_drand48.restype = _d
+        L.ora_drand48.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+        L.ora_lcg_skip.restype = ctypes.c_uint64
+        L.ora_lcg_skip.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.ora_fill_uniform.argtypes = [_i, _i, _p, _i, ctypes.c_long, ctypes.c_int64]
+        L.ora_idamax.restype = _i
+        L.ora_idamax.argtypes = [_i, _p, _i]
+        L.ora_lu_unblocked.argtypes = [_i, _i, _p, _i, _p]
+        L.ora_lu_blocked.argtypes = [_i, _i, _p, _i, _i, _p]
+        L.ora_laswp.argtypes = [_i, _p, _i, _i, _i, _p]
+        L.ora_chol_unblocked.restype = _i
+        L.ora_chol_unblocked.argtypes = [_i, _p, _i]
+        L.ora_chol_blocked.restype = _i
+        L.ora_chol_blocked.argtypes = [_i, _p, _i, _i]
+        L.ora_trsv.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i, _p, _i, _p]
+        L.ora_trsm_left.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i, _i, _d, _p, _i,
+                                    _p, _i]
+        L.ora_gemm.argtypes = [ctypes.c_char, ctypes.c_char, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]
+        L.ora_lu_solve.argtypes = [_i, _i, _p, _i, _p, _p, _i]
+        L.ora_chol_solve.argtypes = [_i, _i, _p, _i, _p, _i]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    a = np.asarray(a, dtype=np.float64)
+    return np.asfortranarray(a)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def drand48_stream(seed, count, skip=0):
+    """count consecutive drand48() values after srand48(seed) and `skip` draws."""
+    L = lib()
+    st = ctypes.c_uint64(L.ora_lcg_skip(L.ora_srand48(seed), skip))
+    return np.array([L.ora_drand48(ctypes.byref(st)) for _ in range(count)])
+
+
+def uniform(m, n, seed=0, skip=0):
+    """A(i,j) = drand48()*2-1, column-major draw order (TEST/create_problem.c:44-52)."""
+    a = np.zeros((m, n), dtype=np.float64, order="F")
+    lib().ora_fill_uniform(m, n, _ptr(a), max(m, 1), seed, skip)
+    return a
+
+
+def idamax(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().ora_idamax(len(x), _ptr(x), 1)
+
+
+def lu_unblocked(a):
+    a = _f(a).copy(order="F")
+    m, n = a.shape
+    piv = np.zeros(min(m, n), dtype=np.int32)
+    lib().ora_lu_unblocked(m, n, _ptr(a), max(m, 1), _ptr(piv))
+    return a, piv
+
+
+def lu_blocked(a, nb=128):
+    a = _f(a).copy(order="F")
+    m, n = a.shape
+    piv = np.zeros(min(m, n), dtype=np.int32)
+    lib().ora_lu_blocked(m, n, _ptr(a), max(m, 1), nb, _ptr(piv))
+    return a, piv
+
+
+def laswp(a, piv, r0=0):
+    a = _f(a).copy(order="F")
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    lib().ora_laswp(a.shape[1], _ptr(a), max(a.shape[0], 1), r0, len(piv), _ptr(piv))
+    return a
+
+
+def chol_unblocked(a):
+    a = _f(a).copy(order="F")
+    info = lib().ora_chol_unblocked(a.shape[0], _ptr(a), max(a.shape[0], 1))
+    return a, info
+
+
+def chol_blocked(a, nb=128):
+    a = _f(a).copy(order="F")
+    info = lib().ora_chol_blocked(a.shape[0], _ptr(a), max(a.shape[0], 1), nb)
+    return a, info
+
+
+def trsm_left(uplo, trans, diag, t, b, alpha=1.0):
+    t = _f(t)
+    b = _f(b).copy(order="F")
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    n, nrhs = b.shape
+    lib().ora_trsm_left(uplo.encode(), trans.encode(), diag.encode(), n, nrhs, alpha, _ptr(t),
+                        max(t.shape[0], 1), _ptr(b), max(n, 1))
+    return b
+
+
+def gemm(ta, tb, alpha, a, b, beta, c):
+    a, b = _f(a), _f(b)
+    c = _f(c).copy(order="F")
+    m, n = c.shape
+    k = a.shape[0] if ta == "T" else a.shape[1]
+    lib().ora_gemm(ta.encode(), tb.encode(), m, n, k, alpha, _ptr(a), max(a.shape[0], 1), _ptr(b),
+                   max(b.shape[0], 1), beta, _ptr(c), max(m, 1))
+    return c
+
+
+def lu_solve(lu, piv, b):
+    lu = _f(lu)
+    b = _f(b).copy(order="F")
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    n, nrhs = b.shape
+    lib().ora_lu_solve(n, nrhs, _ptr(lu), max(lu.shape[0], 1), _ptr(piv), _ptr(b), max(n, 1))
+    return b
+
+
+def chol_solve(l, b):
+    l = _f(l)
+    b = _f(b).copy(order="F")
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    n, nrhs = b.shape
+    lib().ora_chol_solve(n, nrhs, _ptr(l), max(l.shape[0], 1), _ptr(b), max(n, 1))
+    return b
+
+
+def split_lu(lu, piv):
+    """(P, L, U) with P A = L U from the packed factors and reference pivots."""
+    m, n = lu.shape
+    k = min(m, n)
+    L = np.tril(lu[:, :k], -1) + np.eye(m, k)
+    U = np.triu(lu[:k, :])
+    perm = np.arange(m)
+    for j, p in enumerate(piv):
+        if p:
+            perm[[j, j + p]] = perm[[j + p, j]]
+    return perm, L, U
+
+
+def lu_scaled_residual(a, lu, piv):
+    """||P A - L U||_1 / (n ||A||_1 eps), the acceptance measure (< 16)."""
+    perm, L, U = split_lu(lu, piv)
+    pa = np.asarray(a)[perm, :]
+    r = np.abs(pa - L @ U).sum(axis=0).max()
+    na = np.abs(np.asarray(a)).sum(axis=0).max()
+    n = max(a.shape)
+    return r / (n * na * np.finfo(np.float64).eps)

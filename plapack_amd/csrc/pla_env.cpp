@@ -1,0 +1,339 @@
+// pla_env.cpp -- initialisation, 1x1 template, block-size environment,
+// checking flags, error reporting and device workspaces.
+//
+// Reference counterparts: TEMPLATE/PLA_Init.c, TEMPLATE/PLA_Temp.c,
+// TEMPLATE/PLA_Environ.c, OBJ/PLA_Diagnostics.c.
+#include "pla_internal.hpp"
+
+#include <dlfcn.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+int PLA_ERROR_CHECKING = 1;            // OBJ/PLA_Diagnostics.c:16-19
+int PLA_CHECK_PARAMETERS = 1;
+int PLA_CHECK_AGAINST_SEQUENTIAL = 0;
+int PLA_CHECK_AGAINST_R12 = 0;
+
+namespace pla {
+
+static std::string g_last_error;
+static context g_ctx;
+context &ctx() { return g_ctx; }
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(e == hipErrorOutOfMemory ? PLA_ERR_NO_MEMORY : PLA_ERR_DEVICE, "%s: %s", what,
+              hipGetErrorString(e));
+}
+
+int ensure_device() {
+  context &c = g_ctx;
+  if (c.stream) return PLA_SUCCESS;
+  int count = 0;
+  PLA_HIP(hipGetDeviceCount(&count));
+  if (count < 1) return fail(PLA_ERR_DEVICE, "no HIP device visible");
+  PLA_HIP(hipGetDevice(&c.device));
+  PLA_HIP(hipSetDevice(c.device));
+  int cus = 0;
+  PLA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+  c.cu_count = cus > 0 ? cus : 256;
+  PLA_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  PLA_HIP(hipStreamCreateWithFlags(&c.stream_la, hipStreamNonBlocking));
+  return PLA_SUCCESS;
+}
+
+struct ws_slot {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+static ws_slot g_ws[8];
+
+void *workspace(size_t bytes, int slot) {
+  if (slot < 0 || slot >= 8) return nullptr;
+  ws_slot &w = g_ws[slot];
+  if (w.bytes >= bytes && w.p) return w.p;
+  if (w.p) {
+    // in-flight kernels may still use the old buffer
+    if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+    (void)hipFree(w.p);
+    w.p = nullptr;
+    w.bytes = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  if (hipMalloc(&w.p, want) != hipSuccess) {
+    w.p = nullptr;
+    fail(PLA_ERR_NO_MEMORY, "workspace allocation of %zu bytes failed", want);
+    return nullptr;
+  }
+  w.bytes = want;
+  return w.p;
+}
+
+static leaf_ws g_leaf;
+
+leaf_ws &leaf_workspace(int G) {
+  if (g_leaf.base && g_leaf.capacity >= G) return g_leaf;
+  if (g_leaf.base) {
+    if (hipDeviceSynchronize() != hipSuccess) return g_leaf;
+    (void)hipFree(g_leaf.base);
+    g_leaf = leaf_ws();
+  }
+  const size_t W = 32;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_row = take(sizeof(double) * 2 * G * W);
+  const size_t o_val = take(sizeof(double) * 2 * G);
+  const size_t o_idx = take(sizeof(int) * 2 * G);
+  const size_t o_rowj = take(sizeof(double) * 2 * W);
+  const size_t o_flags = take(sizeof(unsigned) * G);
+  const size_t o_err = take(sizeof(int));
+  void *p = nullptr;
+  if (hipMalloc(&p, off) != hipSuccess) {
+    fail(PLA_ERR_NO_MEMORY, "panel exchange workspace allocation failed");
+    return g_leaf;
+  }
+  if (hipMemset(p, 0, off) != hipSuccess) {
+    (void)hipFree(p);
+    fail(PLA_ERR_DEVICE, "panel exchange workspace initialisation failed");
+    return g_leaf;
+  }
+  char *b = (char *)p;
+  g_leaf.base = p;
+  g_leaf.capacity = G;
+  g_leaf.row = (double *)(b + o_row);
+  g_leaf.val = (double *)(b + o_val);
+  g_leaf.idx = (int *)(b + o_idx);
+  g_leaf.rowj = (double *)(b + o_rowj);
+  g_leaf.flags = (unsigned *)(b + o_flags);
+  g_leaf.err = (int *)(b + o_err);
+  return g_leaf;
+}
+
+int leaf_check_error(hipStream_t s) {
+  if (!g_leaf.base) return PLA_SUCCESS;
+  int h = 0;
+  PLA_HIP(hipMemcpyAsync(&h, g_leaf.err, sizeof(int), hipMemcpyDeviceToHost, s));
+  PLA_HIP(hipStreamSynchronize(s));
+  if (h) {
+    PLA_HIP(hipMemsetAsync(g_leaf.err, 0, sizeof(int), s));
+    PLA_HIP(hipStreamSynchronize(s));
+    return fail(PLA_ERR_DEVICE, "panel factorization: a cross-workgroup exchange timed out");
+  }
+  return PLA_SUCCESS;
+}
+
+void release_workspaces() {
+  for (auto &w : g_ws) {
+    if (w.p) (void)hipFree(w.p);
+    w.p = nullptr;
+    w.bytes = 0;
+  }
+  if (g_leaf.base) (void)hipFree(g_leaf.base);
+  g_leaf = leaf_ws();
+}
+
+int choose_nb(int n, int op) {
+  const context &c = g_ctx;
+  if (c.nb_user > 0) return c.nb_user;
+  if (c.nb_device > 0) return c.nb_device;
+  (void)op;
+  if (n >= 8192) return 256;
+  return 128;
+}
+
+}  // namespace pla
+
+using pla::fail;
+
+static int g_initialized = 0;
+static MPI_Comm g_comm = MPI_COMM_NULL;
+static int g_nb_table[6] = {128, 128, 128, 128, 128, 128};  // TEMPLATE/PLA_Environ.c:37-44
+
+// If the calling process has initialised an MPI library, ask it for the
+// communicator size (1x1 template only).  No MPI library is linked.
+static int mpi_comm_size(MPI_Comm comm, int *size) {
+  *size = 1;
+  typedef int (*initialized_fn)(int *);
+  typedef int (*size_fn)(MPI_Comm, int *);
+  initialized_fn initd = (initialized_fn)dlsym(RTLD_DEFAULT, "MPI_Initialized");
+  size_fn sz = (size_fn)dlsym(RTLD_DEFAULT, "MPI_Comm_size");
+  if (!initd || !sz) return 0;
+  int flag = 0;
+  if (initd(&flag) != 0 || !flag) return 0;
+  if (comm == MPI_COMM_NULL) return 0;
+  return sz(comm, size);
+}
+
+extern "C" {
+
+const char *PLA_Last_error(void) { return pla::g_last_error.c_str(); }
+
+int PLA_Comm_1D_to_2D(MPI_Comm comm1d, int rows, int cols, MPI_Comm *comm2d) {
+  if (!comm2d) return fail(PLA_ERR_INVALID_PARAM, "PLA_Comm_1D_to_2D: comm2d is NULL");
+  if (rows * cols != 1)
+    return fail(PLA_ERR_MULTI_PROCESS,
+                "PLA_Comm_1D_to_2D: only a 1x1 mesh is supported (asked for %d x %d)", rows, cols);
+  int size = 1;
+  mpi_comm_size(comm1d, &size);
+  if (size != 1)
+    return fail(PLA_ERR_MULTI_PROCESS, "PLA_Comm_1D_to_2D: communicator has %d ranks", size);
+  *comm2d = comm1d;
+  return PLA_SUCCESS;
+}
+
+int PLA_Init(MPI_Comm comm) {
+  if (g_initialized) return fail(PLA_ERR_INVALID_PARAM, "PLA_Init called more than once");
+  int size = 1;
+  mpi_comm_size(comm, &size);
+  if (size != 1)
+    return fail(PLA_ERR_MULTI_PROCESS, "PLA_Init: communicator has %d ranks, need 1", size);
+  int rc = pla::ensure_device();
+  if (rc) return rc;
+  g_comm = comm;
+  g_initialized = 1;
+  pla::ctx().initialized = true;
+  return PLA_SUCCESS;
+}
+
+int PLA_Finalize(void) {
+  if (!g_initialized) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
+  pla::context &c = pla::ctx();
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  if (c.stream_la) (void)hipStreamSynchronize(c.stream_la);
+  pla::release_workspaces();
+  g_initialized = 0;
+  c.initialized = false;
+  return PLA_SUCCESS;
+}
+
+int PLA_Initialized(int *initialized) {
+  if (initialized) *initialized = g_initialized;
+  return PLA_SUCCESS;
+}
+
+int PLA_Temp_create(int nb, int zero_or_one, PLA_Template *templ) {
+  if (!templ) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_create: templ is NULL");
+  if (!g_initialized) return fail(PLA_ERR_NOT_INITIALIZED, "PLA_Temp_create before PLA_Init");
+  if (nb <= 0) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_create: nb = %d", nb);
+  if (zero_or_one != 0 && zero_or_one != 1)
+    return fail(PLA_ERR_INVALID_PARAM, "Error in PLA_temp_create: zero_or_one");
+  PLA_Template t = (PLA_Template)calloc(1, sizeof(struct PLA_Template_struct));
+  if (!t) return fail(PLA_ERR_NO_MEMORY, "PLA_Temp_create: out of host memory");
+  t->comm_all = t->comm_row = t->comm_col = g_comm;
+  t->nb = nb;
+  t->zero_or_one = zero_or_one;
+  t->nprows = t->npcols = 1;
+  t->times_used = 1;
+  *templ = t;
+  return PLA_SUCCESS;
+}
+
+int PLA_Temp_free(PLA_Template *templ) {
+  if (!templ || !*templ) return PLA_SUCCESS;
+  if (--(*templ)->times_used <= 0) free(*templ);
+  *templ = NULL;
+  return PLA_SUCCESS;
+}
+
+int PLA_Temp_nb(PLA_Template templ, int *nb) {
+  if (!templ || !nb) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_nb: NULL argument");
+  *nb = templ->nb;
+  return PLA_SUCCESS;
+}
+int PLA_Temp_zero_or_one(PLA_Template templ, int *zero_or_one) {
+  if (!templ || !zero_or_one) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_zero_or_one: NULL");
+  *zero_or_one = templ->zero_or_one;
+  return PLA_SUCCESS;
+}
+int PLA_Temp_comm_all_size(PLA_Template templ, int *nprocs) {
+  if (!templ || !nprocs) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_comm_all_size: NULL");
+  *nprocs = 1;
+  return PLA_SUCCESS;
+}
+int PLA_Temp_comm_all(PLA_Template templ, MPI_Comm *comm) {
+  if (!templ || !comm) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_comm_all: NULL");
+  *comm = templ->comm_all;
+  return PLA_SUCCESS;
+}
+int PLA_Temp_comm_all_rank(PLA_Template templ, int *rank) {
+  if (!templ || !rank) return fail(PLA_ERR_INVALID_PARAM, "PLA_Temp_comm_all_rank: NULL");
+  *rank = 0;
+  return PLA_SUCCESS;
+}
+
+int pla_Environ_set_nb_alg(int type, int nb_alg) {
+  if (nb_alg <= 0) return fail(PLA_ERR_INVALID_PARAM, "pla_Environ_set_nb_alg: nb_alg = %d", nb_alg);
+  if (type == PLA_OP_ALL_ALG) {
+    for (int &v : g_nb_table) v = nb_alg;
+  } else if (type >= PLA_OP_MAT_PAN && type <= PLA_OP_TRIAN_MAT_PAN) {
+    g_nb_table[type - PLA_OP_MAT_PAN] = nb_alg;
+  } else {
+    return fail(PLA_ERR_INVALID_PARAM, "Invalid operation type");
+  }
+  // the factorisations use the PAN_PAN entry (SOLVERS/LU/PLA_LU.c:26)
+  pla::ctx().nb_user = g_nb_table[PLA_OP_PAN_PAN - PLA_OP_MAT_PAN];
+  return PLA_SUCCESS;
+}
+
+int PLA_Environ_nb_alg(int type, PLA_Template templ, int *nb_alg) {
+  (void)templ;
+  if (!nb_alg) return fail(PLA_ERR_INVALID_PARAM, "PLA_Environ_nb_alg: NULL");
+  if (type == PLA_OP_ALL_ALG) type = PLA_OP_PAN_PAN;
+  if (type < PLA_OP_MAT_PAN || type > PLA_OP_TRIAN_MAT_PAN)
+    return fail(PLA_ERR_INVALID_PARAM, "Invalid operation type");
+  *nb_alg = g_nb_table[type - PLA_OP_MAT_PAN];
+  return PLA_SUCCESS;
+}
+
+int PLA_Set_error_checking(int error, int parameters, int sequential, int r12) {
+  if (error >= 0) PLA_ERROR_CHECKING = (error > 0);
+  if (parameters >= 0) PLA_CHECK_PARAMETERS = (parameters > 0);
+  if (sequential >= 0) PLA_CHECK_AGAINST_SEQUENTIAL = (sequential > 0);
+  if (r12 >= 0) PLA_CHECK_AGAINST_R12 = (r12 > 0);
+  return PLA_SUCCESS;
+}
+
+int PLA_Get_error_checking(int *error, int *parameters, int *sequential, int *r12) {
+  if (error) *error = PLA_ERROR_CHECKING;
+  if (parameters) *parameters = PLA_CHECK_PARAMETERS;
+  if (sequential) *sequential = PLA_CHECK_AGAINST_SEQUENTIAL;
+  if (r12) *r12 = PLA_CHECK_AGAINST_R12;
+  return PLA_SUCCESS;
+}
+
+int PLA_Device_synchronize(void) {
+  if (!pla::ctx().stream) return PLA_SUCCESS;
+  PLA_HIP(hipDeviceSynchronize());
+  return PLA_SUCCESS;
+}
+
+int PLA_Set_device_nb(int nb) {
+  if (nb < 0 || (nb > 0 && (nb % 32) != 0))
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Set_device_nb: nb must be 0 or a multiple of 32");
+  pla::ctx().nb_device = nb;
+  return PLA_SUCCESS;
+}
+
+int PLA_Get_device_nb(int n, int *nb) {
+  if (!nb) return fail(PLA_ERR_INVALID_PARAM, "PLA_Get_device_nb: NULL");
+  *nb = pla::choose_nb(n, PLA_OP_PAN_PAN);
+  return PLA_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// pla_internal.hpp -- internal types of libpla_mi355x (not installed).
+//
+// A PLA_Obj is a view {objtype, shape, offset} onto a reference-counted base
+// object that owns one column-major device buffer in HBM, plus an optional
+// pinned host mirror for callers that use PLA_Obj_local_buffer.  This is the
+// reference's base/view split (INCLUDE/PLA_Obj.h:15-82) on a 1x1 template,
+// where every "local" quantity equals the global one.
+#pragma once
+
+#define PLA_NO_MPI_H
+#include "PLA.h"
+#undef min  // PLA.h keeps the reference's min() macro for C drivers
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+struct pla_base {
+  MPI_Datatype datatype;
+  int typesize;           // bytes per element
+  int length, width;      // base dimensions
+  int ldim;               // elements, >= max(1, length)
+  void *dev;              // device buffer (HBM)
+  void *host;             // pinned mirror or nullptr
+  bool host_dirty;        // mirror may hold data newer than the device copy
+  int number_of_views;    // reference count (OBJ/PLA_Obj.c:2019)
+  PLA_Template templ;
+};
+
+struct PLA_Obj_view_struct {
+  int objtype;
+  int fixed;              // constants from PLA_Create_constants_conf_to
+  int length, width;      // view dimensions
+  int row0, col0;         // offset of the view's top-left element in the base
+  int owner_row, owner_col;
+  int proj_onto;
+  pla_base *base;
+};
+
+struct PLA_Template_struct {
+  MPI_Comm comm_all, comm_row, comm_col;
+  int nb, zero_or_one, me, myrow, mycol, nprows, npcols, times_used;
+};
+
+namespace pla {
+
+// ---- error reporting ----
+int fail(int code, const char *fmt, ...);      // records message, returns code
+int hip_fail(hipError_t e, const char *what);  // maps a HIP error
+#define PLA_HIP(call)                                         \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return pla::hip_fail(e_, #call);    \
+  } while (0)
+
+// ---- library context ----
+struct context {
+  bool initialized = false;
+  int device = 0;
+  hipStream_t stream = nullptr;      // main stream (default priority)
+  hipStream_t stream_la = nullptr;   // look-ahead stream (default priority)
+  int nb_user = 0;                   // set by pla_Environ_set_nb_alg (0 = not set)
+  int nb_device = 0;                 // PLA_Set_device_nb (0 = built-in choice)
+  int cu_count = 256;
+};
+context &ctx();
+int ensure_device();                 // lazily initialise device + streams
+
+// Grow-only device workspace (reused across calls, freed at PLA_Finalize).
+void *workspace(size_t bytes, int slot);
+void release_workspaces();
+
+// ---- object helpers ----
+bool valid(PLA_Obj obj);
+inline size_t elem_offset(PLA_Obj o) {
+  return (size_t)o->col0 * (size_t)o->base->ldim + (size_t)o->row0;
+}
+// Device pointer to the view's top-left element, after syncing a dirty
+// host mirror to the device.
+int dev_ptr(PLA_Obj o, void **p);
+// After the library wrote the object on the device: refresh its mirror.
+int written(PLA_Obj o);
+int ld(PLA_Obj o);
+
+// Block size the solvers use for an n x n problem.
+int choose_nb(int n, int op);
+
+// Device scalar (1x1 object) read back to the host.
+int scalar_value(PLA_Obj o, double *v);
+
+// ---- exchange slots of the panel-LU leaf kernel (allocated once, zeroed) ----
+struct leaf_ws {
+  void *base = nullptr;
+  int capacity = 0;         // workgroups the slots were sized for
+  double *row = nullptr;    // [2][G][32]
+  double *val = nullptr;    // [2][G]
+  int *idx = nullptr;       // [2][G]
+  double *rowj = nullptr;   // [2][32]
+  unsigned *flags = nullptr;// [G]
+  int *err = nullptr;       // timeout flag
+};
+leaf_ws &leaf_workspace(int G);
+// Returns PLA_ERR_DEVICE if any panel exchange timed out since the last call.
+int leaf_check_error(hipStream_t s);
+
+// ---- kernel launchers (pla_kernels.hip); all asynchronous on stream s ----
+int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, double alpha,
+           const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
+           const int *abort_flag);
+int k_syrk_lower(hipStream_t s, int role, int n, int k, double alpha, const double *X, long ldx,
+                 double beta, double *C, long ldc, const int *abort_flag);
+int k_laswp(hipStream_t s, double *A, long lda, int ncols, int np, const int *piv);
+int k_trsm_leaf(hipStream_t s, bool lower, bool trans, bool unit, int b, int ncols, const double *T,
+                long ldt, double *B, long ldb, const int *abort_flag);
+int k_trsm_rltn_leaf(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
+                     const int *abort_flag);
+int k_potrf_leaf(hipStream_t s, int b, double *A, long lda, int offset, int *info);
+int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D, long ldd,
+             const int *abort_flag);
+int k_scale2d(hipStream_t s, int m, int n, double alpha, double *D, long ldd);
+int k_fill_drand48(hipStream_t s, int m, int n, double *D, long ldd, uint64_t state0, uint64_t skip);
+int k_lu_leaf(hipStream_t s, double *P, int mp, int w, long lda, int *piv);
+int leaf_max_width();
+int trsm_leaf_size();
+
+// ---- blocked/recursive building blocks (pla_blas.cpp) ----
+// B <- op(T)^{-1} B, T n x n triangular, B n x nrhs.
+int trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *T,
+              long ldt, double *B, long ldb, const int *abort_flag);
+// X <- X inv(L)^T, X nrows x b, L b x b lower non-unit.
+int trsm_rltn(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
+              const int *abort_flag);
+// Recursive panel LU of an mp x w panel (mp >= w); pivots relative to each row.
+int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv);
+// Recursive lower Cholesky of a b x b block; failures go to *info (1-based,
+// offset added).
+int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info);
+
+// ---- kernel timing (hipEvents on the library stream, per role) ----
+// role 1: trailing update of PLA_LU (GEMM); role 2: trailing update of PLA_Chol (SYRK)
+bool timing_on();
+void timing_begin(hipStream_t s, int role);
+void timing_end(hipStream_t s, int role, double flops);
+
+}  // namespace pla

@@ -1,0 +1,719 @@
+// pla_kernels.hip -- HIP kernels for gfx950 (AMD Instinct MI355X) behind the
+// PLAPACK dense factor-and-solve path.
+//
+// Kernels and the reference routine each one replaces:
+//   dgemm_kernel      PLA_Local_gemm (Local_BLAS/PLA_Local_gemm.c:102) and the
+//                     dgemm/dsyrk calls of PLA_Syrk_perform_local_part
+//                     (BLAS3/PLA_Syrk_perform_local_part.c:54,133,136):
+//                     FP64 MFMA (v_mfma_f64_16x16x4_f64), 128x128 block
+//                     tile, LDS-staged 16-deep K panels, double buffered.
+//   lu_leaf_kernel    PLA_LU_right_mv (SOLVERS/LU/PLA_LU_mv.c:52-82):
+//                     unblocked panel LU of up to 32 columns; PLA_Iamax
+//                     (BLAS1/PLA_Iamax.c:37-124) becomes a workgroup
+//                     reduction plus a cross-workgroup exchange of
+//                     (value, index, row) candidates, Pivot_rows_mv
+//                     (PLA_Apply_pivots_new.c:208-305) a row exchange in
+//                     LDS, PLA_Local_inv_scal (PLA_Local_BLAS1.c:258) a
+//                     reciprocal-multiply, PLA_Local_ger a rank-1 update.
+//   laswp_kernel      PLA_Apply_pivots_to_rows (PLA_Apply_pivots_new.c:13-179)
+//   trsm_leaf_kernel  PLA_Local_trsm / dtrsm (Local_BLAS/PLA_Local_trsm.c:67),
+//                     left side, one thread per right-hand side column.
+//   trsm_rltn_leaf    dtrsm("R","L","T","N") of PLA_Chol.c:193-195, one
+//                     thread per row.
+//   potrf_leaf_kernel PLA_dchol (SOLVERS/Cholesky/PLA_Local_chol.c:103-143)
+//   fill_drand48      TEST/create_problem.c:31-52 (drand48()*2-1, column-major)
+//
+// All matrices are column-major double with a 64-bit leading dimension.
+#include "pla_internal.hpp"
+
+#include <climits>
+#include <cstdint>
+
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// DGEMM: C = beta*C + alpha*op(A)*op(B), m x n result, inner dimension k.
+// Block tile 128 x 128, 4 waves in a 2 x 2 arrangement, 64 x 64 per wave as a
+// 4 x 4 grid of 16 x 16 MFMA tiles.  K is staged through LDS 16 at a time.
+// MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16];
+// result lane l holds D[4*(l/16)+r][l%16], r = 0..3.
+// LOWER: only the lower triangle (row >= col) of C is read or written and
+// tiles entirely above the diagonal exit at once (SYRK).
+// ROLE only separates kernel symbols (the trailing update is ROLE 1) so that
+// profiles can attribute time to the LU/Cholesky trailing update.
+constexpr int GBM = 128, GBN = 128, GBK = 16;
+constexpr int APAD = 4;   // As row stride GBM + APAD doubles
+constexpr int BPAD = 1;   // Bs row stride GBK + BPAD doubles
+
+template <bool TA, bool TB, bool LOWER, int ROLE>
+__global__ __launch_bounds__(256, 2) void dgemm_kernel(
+    int m, int n, int k, double alpha, const double *__restrict__ A, long lda,
+    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n, const int *__restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  const int ntiles = tiles_m * tiles_n;
+  // Consecutive workgroup ids are dispatched round-robin over the 8 XCDs;
+  // give each XCD a contiguous range of tiles so its L2 sees neighbours.
+  const int nblk = gridDim.x;  // a multiple of 8
+  int bid = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+  if (bid >= ntiles) return;
+  constexpr int GROUP = 8;  // tile rows per group: A rows are reused inside a group
+  const int group = bid / (GROUP * tiles_n);
+  const int first_m = group * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int within = bid - group * GROUP * tiles_n;
+  const int tm = first_m + within % gsz;
+  const int tn = within / gsz;
+  const int i0 = tm * GBM, j0 = tn * GBN;
+  if (LOWER && i0 + GBM <= j0) return;
+
+  __shared__ double As[2][GBK][GBM + APAD];
+  __shared__ double Bs[2][GBN][GBK + BPAD];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lq = lane >> 4;
+
+  dbl4 acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = i0 + wm * 64 + bi * 16 + lq * 4 + r;
+        const int gj = j0 + wn * 64 + bj * 16 + l16;
+        double v = 0.0;
+        if (beta != 0.0 && gi < m && gj < n && (!LOWER || gi >= gj))
+          v = beta * C[(long)gj * ldc + gi];
+        acc[bi][bj][r] = v;
+      }
+
+  double ra[8], rb[8];
+  auto load_tile = [&](int kt) {
+    const int kb = kt * GBK;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int idx = tid + 256 * e;
+      int i, kk;
+      if (!TA) { i = idx & 127; kk = idx >> 7; }
+      else { kk = idx & 15; i = idx >> 4; }
+      const int gi = i0 + i, gk = kb + kk;
+      double v = 0.0;
+      if (gi < m && gk < k) v = TA ? A[(long)gi * lda + gk] : A[(long)gk * lda + gi];
+      ra[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int idx = tid + 256 * e;
+      int j, kk;
+      if (!TB) { kk = idx & 15; j = idx >> 4; }
+      else { j = idx & 127; kk = idx >> 7; }
+      const int gj = j0 + j, gk = kb + kk;
+      double v = 0.0;
+      if (gj < n && gk < k) v = TB ? B[(long)gk * ldb + gj] : B[(long)gj * ldb + gk];
+      rb[e] = v;
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int idx = tid + 256 * e;
+      int i, kk;
+      if (!TA) { i = idx & 127; kk = idx >> 7; }
+      else { kk = idx & 15; i = idx >> 4; }
+      As[buf][kk][i] = alpha * ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int idx = tid + 256 * e;
+      int j, kk;
+      if (!TB) { kk = idx & 15; j = idx >> 4; }
+      else { j = idx & 127; kk = idx >> 7; }
+      Bs[buf][j][kk] = rb[e];
+    }
+  };
+
+  const int nk = (k + GBK - 1) / GBK;
+  if (nk > 0) {
+    load_tile(0);
+    store_tile(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tile(kt + 1);
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++) {
+      double af[4], bf[4];
+#pragma unroll
+      for (int bi = 0; bi < 4; bi++) af[bi] = As[cur][kq * 4 + lq][wm * 64 + bi * 16 + l16];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) bf[bj] = Bs[cur][wn * 64 + bj * 16 + l16][kq * 4 + lq];
+#pragma unroll
+      for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store_tile(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = i0 + wm * 64 + bi * 16 + lq * 4 + r;
+        const int gj = j0 + wn * 64 + bj * 16 + l16;
+        if (gi < m && gj < n && (!LOWER || gi >= gj)) C[(long)gj * ldc + gi] = acc[bi][bj][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Unblocked panel LU of a tall panel P (mp rows, w <= 32 columns), one
+// persistent grid of G workgroups; workgroup g keeps rows [g*R, g*R+R) in LDS.
+// Per column: local (|value|, index) maximum, publication of the candidate
+// (value, index, whole row) and of the current row j, one cross-workgroup
+// exchange, then every workgroup picks the same winner (largest |value|,
+// ties to the smallest index, as PLA_Iamax.c:108-115), swaps rows in LDS,
+// scales by the reciprocal of the pivot and applies the rank-1 update.
+//
+// Exchange protocol: wave 0 of each workgroup writes its slot with
+// device-scope stores and then releases flags[g] = tag0 + j; readers poll
+// all G flags (bounded spin), acquire, and read slots with device-scope
+// loads.  Slots are double buffered by column parity; one flag per
+// workgroup suffices because no workgroup can run two columns ahead.
+constexpr int LEAF_W = 32;
+constexpr int LEAF_THREADS = 256;
+
+struct leaf_slots {
+  double *row;      // [2][G][LEAF_W] candidate rows
+  double *val;      // [2][G] signed candidate values
+  int *idx;         // [2][G] candidate global row index (INT_MAX: none)
+  double *rowj;     // [2][LEAF_W] row j before the swap
+  unsigned *flags;  // [G] publication tags
+  int *err;         // set to 1 if a wait timed out
+};
+
+__device__ inline void argmax_combine(double &bv, int &bi, double v, int i) {
+  if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
+}
+
+// Block-wide (|v|, idx) argmax; result valid in every thread.
+__device__ inline void block_argmax(double &bv, int &bi, double *red_v, int *red_i) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    double ov = __shfl_xor(bv, off, 64);
+    int oi = __shfl_xor(bi, off, 64);
+    argmax_combine(bv, bi, ov, oi);
+  }
+  if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+  __syncthreads();
+  bv = red_v[0];
+  bi = red_i[0];
+  for (int w = 1; w < LEAF_THREADS / 64; w++) argmax_combine(bv, bi, red_v[w], red_i[w]);
+  __syncthreads();
+}
+
+__device__ inline void st_dev(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_dev(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double ld_dev(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_dev(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(
+    double *__restrict__ P, int mp, int w, long lda, int *__restrict__ piv, int R,
+    leaf_slots s, unsigned tag0, unsigned spin_limit) {
+  extern __shared__ double lds[];
+  double *L = lds;                        // L[c*R + r]
+  double *prow = lds + (size_t)w * R;     // pivot row (new row j)
+  double *orow = prow + LEAF_W;           // old row j
+  __shared__ double red_v[LEAF_THREADS / 64];
+  __shared__ int red_i[LEAF_THREADS / 64];
+
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int r0 = g * R;
+  const int rows = max(0, min(mp - r0, R));
+
+  for (int e = tid; e < rows * w; e += LEAF_THREADS) {
+    const int c = e / rows, r = e - c * rows;
+    L[c * R + r] = P[(long)c * lda + r0 + r];
+  }
+  __syncthreads();
+
+  for (int j = 0; j < w; j++) {
+    const int par = j & 1;
+    const unsigned tag = tag0 + (unsigned)j;
+    // local candidate
+    double bv = -1.0;
+    int bi = INT_MAX;
+    for (int r = tid; r < rows; r += LEAF_THREADS)
+      if (r0 + r >= j) argmax_combine(bv, bi, fabs(L[j * R + r]), r0 + r);
+    block_argmax(bv, bi, red_v, red_i);
+
+    // publish (wave 0 only, then its lane 0 releases the flag)
+    if (tid < 64) {
+      const bool have = (bi != INT_MAX);
+      if (tid < w && have) st_dev(&s.row[((size_t)par * G + g) * LEAF_W + tid], L[tid * R + (bi - r0)]);
+      if (j >= r0 && j < r0 + rows && tid >= 32 && tid - 32 < w)
+        st_dev(&s.rowj[par * LEAF_W + (tid - 32)], L[(tid - 32) * R + (j - r0)]);
+      if (tid == 0) {
+        st_dev(&s.val[par * G + g], have ? L[j * R + (bi - r0)] : 0.0);
+        st_dev(&s.idx[par * G + g], bi);
+        __hip_atomic_store(&s.flags[g], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    // wait for every workgroup's publication of column j
+    if (tid < G) {
+      unsigned it = 0;
+      while (__hip_atomic_load(&s.flags[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tag) {
+        if (++it > spin_limit) {
+          __hip_atomic_store(s.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    __syncthreads();
+
+    // global winner (identical in every workgroup)
+    double gv = -1.0;
+    int gi = INT_MAX, gw = -1;
+    for (int t = tid; t < G; t += LEAF_THREADS) {
+      const int ci = ld_dev(&s.idx[par * G + t]);
+      if (ci != INT_MAX) argmax_combine(gv, gi, fabs(ld_dev(&s.val[par * G + t])), ci);
+    }
+    block_argmax(gv, gi, red_v, red_i);
+    gw = (gi != INT_MAX) ? gi / R : -1;  // workgroup owning the winning row
+    const int p = (gi != INT_MAX) ? gi : j;  // all-NaN column: no interchange
+    if (tid < w) {
+      orow[tid] = ld_dev(&s.rowj[par * LEAF_W + tid]);
+      prow[tid] = (gw >= 0) ? ld_dev(&s.row[((size_t)par * G + gw) * LEAF_W + tid]) : orow[tid];
+    }
+    __syncthreads();
+    if (p != j && tid < w) {
+      if (p >= r0 && p < r0 + rows) L[tid * R + (p - r0)] = orow[tid];
+      if (j >= r0 && j < r0 + rows) L[tid * R + (j - r0)] = prow[tid];
+    }
+    __syncthreads();
+    const double inv = 1.0 / prow[j];
+    for (int r = tid; r < rows; r += LEAF_THREADS) {
+      if (r0 + r > j) {
+        const double l = L[j * R + r] * inv;
+        L[j * R + r] = l;
+        for (int c = j + 1; c < w; c++) L[c * R + r] = fma(l, -prow[c], L[c * R + r]);
+      }
+    }
+    if (g == 0 && tid == 0) piv[j] = p - j;
+    __syncthreads();
+  }
+
+  for (int e = tid; e < rows * w; e += LEAF_THREADS) {
+    const int c = e / rows, r = e - c * rows;
+    P[(long)c * lda + r0 + r] = L[c * R + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward sequential row interchanges (LAPACK dlaswp semantics): for
+// i = 0..np-1, swap rows i and i + piv[i] of columns [0, ncols).  One thread
+// per column; the pivots are staged in LDS 1024 at a time.
+__global__ __launch_bounds__(256) void laswp_kernel(double *__restrict__ A, long lda, int ncols,
+                                                    int np, const int *__restrict__ piv) {
+  __shared__ int sp[1024];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  double *col = A + (long)c * lda;
+  for (int base = 0; base < np; base += 1024) {
+    const int cnt = min(1024, np - base);
+    __syncthreads();
+    for (int t = threadIdx.x; t < cnt; t += 256) sp[t] = piv[base + t];
+    __syncthreads();
+    if (c < ncols) {
+      for (int t = 0; t < cnt; t++) {
+        const int p = sp[t];
+        if (p != 0) {
+          const int r = base + t;
+          const double a = col[r], b = col[r + p];
+          col[r] = b;
+          col[r + p] = a;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Left triangular solve of a b x b block (b <= 64), one thread per column of
+// B: B <- op(T)^{-1} B.  The triangle is staged in LDS.  Operation order per
+// element follows the netlib dtrsm loops (column axpys for NoTrans, dot
+// products for Trans; a divide by the diagonal for NonUnit).
+constexpr int TRSM_LEAF = 64;
+
+template <bool LOWER, bool TRANS, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_leaf_kernel(int b, int ncols, const double *__restrict__ T,
+                                                        long ldt, double *__restrict__ B, long ldb,
+                                                        const int *__restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ double sT[TRSM_LEAF][TRSM_LEAF + 1];  // sT[i][k] = T(i,k)
+  for (int e = threadIdx.x; e < b * b; e += 256) {
+    const int kcol = e / b, i = e - kcol * b;
+    sT[i][kcol] = T[(long)kcol * ldt + i];
+  }
+  __syncthreads();
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncols) return;
+  double *x = B + (long)c * ldb;
+  double v[TRSM_LEAF];
+#pragma unroll
+  for (int i = 0; i < TRSM_LEAF; i++) v[i] = (i < b) ? x[i] : 0.0;
+  // effective matrix M = op(T): lower-type (forward) when LOWER != TRANS
+  constexpr bool FORWARD = (LOWER != TRANS);
+  if (FORWARD) {
+#pragma unroll
+    for (int i = 0; i < TRSM_LEAF; i++) {
+      if (i < b) {
+        double t = v[i];
+#pragma unroll
+        for (int kk = 0; kk < i; kk++) t = fma(-(TRANS ? sT[kk][i] : sT[i][kk]), v[kk], t);
+        if (!UNIT) t = t / sT[i][i];
+        v[i] = t;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int ii = TRSM_LEAF - 1; ii >= 0; ii--) {
+      if (ii < b) {
+        double t = v[ii];
+#pragma unroll
+        for (int kk = TRSM_LEAF - 1; kk > ii; kk--)
+          if (kk < b) t = fma(-(TRANS ? sT[kk][ii] : sT[ii][kk]), v[kk], t);
+        if (!UNIT) t = t / sT[ii][ii];
+        v[ii] = t;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TRSM_LEAF; i++)
+    if (i < b) x[i] = v[i];
+}
+
+// X <- X * inv(L^T), X: nrows x b (b <= 64), L lower b x b non-unit.
+// One thread per row of X; netlib dtrsm("R","L","T","N") order: scale column
+// k by 1/L(k,k), then subtract L(j,k) * X(:,k) from the later columns j.
+__global__ __launch_bounds__(256) void trsm_rltn_leaf_kernel(int nrows, int b, const double *__restrict__ Lm,
+                                                            long ldl, double *__restrict__ X, long ldx,
+                                                            const int *__restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ double sL[TRSM_LEAF][TRSM_LEAF + 1];
+  for (int e = threadIdx.x; e < b * b; e += 256) {
+    const int kcol = e / b, i = e - kcol * b;
+    sL[i][kcol] = Lm[(long)kcol * ldl + i];
+  }
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nrows) return;
+  double v[TRSM_LEAF];
+#pragma unroll
+  for (int c = 0; c < TRSM_LEAF; c++) v[c] = (c < b) ? X[(long)c * ldx + r] : 0.0;
+#pragma unroll
+  for (int kk = 0; kk < TRSM_LEAF; kk++) {
+    if (kk < b) {
+      v[kk] *= 1.0 / sL[kk][kk];
+#pragma unroll
+      for (int j = kk + 1; j < TRSM_LEAF; j++)
+        if (j < b) v[j] = fma(-sL[j][kk], v[kk], v[j]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < TRSM_LEAF; c++)
+    if (c < b) X[(long)c * ldx + r] = v[c];
+}
+
+// ---------------------------------------------------------------------------
+// Unblocked lower Cholesky of a b x b diagonal block (b <= 64) in one
+// workgroup: PLA_dchol's left-looking order (dgemv update of column k, test
+// a(k,k) <= 0, sqrt, scale by the reciprocal).  On failure *info receives
+// offset + k + 1 (1-based global index) and the block is left partially
+// factored; the caller restores it from its backup.
+__global__ __launch_bounds__(64) void potrf_leaf_kernel(int b, double *__restrict__ Ablk, long lda,
+                                                        int offset, int *__restrict__ info) {
+  if (*info) return;
+  __shared__ double S[TRSM_LEAF][TRSM_LEAF + 1];  // S[i][j] = A(i,j), lower part used
+  __shared__ int fail;
+  const int i = threadIdx.x;
+  for (int j = 0; j < b; j++)
+    if (i < b && i >= j) S[i][j] = Ablk[(long)j * lda + i];
+  if (i == 0) fail = 0;
+  __syncthreads();
+  for (int kk = 0; kk < b; kk++) {
+    if (i < b && i >= kk) {
+      double y = S[i][kk];
+      for (int j = 0; j < kk; j++) y = fma(-S[kk][j], S[i][j], y);
+      S[i][kk] = y;
+    }
+    __syncthreads();
+    const double d = S[kk][kk];
+    if (d <= 0.0) {
+      if (i == 0) fail = kk + 1;
+      break;
+    }
+    const double sq = sqrt(d);
+    const double invert = 1.0 / sq;
+    __syncthreads();
+    if (i == kk) S[kk][kk] = sq;
+    if (i < b && i > kk) S[i][kk] *= invert;
+    __syncthreads();
+  }
+  __syncthreads();
+  if (fail) {
+    if (i == 0) atomicCAS(info, 0, offset + fail);
+    return;
+  }
+  for (int j = 0; j < b; j++)
+    if (i < b && i >= j) Ablk[(long)j * lda + i] = S[i][j];
+}
+
+// ---------------------------------------------------------------------------
+// 2-D copy (m x n, column-major), optionally skipped after a failure.
+__global__ __launch_bounds__(256) void copy2d_kernel(int m, int n, const double *__restrict__ S, long lds_,
+                                                     double *__restrict__ D, long ldd,
+                                                     const int *__restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int j = (int)(e / m), i = (int)(e - (long)j * m);
+    D[(long)j * ldd + i] = S[(long)j * lds_ + i];
+  }
+}
+
+// Scale an m x n block by alpha.
+__global__ __launch_bounds__(256) void scale2d_kernel(int m, int n, double alpha, double *__restrict__ D,
+                                                      long ldd) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int j = (int)(e / m), i = (int)(e - (long)j * m);
+    D[(long)j * ldd + i] *= alpha;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// drand48 fill: element (i, j) takes draw number skip + j*m + i after
+// srand48(seed); value = X * 2^-48 * 2 - 1 (exact in double).  Each thread
+// jumps ahead once (affine LCG map squared log2 times) and then steps
+// through a run of 64 consecutive draws.
+constexpr uint64_t LCG_A = 0x5DEECE66DULL, LCG_C = 0xBULL, LCG_MASK = (1ULL << 48) - 1;
+
+__device__ inline uint64_t lcg_skip(uint64_t x, uint64_t k) {
+  uint64_t mul = LCG_A, add = LCG_C, am = 1, aa = 0;
+  while (k) {
+    if (k & 1) {
+      am = (am * mul) & LCG_MASK;
+      aa = (aa * mul + add) & LCG_MASK;
+    }
+    add = ((mul + 1) * add) & LCG_MASK;
+    mul = (mul * mul) & LCG_MASK;
+    k >>= 1;
+  }
+  return (am * x + aa) & LCG_MASK;
+}
+
+__global__ __launch_bounds__(256) void fill_drand48_kernel(int m, int n, double *__restrict__ D, long ldd,
+                                                           uint64_t state0, uint64_t skip) {
+  const uint64_t total = (uint64_t)m * (uint64_t)n;
+  const uint64_t run = 64;
+  const uint64_t start = ((uint64_t)blockIdx.x * 256 + threadIdx.x) * run;
+  if (start >= total) return;
+  uint64_t x = lcg_skip(state0, skip + start);
+  const uint64_t end = start + run < total ? start + run : total;
+  for (uint64_t e = start; e < end; e++) {
+    x = (LCG_A * x + LCG_C) & LCG_MASK;
+    const double v = ldexp((double)x, -48) * 2.0 - 1.0;
+    const uint64_t j = e / (uint64_t)m, i = e - j * (uint64_t)m;
+    D[(long)j * ldd + (long)i] = v;
+  }
+}
+
+inline unsigned grid1d(long total, int per_block) {
+  long g = (total + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 65535L * 64) g = 65535L * 64;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+namespace pla {
+
+template <bool TA, bool TB, bool LOWER, int ROLE>
+static void launch_gemm(hipStream_t s, int m, int n, int k, double alpha, const double *A, long lda,
+                        const double *B, long ldb, double beta, double *C, long ldc,
+                        const int *abort_flag) {
+  const int tm = (m + GBM - 1) / GBM, tn = (n + GBN - 1) / GBN;
+  const int nblk = ((tm * tn + 7) / 8) * 8;
+  hipLaunchKernelGGL((dgemm_kernel<TA, TB, LOWER, ROLE>), dim3(nblk), dim3(256), 0, s, m, n, k, alpha,
+                     A, lda, B, ldb, beta, C, ldc, tm, tn, abort_flag);
+}
+
+int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, double alpha,
+           const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
+           const int *abort_flag) {
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  if ((long)((m + GBM - 1) / GBM) * ((n + GBN - 1) / GBN) > (1L << 31) - 8)
+    return fail(PLA_ERR_DIMENSIONS, "gemm: too many tiles");
+  if (role == 1 && !ta && !tb)
+    launch_gemm<false, false, false, 1>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  else if (!ta && !tb)
+    launch_gemm<false, false, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  else if (!ta && tb)
+    launch_gemm<false, true, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  else if (ta && !tb)
+    launch_gemm<true, false, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  else
+    launch_gemm<true, true, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_syrk_lower(hipStream_t s, int role, int n, int k, double alpha, const double *X, long ldx,
+                 double beta, double *C, long ldc, const int *abort_flag) {
+  if (n <= 0) return PLA_SUCCESS;
+  if (role == 1)
+    launch_gemm<false, true, true, 1>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, abort_flag);
+  else
+    launch_gemm<false, true, true, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, abort_flag);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_laswp(hipStream_t s, double *A, long lda, int ncols, int np, const int *piv) {
+  if (ncols <= 0 || np <= 0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(laswp_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, A, lda, ncols, np, piv);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_trsm_leaf(hipStream_t s, bool lower, bool trans, bool unit, int b, int ncols, const double *T,
+                long ldt, double *B, long ldb, const int *abort_flag) {
+  if (b <= 0 || ncols <= 0) return PLA_SUCCESS;
+  if (b > TRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "trsm leaf: b = %d > %d", b, TRSM_LEAF);
+  dim3 grid((ncols + 255) / 256), blk(256);
+#define PLA_TRSM_CASE(LO, TR, UN)                                                                     \
+  if (lower == LO && trans == TR && unit == UN) {                                                     \
+    hipLaunchKernelGGL((trsm_leaf_kernel<LO, TR, UN>), grid, blk, 0, s, b, ncols, T, ldt, B, ldb,     \
+                       abort_flag);                                                                   \
+  }
+  PLA_TRSM_CASE(true, false, true)
+  PLA_TRSM_CASE(true, false, false)
+  PLA_TRSM_CASE(true, true, true)
+  PLA_TRSM_CASE(true, true, false)
+  PLA_TRSM_CASE(false, false, true)
+  PLA_TRSM_CASE(false, false, false)
+  PLA_TRSM_CASE(false, true, true)
+  PLA_TRSM_CASE(false, true, false)
+#undef PLA_TRSM_CASE
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_trsm_rltn_leaf(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
+                     const int *abort_flag) {
+  if (nrows <= 0 || b <= 0) return PLA_SUCCESS;
+  if (b > TRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "trsm_rltn leaf: b = %d", b);
+  hipLaunchKernelGGL(trsm_rltn_leaf_kernel, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, b, L,
+                     ldl, X, ldx, abort_flag);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_potrf_leaf(hipStream_t s, int b, double *A, long lda, int offset, int *info) {
+  if (b <= 0) return PLA_SUCCESS;
+  if (b > TRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "potrf leaf: b = %d", b);
+  hipLaunchKernelGGL(potrf_leaf_kernel, dim3(1), dim3(64), 0, s, b, A, lda, offset, info);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D, long ldd,
+             const int *abort_flag) {
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(copy2d_kernel, dim3(grid1d((long)m * n, 256 * 4)), dim3(256), 0, s, m, n, S, lds_,
+                     D, ldd, abort_flag);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_scale2d(hipStream_t s, int m, int n, double alpha, double *D, long ldd) {
+  if (m <= 0 || n <= 0 || alpha == 1.0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(scale2d_kernel, dim3(grid1d((long)m * n, 256 * 4)), dim3(256), 0, s, m, n, alpha,
+                     D, ldd);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_fill_drand48(hipStream_t s, int m, int n, double *D, long ldd, uint64_t state0, uint64_t skip) {
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  const uint64_t total = (uint64_t)m * n;
+  const uint64_t threads = (total + 63) / 64;
+  const uint64_t blocks = (threads + 255) / 256;
+  if (blocks > 0x7fffffffULL) return fail(PLA_ERR_DIMENSIONS, "fill: too large");
+  hipLaunchKernelGGL(fill_drand48_kernel, dim3((unsigned)blocks), dim3(256), 0, s, m, n, D, ldd, state0,
+                     skip);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+// Panel leaf: chooses the grid, stages the exchange slots in a workspace and
+// checks the timeout flag lazily (see lu_leaf_error).
+static unsigned g_leaf_tag = 1;
+
+int k_lu_leaf(hipStream_t s, double *P, int mp, int w, long lda, int *piv) {
+  if (mp <= 0 || w <= 0) return PLA_SUCCESS;
+  if (w > LEAF_W) return fail(PLA_ERR_INVALID_PARAM, "lu leaf: w = %d > %d", w, LEAF_W);
+  int maxG = ctx().cu_count > 0 ? ctx().cu_count : 256;
+  if (maxG > LEAF_THREADS) maxG = LEAF_THREADS;  // one polling thread per workgroup
+  int G = (mp + 127) / 128;
+  if (G > maxG) G = maxG;
+  if (G < 1) G = 1;
+  int R = (mp + G - 1) / G;
+  const size_t lds_bytes = ((size_t)w * R + 2 * LEAF_W) * sizeof(double);
+  if (lds_bytes > 150 * 1024)
+    return fail(PLA_ERR_DIMENSIONS, "lu leaf: panel of %d rows exceeds the LDS budget", mp);
+  leaf_ws &ws = leaf_workspace(maxG);
+  if (!ws.base) return PLA_ERR_NO_MEMORY;
+  leaf_slots sl;
+  sl.row = ws.row;
+  sl.val = ws.val;
+  sl.idx = ws.idx;
+  sl.rowj = ws.rowj;
+  sl.flags = ws.flags;
+  sl.err = ws.err;
+  if (g_leaf_tag > 0xF0000000u) {  // tags only grow; restart far below wrap-around
+    PLA_HIP(hipMemsetAsync(ws.flags, 0, sizeof(unsigned) * maxG, s));
+    g_leaf_tag = 1;
+  }
+  const unsigned tag0 = g_leaf_tag;
+  g_leaf_tag += (unsigned)w;
+  hipLaunchKernelGGL(lu_leaf_kernel, dim3(G), dim3(LEAF_THREADS), lds_bytes, s, P, mp, w, lda, piv, R,
+                     sl, tag0, 200000u);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int leaf_max_width() { return LEAF_W; }
+int trsm_leaf_size() { return TRSM_LEAF; }
+
+}  // namespace pla

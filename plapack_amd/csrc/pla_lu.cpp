@@ -1,0 +1,105 @@
+// pla_lu.cpp -- PLA_LU and PLA_General_solve on a 1x1 template, device resident.
+//
+// Reference: SOLVERS/LU/PLA_LU.c:13-212 (blocked right-looking LU with
+// partial pivoting), SOLVERS/LU/PLA_General.c:67-119 (factor + solve).
+//
+// Per block step k of width b (PLA_LU.c:103-192):
+//   1. lu_panel on A[k:m, k:k+b]          (PLA_LU_right_mv, :144)
+//   2. row interchanges on the columns left and right of the panel
+//                                           (PLA_Apply_pivots_to_rows, :152)
+//   3. A12 <- inv(L11) A12                  (PLA_Local_trsm, :174-182)
+//   4. A22 <- A22 - A21 A12                 (PLA_Local_gemm, :185-186)
+// The panel copies PLA_Copy makes between distributions (:115-118, :160-165)
+// vanish: every operand is a view into the one HBM buffer.
+#include "pla_internal.hpp"
+
+using namespace pla;
+
+static int check_lu_args(PLA_Obj A, PLA_Obj pivots) {
+  // SOLVERS/LU/PLA_LU_enter_exit.c:17-105
+  if (!valid(A)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_LU: invalid A");
+  if (A->base->datatype != MPI_DOUBLE)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_LU: only MPI_DOUBLE matrices are supported");
+  if (A->objtype != PLA_MATRIX) return fail(PLA_ERR_INVALID_PARAM, "PLA_LU: A must be a PLA_MATRIX");
+  if (!valid(pivots)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_LU: invalid pivots");
+  if (pivots->base->datatype != MPI_INT)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_LU: pivots must be of type MPI_INT");
+  const int mn = A->length < A->width ? A->length : A->width;
+  if (pivots->length != mn || pivots->width != 1)
+    return fail(PLA_ERR_DIMENSIONS, "PLA_LU: pivots must be %d x 1, got %d x %d", mn, pivots->length,
+                pivots->width);
+  return PLA_SUCCESS;
+}
+
+namespace pla {
+
+// Blocked right-looking LU of the m x n column-major device matrix a.
+int lu_factor(hipStream_t s, int m, int n, double *a, long lda, int *piv, int nb) {
+  const int mn = m < n ? m : n;
+  int rc;
+  for (int k = 0; k < mn; k += nb) {
+    const int b = mn - k < nb ? mn - k : nb;
+    double *Akk = a + (long)k * lda + k;
+    if ((rc = lu_panel(s, Akk, m - k, b, lda, piv + k))) return rc;
+    if ((rc = k_laswp(s, a + k, lda, k, b, piv + k))) return rc;
+    const int nr = n - k - b;
+    if (nr > 0) {
+      double *A12 = a + (long)(k + b) * lda + k;
+      if ((rc = k_laswp(s, A12, lda, nr, b, piv + k))) return rc;
+      if ((rc = trsm_left(s, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
+      const int mr = m - k - b;
+      if (mr > 0) {
+        timing_begin(s, 1);
+        rc = k_gemm(s, 1, false, false, mr, nr, b, -1.0, Akk + b, lda, A12, lda, 1.0, A12 + b, lda,
+                    nullptr);
+        timing_end(s, 1, 2.0 * mr * (double)nr * b);
+        if (rc) return rc;
+      }
+    }
+  }
+  return leaf_check_error(s);
+}
+
+}  // namespace pla
+
+extern "C" {
+
+int PLA_LU(PLA_Obj A, PLA_Obj pivots) {
+  int rc = check_lu_args(A, pivots);
+  if (rc) return rc;
+  const int m = A->length, n = A->width;
+  const int mn = m < n ? m : n;
+  if (mn == 0) return PLA_SUCCESS;
+  double *a;
+  int *piv;
+  if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(pivots, (void **)&piv))) return rc;
+  const int nb = choose_nb(mn, PLA_OP_PAN_PAN);
+  if ((rc = lu_factor(ctx().stream, m, n, a, ld(A), piv, nb))) return rc;
+  if ((rc = written(A))) return rc;
+  return written(pivots);
+}
+
+// Factor A (PLA_LU), then solve A X = B in place: interchanges on B, then the
+// unit-lower and upper triangular solves (PLA_General.c:80-111).
+int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
+  int rc = PLA_LU(A, ipiv);
+  if (rc) return rc;
+  if (!valid(B) || B->base->datatype != MPI_DOUBLE)
+    return fail(PLA_ERR_INVALID_OBJECT, "PLA_General_solve: invalid B");
+  const int n = A->length;
+  if (A->width != n || B->length != n)
+    return fail(PLA_ERR_DIMENSIONS, "PLA_General_solve: A %d x %d, B %d x %d", A->length, A->width,
+                B->length, B->width);
+  double *a, *b;
+  int *piv;
+  if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(B, (void **)&b)) ||
+      (rc = dev_ptr(ipiv, (void **)&piv)))
+    return rc;
+  hipStream_t s = ctx().stream;
+  if ((rc = k_laswp(s, b, ld(B), B->width, n, piv))) return rc;
+  if ((rc = trsm_left(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  if ((rc = trsm_left(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  return written(B);
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+"""Pins the CPU oracle (oracle/pla_oracle.c) against checks that do not share
+its code: glibc's drand48, the committed golden vectors (LAPACK pivots of the
+reference's n=2048 seed-0 input, TEST/input_test_chol's case), LAPACK/numpy
+factorizations and solves, exact rational elimination on tiny inputs, and the
+algebraic invariants each routine must satisfy.  CPU only.
+"""
+import ctypes
+import ctypes.util
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------- generator
+
+def test_drand48_matches_glibc(ora):
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    for seed in (0, 1, 1793, 123456789):
+        libc.srand48(seed)
+        ref = np.array([libc.drand48() for _ in range(2000)])
+        got = ora.drand48_stream(seed, 2000)
+        assert np.array_equal(ref, got)
+
+
+def test_lcg_skip_equals_stepping(ora):
+    for skip in (0, 1, 2, 63, 64, 1000, 4097):
+        assert np.array_equal(ora.drand48_stream(7, 5, skip=skip),
+                              ora.drand48_stream(7, skip + 5)[skip:])
+
+
+def test_fill_matches_golden_head(ora):
+    head = np.load(os.path.join(GOLDEN, "a_n2048_seed0_head.npy"))
+    a = ora.uniform(2048, 2048, seed=0)
+    assert np.array_equal(a.ravel(order="F")[:16], head)
+    # the values quoted for TEST/create_problem.c with seed 0
+    assert a[0, 0] == -0.6583439277874206
+    assert a[1, 0] == 0.49980396096992763
+
+
+def test_fill_skip_continues_the_stream(ora):
+    a = ora.uniform(7, 5, seed=3)
+    b = ora.uniform(7, 3, seed=3, skip=7 * 2)
+    assert np.array_equal(a[:, 2:], b)
+
+
+# ---------------------------------------------------------------- idamax
+
+def test_idamax_first_max(ora):
+    x = np.array([1.0, -3.0, 2.0, 3.0, -3.0])
+    assert ora.idamax(x) == 1  # ties resolve to the first index
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        x = rng.standard_normal(rng.integers(1, 100))
+        assert ora.idamax(x) == int(np.argmax(np.abs(x)))
+
+
+# ---------------------------------------------------------------- LU
+
+def _lapack_piv(a):
+    _, ipiv = scipy.linalg.lu_factor(a, check_finite=False)
+    return (ipiv - np.arange(len(ipiv))).astype(np.int32)
+
+
+def test_lu_pivots_golden_n2048(ora):
+    """n=2048, nb=128, seed 0 -- pivots equal LAPACK's."""
+    golden = np.load(os.path.join(GOLDEN, "lu_piv_n2048_seed0.npy"))
+    a = ora.uniform(2048, 2048, 0)
+    lu, piv = ora.lu_blocked(a, nb=128)
+    assert np.array_equal(piv, golden)
+    assert ora.lu_scaled_residual(a, lu, piv) < 16
+
+
+@pytest.mark.parametrize("m,n,nb", [(50, 50, 8), (64, 40, 16), (40, 64, 16), (129, 129, 32),
+                                    (300, 300, 128), (1, 1, 4), (5, 1, 4), (1, 5, 4)])
+def test_lu_blocked_vs_lapack(ora, m, n, nb):
+    a = ora.uniform(m, n, seed=m * 31 + n)
+    lu, piv = ora.lu_blocked(a, nb=nb)
+    assert np.array_equal(piv, _lapack_piv(a))
+    assert ora.lu_scaled_residual(a, lu, piv) < 16
+    lu2, piv2 = ora.lu_unblocked(a)
+    assert np.array_equal(piv, piv2)
+    # reference's own sequential criterion (PLA_LU_enter_exit.c:130-153)
+    assert np.abs(lu - lu2).max() <= 1e-7 * np.abs(a).max() * max(m, n)
+
+
+def _exact_lu_pivots(a):
+    """Partial pivoting in exact rational arithmetic (first max on ties)."""
+    A = [[Fraction(float(v)) for v in row] for row in np.asarray(a)]
+    m, n = len(A), len(A[0])
+    piv = []
+    for j in range(min(m, n)):
+        col = [abs(A[i][j]) for i in range(j, m)]
+        p = col.index(max(col))
+        piv.append(p)
+        A[j], A[j + p] = A[j + p], A[j]
+        if A[j][j] != 0:
+            for i in range(j + 1, m):
+                f = A[i][j] / A[j][j]
+                A[i][j] = f
+                for c in range(j + 1, n):
+                    A[i][c] -= f * A[j][c]
+    return np.array(piv, dtype=np.int32)
+
+
+def test_lu_exact_small_with_ties(ora):
+    # integer matrices with deliberate |value| ties: first index wins
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        n = int(rng.integers(2, 9))
+        a = rng.integers(-3, 4, size=(n, n)).astype(np.float64)
+        a[rng.integers(0, n), 0] = 3.0
+        a[rng.integers(0, n), 0] = -3.0
+        if abs(np.linalg.det(a)) < 0.5:
+            continue
+        _, piv = ora.lu_blocked(a, nb=2)
+        assert np.array_equal(piv, _exact_lu_pivots(a))
+
+
+def test_lu_planted_error_is_caught(ora):
+    """A dropped row interchange must fail the residual check."""
+    a = ora.uniform(64, 64, 9)
+    lu, piv = ora.lu_blocked(a, nb=16)
+    bad = piv.copy()
+    j = int(np.flatnonzero(bad)[0])
+    bad[j] = 0
+    assert ora.lu_scaled_residual(a, lu, bad) > 1e3
+
+
+# ---------------------------------------------------------------- laswp
+
+def test_laswp_is_sequential_interchange(ora):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((20, 7))
+    piv = np.array([3, 0, 5, 1, 0, 2], dtype=np.int32)
+    got = ora.laswp(a, piv, r0=2)
+    ref = a.copy()
+    for i, p in enumerate(piv):
+        r = 2 + i
+        ref[[r, r + p]] = ref[[r + p, r]]
+    assert np.array_equal(got, ref)
+
+
+# ---------------------------------------------------------------- Cholesky
+
+def _spd(n, seed, delta=1e-2):
+    rng = np.random.default_rng(seed)
+    b = rng.uniform(-1, 1, (n, n))
+    return b @ b.T / n + delta * np.eye(n)
+
+
+@pytest.mark.parametrize("n,nb", [(1, 4), (7, 4), (64, 16), (200, 64), (257, 128)])
+def test_chol_vs_numpy(ora, n, nb):
+    a = _spd(n, n)
+    l, info = ora.chol_blocked(a, nb=nb)
+    assert info == 0
+    ref = np.linalg.cholesky(a)
+    assert np.allclose(np.tril(l), ref, rtol=0, atol=1e-12 * n)
+    assert np.array_equal(np.triu(l, 1), np.triu(a, 1))  # upper triangle untouched
+
+
+def test_chol_input_test_chol_fixture(ora):
+    """TEST/input_test_chol double case: n=20, nb_alg=4, A = create_problem + n I."""
+    g = np.load(os.path.join(GOLDEN, "chol_n20_shift.npz"))
+    a = ora.uniform(20, 20, 0) + 20 * np.eye(20)
+    assert np.array_equal(a, g["a"])
+    l, info = ora.chol_blocked(a, nb=int(g["nb"]))
+    assert info == 0
+    # reference criterion: max|diff| <= 1e-6 max|A| (PLA_Chol_enter_exit.c:115-132)
+    assert np.abs(np.tril(l) - g["l"]).max() <= 1e-6 * np.abs(a).max()
+
+
+def test_chol_failure_index_matches_leading_minors(ora):
+    n = 40
+    a = _spd(n, 3)
+    a[25, 25] -= 50.0  # the 26th leading minor becomes negative first
+    dets = [np.linalg.slogdet(a[:k, :k])[0] for k in range(1, n + 1)]
+    first_bad = next(k for k, s in enumerate(dets, start=1) if s <= 0)
+    _, info = ora.chol_blocked(a, nb=8)
+    assert info == first_bad
+    _, info_u = ora.chol_unblocked(a)
+    assert info_u == first_bad
+
+
+# ---------------------------------------------------------------- solves / BLAS
+
+@pytest.mark.parametrize("uplo,trans,diag", [(u, t, d) for u in "LU" for t in "NT" for d in "UN"])
+def test_trsm_left_vs_scipy(ora, uplo, trans, diag):
+    rng = np.random.default_rng(2)
+    n = 37
+    t = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    b = rng.standard_normal((n, 3))
+    x = ora.trsm_left(uplo, trans, diag, t, b, alpha=2.0)
+    ref = scipy.linalg.solve_triangular(t, 2.0 * b, lower=(uplo == "L"), trans=trans,
+                                        unit_diagonal=(diag == "U"))
+    assert np.allclose(x, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_gemm_vs_numpy(ora, ta, tb):
+    rng = np.random.default_rng(4)
+    m, n, k = 13, 17, 9
+    a = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    b = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    c = rng.standard_normal((m, n))
+    got = ora.gemm(ta, tb, -1.5, a, b, 0.5, c)
+    ref = -1.5 * (a if ta == "N" else a.T) @ (b if tb == "N" else b.T) + 0.5 * c
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_lu_and_chol_solve(ora):
+    a = ora.uniform(100, 100, 11)
+    b = a @ np.ones(100)
+    lu, piv = ora.lu_blocked(a, nb=32)
+    x = ora.lu_solve(lu, piv, b)
+    assert np.abs(x[:, 0] - 1).max() < 1e-10
+    s = _spd(80, 2)
+    l, info = ora.chol_blocked(s, nb=16)
+    x = ora.chol_solve(np.tril(l), s @ np.ones(80))
+    assert info == 0 and np.abs(x[:, 0] - 1).max() < 1e-10
