@@ -257,8 +257,9 @@ int PLA_Obj_device_buffer(PLA_Obj obj, void **device_buffer);
 int PLA_Obj_fill_drand48(PLA_Obj obj, long seed, int64_t skip);
 /* Wait for all device work issued by the library. */
 int PLA_Device_synchronize(void);
-/* Block size the device LU/Cholesky use when pla_Environ_set_nb_alg has not
-   been called (0 restores the built-in choice). */
+/* Block size of the device LU/Cholesky.  A value > 0 takes precedence over
+   pla_Environ_set_nb_alg; 0 restores the order: the PLA_OP_PAN_PAN entry if
+   pla_Environ_set_nb_alg was called, else the built-in choice for n. */
 int PLA_Set_device_nb(int nb);
 int PLA_Get_device_nb(int n, int *nb);
 /* Per-launch hipEvent timing of the trailing-update kernels on the library

@@ -80,14 +80,14 @@ int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv) {
   int rc;
   if ((rc = lu_panel(s, P, mp, w0, lda, piv))) return rc;
   double *Pr = P + (long)w0 * lda;  // right columns
-  if ((rc = k_laswp(s, Pr, lda, w1, w0, piv))) return rc;
+  if ((rc = k_laswp(s, Pr, lda, mp, w1, w0, piv))) return rc;
   if ((rc = trsm_left(s, true, false, true, w0, w1, P, lda, Pr, lda, nullptr))) return rc;
   if ((rc = k_gemm(s, 0, false, false, mp - w0, w1, w0, -1.0, P + w0, lda, Pr, lda, 1.0, Pr + w0, lda,
                    nullptr)))
     return rc;
   if ((rc = lu_panel(s, Pr + w0, mp - w0, w1, lda, piv + w0))) return rc;
   // the right half's interchanges also apply to the left half's rows below w0
-  return k_laswp(s, P + w0, lda, w0, w1, piv + w0);
+  return k_laswp(s, P + w0, lda, mp - w0, w0, w1, piv + w0);
 }
 
 int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info) {
@@ -202,7 +202,15 @@ int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots) {
   double *a;
   int *p;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(pivots, (void **)&p))) return rc;
-  if ((rc = k_laswp(ctx().stream, a, ld(A), A->width, np, p))) return rc;
+  // validate the interchanges on the host: row i + piv[i] must lie in A
+  std::vector<int> hp(np);
+  PLA_HIP(hipStreamSynchronize(ctx().stream));
+  PLA_HIP(hipMemcpy(hp.data(), p, sizeof(int) * np, hipMemcpyDeviceToHost));
+  for (int i = 0; i < np; i++)
+    if (hp[i] < 0 || i + hp[i] >= A->length)
+      return fail(PLA_ERR_INVALID_PARAM, "PLA_Apply_pivots_to_rows: pivot %d (= %d) leaves the matrix", i,
+                  hp[i]);
+  if ((rc = k_laswp(ctx().stream, a, ld(A), A->length, A->width, np, p))) return rc;
   return written(A);
 }
 

@@ -150,8 +150,8 @@ void release_workspaces() {
 
 int choose_nb(int n, int op) {
   const context &c = g_ctx;
-  if (c.nb_user > 0) return c.nb_user;
   if (c.nb_device > 0) return c.nb_device;
+  if (c.nb_user > 0) return c.nb_user;
   (void)op;
   if (n >= 8192) return 256;
   return 128;

@@ -111,7 +111,9 @@ int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, doubl
            const int *abort_flag);
 int k_syrk_lower(hipStream_t s, int role, int n, int k, double alpha, const double *X, long ldx,
                  double beta, double *C, long ldc, const int *abort_flag);
-int k_laswp(hipStream_t s, double *A, long lda, int ncols, int np, const int *piv);
+// rows i and i + piv[i] (i < np) of an nrows x ncols block; entries that point
+// outside the block are ignored
+int k_laswp(hipStream_t s, double *A, long lda, int nrows, int ncols, int np, const int *piv);
 int k_trsm_leaf(hipStream_t s, bool lower, bool trans, bool unit, int b, int ncols, const double *T,
                 long ldt, double *B, long ldb, const int *abort_flag);
 int k_trsm_rltn_leaf(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,

@@ -247,6 +247,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int r0 = g * R;
   const int rows = max(0, min(mp - r0, R));
+  // a timed-out exchange earlier on this stream: leave the panel untouched
+  if (__hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
 
   for (int e = tid; e < rows * w; e += LEAF_THREADS) {
     const int c = e / rows, r = e - c * rows;
@@ -280,7 +282,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(
     if (tid < G) {
       unsigned it = 0;
       while (__hip_atomic_load(&s.flags[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tag) {
-        if (++it > spin_limit) {
+        if (++it > spin_limit ||
+            ((it & 255) == 0 && __hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
           __hip_atomic_store(s.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
@@ -295,7 +298,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(
     int gi = INT_MAX, gw = -1;
     for (int t = tid; t < G; t += LEAF_THREADS) {
       const int ci = ld_dev(&s.idx[par * G + t]);
-      if (ci != INT_MAX) argmax_combine(gv, gi, fabs(ld_dev(&s.val[par * G + t])), ci);
+      if (ci >= j && ci < mp) argmax_combine(gv, gi, fabs(ld_dev(&s.val[par * G + t])), ci);
     }
     block_argmax(gv, gi, red_v, red_i);
     gw = (gi != INT_MAX) ? gi / R : -1;  // workgroup owning the winning row
@@ -332,8 +335,8 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(
 // Forward sequential row interchanges (LAPACK dlaswp semantics): for
 // i = 0..np-1, swap rows i and i + piv[i] of columns [0, ncols).  One thread
 // per column; the pivots are staged in LDS 1024 at a time.
-__global__ __launch_bounds__(256) void laswp_kernel(double *__restrict__ A, long lda, int ncols,
-                                                    int np, const int *__restrict__ piv) {
+__global__ __launch_bounds__(256) void laswp_kernel(double *__restrict__ A, long lda, int nrows,
+                                                    int ncols, int np, const int *__restrict__ piv) {
   __shared__ int sp[1024];
   const int c = blockIdx.x * 256 + threadIdx.x;
   double *col = A + (long)c * lda;
@@ -345,8 +348,8 @@ __global__ __launch_bounds__(256) void laswp_kernel(double *__restrict__ A, long
     if (c < ncols) {
       for (int t = 0; t < cnt; t++) {
         const int p = sp[t];
-        if (p != 0) {
-          const int r = base + t;
+        const int r = base + t;
+        if (p > 0 && r + p < nrows) {  // out-of-range entries are ignored
           const double a = col[r], b = col[r + p];
           col[r] = b;
           col[r + p] = a;
@@ -599,9 +602,10 @@ int k_syrk_lower(hipStream_t s, int role, int n, int k, double alpha, const doub
   return PLA_SUCCESS;
 }
 
-int k_laswp(hipStream_t s, double *A, long lda, int ncols, int np, const int *piv) {
+int k_laswp(hipStream_t s, double *A, long lda, int nrows, int ncols, int np, const int *piv) {
   if (ncols <= 0 || np <= 0) return PLA_SUCCESS;
-  hipLaunchKernelGGL(laswp_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, A, lda, ncols, np, piv);
+  hipLaunchKernelGGL(laswp_kernel, dim3((ncols + 255) / 256), dim3(256), 0, s, A, lda, nrows, ncols, np,
+                     piv);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }

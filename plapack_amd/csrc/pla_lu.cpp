@@ -41,11 +41,11 @@ int lu_factor(hipStream_t s, int m, int n, double *a, long lda, int *piv, int nb
     const int b = mn - k < nb ? mn - k : nb;
     double *Akk = a + (long)k * lda + k;
     if ((rc = lu_panel(s, Akk, m - k, b, lda, piv + k))) return rc;
-    if ((rc = k_laswp(s, a + k, lda, k, b, piv + k))) return rc;
+    if ((rc = k_laswp(s, a + k, lda, m - k, k, b, piv + k))) return rc;
     const int nr = n - k - b;
     if (nr > 0) {
       double *A12 = a + (long)(k + b) * lda + k;
-      if ((rc = k_laswp(s, A12, lda, nr, b, piv + k))) return rc;
+      if ((rc = k_laswp(s, A12, lda, m - k, nr, b, piv + k))) return rc;
       if ((rc = trsm_left(s, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
       const int mr = m - k - b;
       if (mr > 0) {
@@ -96,7 +96,7 @@ int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
       (rc = dev_ptr(ipiv, (void **)&piv)))
     return rc;
   hipStream_t s = ctx().stream;
-  if ((rc = k_laswp(s, b, ld(B), B->width, n, piv))) return rc;
+  if ((rc = k_laswp(s, b, ld(B), n, B->width, n, piv))) return rc;
   if ((rc = trsm_left(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
   if ((rc = trsm_left(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
   return written(B);

@@ -54,7 +54,7 @@ def _compile_c(src, name, cc, extra=()):
     with open(c, "w") as f:
         f.write(src)
     subprocess.run(list(cc) + ["-I", os.path.join(ROOT, "include"), c, "-o", exe] + list(extra),
-                   check=True)
+                   check=True, env=dict(os.environ, MPICH_CC="gcc"))
     return exe
 
 
@@ -77,13 +77,12 @@ def test_reference_style_driver_compiles_against_header():
         pytest.skip("mpicc not available")
     drv = os.path.join(ROOT, "tests", "drivers", "test_lu_driver.c")
     subprocess.run([mpicc, "-fsyntax-only", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                    drv], check=True)
+                    drv], check=True, env=dict(os.environ, MPICH_CC="gcc"))
 
 
 def test_no_gpu_gives_error_not_crash():
-    import torch  # noqa: F401  (only to ask whether a GPU is visible)
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is visible")
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present")
     # run in a child so this process never initialises HIP
     code = ("from plapack_amd import pla\n"
             "rc = pla.lib().PLA_Init(pla.MPI_COMM_WORLD)\n"

@@ -1,0 +1,165 @@
+"""GPU parity of the building blocks beneath PLA_LU / PLA_Chol, called through
+the C ABI: the drand48 fill (bit-exact against the oracle), PLA_Gemm (FP64
+MFMA kernel, every transpose case and ragged edges), PLA_Trsm / PLA_Trsv,
+PLA_Apply_pivots_to_rows (bit-exact), views and copies.
+
+Tolerances: FP64 results agree with the oracle to c * k * eps * scale, the
+usual forward-error bound of a k-term dot product (c = 4)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from plapack_amd import pla
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def _obj(s, arr):
+    return pla.from_numpy(s, np.asfortranarray(arr))
+
+
+def test_fill_drand48_bit_exact(pla_session, ora):
+    s = pla_session
+    A = s.matrix(300, 211)
+    pla.call("PLA_Obj_fill_drand48", A, 0, 0)
+    assert np.array_equal(pla.get_contents(A), ora.uniform(300, 211, 0))
+    pla.call("PLA_Obj_fill_drand48", A, 5, 12345)
+    assert np.array_equal(pla.get_contents(A), ora.uniform(300, 211, 5, 12345))
+    pla.free(A)
+
+
+def test_fill_drand48_far_into_stream(pla_session, ora):
+    s = pla_session
+    n = 8192
+    A = s.matrix(n, n)
+    pla.call("PLA_Obj_fill_drand48", A, 0, 0)
+    left, right = ctypes.c_void_p(), ctypes.c_void_p()
+    pla.call("PLA_Obj_vert_split_2", A, -3, ctypes.byref(left), ctypes.byref(right))
+    assert pla.dims(right) == (n, 3)
+    got = pla.get_contents(right)
+    assert np.array_equal(got, ora.uniform(n, 3, 0, skip=n * (n - 3)))
+    pla.free(left, right, A)
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (False, True), (True, False), (True, True)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 4), (130, 257, 33), (256, 128, 256),
+                                   (77, 5, 300), (300, 1, 129)])
+def test_gemm_vs_oracle(pla_session, ora, ta, tb, m, n, k):
+    s = pla_session
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    a = rng.uniform(-1, 1, (k, m) if ta else (m, k))
+    b = rng.uniform(-1, 1, (n, k) if tb else (k, n))
+    c = rng.uniform(-1, 1, (m, n))
+    A, B, C = _obj(s, a), _obj(s, b), _obj(s, c)
+    al, be = s.scalar(-1.0), s.scalar(0.5)
+    pla.call("PLA_Gemm", pla.PLA_TRANSPOSE if ta else pla.PLA_NO_TRANSPOSE,
+             pla.PLA_TRANSPOSE if tb else pla.PLA_NO_TRANSPOSE, al, A, B, be, C)
+    got = pla.get_contents(C)
+    ref = ora.gemm("T" if ta else "N", "T" if tb else "N", -1.0, a, b, 0.5, c)
+    assert np.abs(got - ref).max() <= 8 * k * EPS * max(1.0, np.abs(ref).max())
+    pla.free(A, B, C, al, be)
+
+
+def test_gemm_views_leave_outside_untouched(pla_session, ora):
+    """Trailing-update shape on views: C = A22 inside a larger matrix."""
+    s = pla_session
+    rng = np.random.default_rng(1)
+    full = rng.uniform(-1, 1, (400, 380))
+    F = _obj(s, full)
+    # A21 = F[100:, 0:64], A12 = F[0:64, 100:], A22 = F[100:, 100:]
+    tl, tr, bl, br = (ctypes.c_void_p() for _ in range(4))
+    pla.call("PLA_Obj_split_4", F, 100, 100, ctypes.byref(tl), ctypes.byref(tr), ctypes.byref(bl),
+             ctypes.byref(br))
+    a21, x1, a12, x2 = (ctypes.c_void_p() for _ in range(4))
+    pla.call("PLA_Obj_vert_split_2", bl, 64, ctypes.byref(a21), ctypes.byref(x1))
+    pla.call("PLA_Obj_horz_split_2", tr, 64, ctypes.byref(a12), ctypes.byref(x2))
+    one, mone = s.scalar(1.0), s.scalar(-1.0)
+    pla.call("PLA_Gemm", pla.PLA_NO_TRANSPOSE, pla.PLA_NO_TRANSPOSE, mone, a21, a12, one, br)
+    got = pla.get_contents(F)
+    ref = full.copy()
+    ref[100:, 100:] -= full[100:, :64] @ full[:64, 100:]
+    assert np.abs(got - ref).max() <= 4 * 64 * EPS * 2
+    assert np.array_equal(got[:100, :], full[:100, :])
+    assert np.array_equal(got[:, :100], full[:, :100])
+    pla.free(tl, tr, bl, br, a21, x1, a12, x2, one, mone, F)
+
+
+@pytest.mark.parametrize("uplo", [pla.PLA_LOWER_TRIANGULAR, pla.PLA_UPPER_TRIANGULAR])
+@pytest.mark.parametrize("trans", [pla.PLA_NO_TRANSPOSE, pla.PLA_TRANSPOSE])
+@pytest.mark.parametrize("diag", [pla.PLA_UNIT_DIAG, pla.PLA_NONUNIT_DIAG])
+@pytest.mark.parametrize("n,nrhs", [(50, 3), (200, 70), (300, 1)])
+def test_trsm_left_vs_oracle(pla_session, ora, uplo, trans, diag, n, nrhs):
+    s = pla_session
+    rng = np.random.default_rng(n + nrhs)
+    t = rng.uniform(-1, 1, (n, n)) / n + np.eye(n)
+    b = rng.uniform(-1, 1, (n, nrhs))
+    T, B, al = _obj(s, t), _obj(s, b), s.scalar(2.0)
+    pla.call("PLA_Trsm", pla.PLA_SIDE_LEFT, uplo, trans, diag, al, T, B)
+    got = pla.get_contents(B)
+    ref = ora.trsm_left("L" if uplo == pla.PLA_LOWER_TRIANGULAR else "U",
+                        "N" if trans == pla.PLA_NO_TRANSPOSE else "T",
+                        "U" if diag == pla.PLA_UNIT_DIAG else "N", t, b, alpha=2.0)
+    assert np.abs(got - ref).max() <= 64 * n * EPS * max(1.0, np.abs(ref).max())
+    pla.free(T, B, al)
+
+
+def test_trsm_right_lower_trans_vs_oracle(pla_session, ora):
+    s = pla_session
+    rng = np.random.default_rng(3)
+    n, m = 150, 90
+    l = np.tril(rng.uniform(-1, 1, (n, n))) / n + np.eye(n)
+    x = rng.uniform(-1, 1, (m, n))
+    L, X, al = _obj(s, l), _obj(s, x), s.scalar(1.0)
+    pla.call("PLA_Trsm", pla.PLA_SIDE_RIGHT, pla.PLA_LOWER_TRIANGULAR, pla.PLA_TRANSPOSE,
+             pla.PLA_NONUNIT_DIAG, al, L, X)
+    got = pla.get_contents(X)
+    ref = ora.trsm_left("L", "N", "N", l, x.T).T  # X L^T = B  <=>  L X^T = B^T
+    assert np.abs(got - ref).max() <= 64 * n * EPS
+    pla.free(L, X, al)
+
+
+def test_trsv_vs_oracle(pla_session, ora):
+    s = pla_session
+    rng = np.random.default_rng(4)
+    n = 333
+    t = rng.uniform(-1, 1, (n, n)) / n + np.eye(n)
+    x = rng.uniform(-1, 1, n)
+    T = _obj(s, t)
+    X = s.mvector(n)
+    pla.set_contents(X, x)
+    pla.call("PLA_Trsv", pla.PLA_UPPER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG, T, X)
+    got = pla.get_contents(X)[:, 0]
+    ref = ora.trsm_left("U", "N", "N", t, x)[:, 0]
+    assert np.abs(got - ref).max() <= 64 * n * EPS
+    pla.free(T, X)
+
+
+@pytest.mark.parametrize("m,ncols,npiv", [(50, 7, 20), (3000, 513, 1500), (10, 1, 10)])
+def test_apply_pivots_bit_exact(pla_session, ora, m, ncols, npiv):
+    s = pla_session
+    rng = np.random.default_rng(m)
+    a = rng.standard_normal((m, ncols))
+    piv = np.array([rng.integers(0, m - i) for i in range(npiv)], dtype=np.int32)
+    A = _obj(s, a)
+    P = s.mvector(npiv, dtype=pla.MPI_INT)
+    pla.set_contents(P, piv)
+    pla.call("PLA_Apply_pivots_to_rows", A, P)
+    assert np.array_equal(pla.get_contents(A), ora.laswp(a, piv))
+    pla.free(A, P)
+
+
+def test_copy_and_constants(pla_session):
+    s = pla_session
+    a = np.arange(12.0).reshape(3, 4)
+    A = _obj(s, a)
+    B = s.matrix(3, 4)
+    pla.call("PLA_Copy", A, B)
+    assert np.array_equal(pla.get_contents(B), a)
+    mone, zero, one = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    pla.call("PLA_Create_constants_conf_to", A, ctypes.byref(mone), ctypes.byref(zero),
+             ctypes.byref(one))
+    assert pla.get_contents(mone)[0, 0] == -1.0 and pla.get_contents(one)[0, 0] == 1.0
+    assert pla.get_contents(zero)[0, 0] == 0.0
+    pla.free(A, B, mone, zero, one)

@@ -1,0 +1,167 @@
+"""GPU parity of PLA_LU (SOLVERS/LU/PLA_LU.c:13) and PLA_General_solve
+(SOLVERS/LU/PLA_General.c:67) through the C ABI.
+
+Bar: pivot vectors equal (integers, bit-exact) to the oracle's and, for the
+reference's own n=2048 seed-0 case, to the committed LAPACK golden vector;
+factors within the reference's sequential-check tolerance 1e-7 * max|A|
+(SOLVERS/LU/PLA_LU_enter_exit.c:130-153) of the oracle; scaled residual
+||PA - LU||_1 / (n ||A||_1 eps) < 16.  At n=16384 (too large for the CPU
+oracle) the size-independent checks are the solve of A x = A*1 and its
+normwise backward error."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from plapack_amd import pla
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _lu_gpu(s, a=None, n=None, m=None, seed=0, nb=0):
+    """Run PLA_LU on a (host array) or on the drand48 seed matrix of size m x n."""
+    pla.call("PLA_Set_device_nb", nb)
+    if a is not None:
+        A = pla.from_numpy(s, a)
+        m, n = a.shape
+    else:
+        m = m or n
+        A = s.matrix(m, n)
+        pla.call("PLA_Obj_fill_drand48", A, seed, 0)
+    P = s.mvector(min(m, n), dtype=pla.MPI_INT)
+    pla.call("PLA_LU", A, P)
+    lu, piv = pla.get_contents(A), pla.get_contents(P)[:, 0]
+    pla.free(A, P)
+    pla.call("PLA_Set_device_nb", 0)
+    return lu, piv
+
+
+def test_lu_golden_n2048_seed0(pla_session, ora):
+    """The reference's CPU-runnable case: n=2048, nb=128, A from drand48 seed 0."""
+    golden = np.load(os.path.join(GOLDEN, "lu_piv_n2048_seed0.npy"))
+    lu, piv = _lu_gpu(pla_session, n=2048, seed=0, nb=128)
+    assert np.array_equal(piv, golden)
+    a = ora.uniform(2048, 2048, 0)
+    assert ora.lu_scaled_residual(a, lu, piv) < 16
+    lu_o, piv_o = ora.lu_blocked(a, nb=128)
+    assert np.array_equal(piv, piv_o)
+    assert np.abs(lu - lu_o).max() <= 1e-7 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("m,n,nb", [(1, 1, 128), (5, 5, 128), (100, 37, 32), (37, 100, 32),
+                                    (300, 300, 64), (1000, 1000, 128), (1000, 1000, 256),
+                                    (2049, 2049, 128), (1500, 700, 256), (700, 1500, 512)])
+def test_lu_shapes_vs_oracle(pla_session, ora, m, n, nb):
+    a = ora.uniform(m, n, seed=m + 3 * n)
+    lu, piv = _lu_gpu(pla_session, a=a, nb=nb)
+    lu_o, piv_o = ora.lu_blocked(a, nb=nb)
+    assert np.array_equal(piv, piv_o)
+    assert np.abs(lu - lu_o).max() <= 1e-7 * np.abs(a).max()
+    assert ora.lu_scaled_residual(a, lu, piv) < 16
+
+
+def test_lu_zero_column_not_flagged(pla_session, ora):
+    """Like the reference, a zero pivot is not detected: PLA_LU returns 0."""
+    a = ora.uniform(64, 64, 2)
+    a[:, 10] = 0.0
+    lu, piv = _lu_gpu(pla_session, a=a, nb=32)
+    _, piv_o = ora.lu_blocked(a, nb=32)
+    assert np.array_equal(piv[:11], piv_o[:11])
+    assert piv[10] == 0  # all-zero column: first index, no interchange
+
+
+def test_lu_argument_errors(pla_session):
+    s = pla_session
+    A = s.matrix(10, 10)
+    P = s.mvector(9, dtype=pla.MPI_INT)
+    rc = pla.lib().PLA_LU(A, P)
+    assert rc < 0 and "pivots" in pla.last_error()
+    Pd = s.mvector(10, dtype=pla.MPI_DOUBLE)
+    assert pla.lib().PLA_LU(A, Pd) < 0
+    pla.free(A, P, Pd)
+
+
+def test_general_solve_multi_rhs(pla_session, ora):
+    s = pla_session
+    n = 777
+    a = ora.uniform(n, n, 4)
+    x = np.random.default_rng(0).uniform(-1, 1, (n, 16))
+    b = a @ x
+    A, B = pla.from_numpy(s, a), pla.from_numpy(s, b)
+    P = s.mvector(n, dtype=pla.MPI_INT)
+    pla.call("PLA_General_solve", A, P, B)
+    got = pla.get_contents(B)
+    lu_o, piv_o = ora.lu_blocked(a, nb=128)
+    assert np.array_equal(pla.get_contents(P)[:, 0], piv_o)
+    assert np.abs(got - x).max() < 1e-9
+    pla.free(A, B, P)
+
+
+def test_lu_solve_n16384_backward_error(pla_session):
+    """Full-path property at a size the CPU oracle cannot finish quickly."""
+    s = pla_session
+    n = 16384
+    A = s.matrix(n, n)
+    A0 = s.matrix(n, n)
+    pla.call("PLA_Obj_fill_drand48", A, 0, 0)
+    pla.call("PLA_Obj_fill_drand48", A0, 0, 0)
+    ones, b, x = s.mvector(n), s.mvector(n), s.mvector(n)
+    pla.call("PLA_Obj_set_to_one", ones)
+    one, zero, mone = s.scalar(1.0), s.scalar(0.0), s.scalar(-1.0)
+    pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, one, A0, ones, zero, b)   # b = A*1
+    pla.call("PLA_Copy", b, x)
+    P = s.mvector(n, dtype=pla.MPI_INT)
+    pla.call("PLA_Set_device_nb", 256)
+    pla.call("PLA_LU", A, P)
+    pla.call("PLA_Set_device_nb", 0)
+    pla.call("PLA_Apply_pivots_to_rows", x, P)
+    pla.call("PLA_Trsv", pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_UNIT_DIAG, A, x)
+    pla.call("PLA_Trsv", pla.PLA_UPPER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG, A, x)
+    xs = pla.get_contents(x)[:, 0]
+    r = s.mvector(n)
+    pla.call("PLA_Copy", b, r)
+    pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, mone, A0, x, one, r)      # r = b - A x
+    rv, bv = pla.get_contents(r)[:, 0], pla.get_contents(b)[:, 0]
+    piv = pla.get_contents(P)[:, 0]
+    assert piv.min() >= 0 and (piv + np.arange(n)).max() < n
+    # ||A||_inf of uniform[-1,1] is about n/2; use the bound n for the scale
+    eta = np.abs(rv).max() / (n * np.abs(xs).max() + np.abs(bv).max())
+    assert eta < 1e-14
+    assert np.abs(xs - 1.0).max() < 1e-6
+    pla.free(A, A0, ones, b, x, r, P, one, zero, mone)
+
+
+def _driver_exe():
+    mpicc = shutil.which("mpicc") or "/opt/conda/bin/mpicc"
+    if not os.path.exists(mpicc):
+        pytest.skip("mpicc not available")
+    out = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(out, exist_ok=True)
+    exe = os.path.join(out, "test_lu_driver")
+    libdir = os.path.join(ROOT, "plapack_amd")
+    subprocess.run([mpicc, "-O2", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "drivers", "test_lu_driver.c"), "-o", exe,
+                    "-L", libdir, "-lpla_mi355x", "-Wl,-rpath," + libdir], check=True,
+                   env=dict(os.environ, MPICH_CC="gcc"))
+    return exe
+
+
+def test_reference_style_c_driver(tmp_path):
+    """A TEST/test_lu.c-style C program, host-filled through PLA_Obj_local_buffer."""
+    exe = _driver_exe()
+    pivfile = str(tmp_path / "piv.bin")
+    stdin = "1 1\n128\n128\n1\n1\n0\n1 2048\n-1\n"
+    out = subprocess.run([exe, pivfile], input=stdin, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "MFLOPS/node" in out.stdout
+    piv = np.fromfile(pivfile, dtype=np.int32)
+    assert np.array_equal(piv, np.load(os.path.join(GOLDEN, "lu_piv_n2048_seed0.npy")))
+    # a 2 x 1 mesh is refused with an error code
+    out = subprocess.run([exe], input="2 1\n128\n128\n1\n1\n0\n-1\n", capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 3 and "1x1" in out.stdout
