@@ -240,6 +240,10 @@ int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_
 /* BLAS3/PLA_Gemm.c:13 */
 int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta,
              PLA_Obj C);
+/* BLAS2/PLA_Matrix_infinity_norm.c:13, BLAS2/PLA_Matrix_one_norm.c:13 --
+   alpha (1x1 MPI_DOUBLE) <- max absolute row / column sum of A */
+int PLA_Matrix_infinity_norm(PLA_Obj A, PLA_Obj alpha);
+int PLA_Matrix_one_norm(PLA_Obj A, PLA_Obj alpha);
 /* SOLVERS/LU/PLA_General.c:67 -- factor A, then solve A X = B in place */
 int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B);
 /* SOLVERS/Cholesky/PLA_Pos_def.c:44 -- factor A = L L^T, then solve in place */

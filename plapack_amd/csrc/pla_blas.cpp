@@ -293,6 +293,26 @@ int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Ob
   return written(C);
 }
 
+static int matrix_norm(PLA_Obj A, PLA_Obj alpha, bool inf_norm, const char *who) {
+  int rc;
+  if ((rc = check_double(A, who)) || (rc = check_double(alpha, who))) return rc;
+  double *a, v = 0.0;
+  if ((rc = dev_ptr(A, (void **)&a))) return rc;
+  if ((rc = k_norm(ctx().stream, inf_norm, A->length, A->width, a, ld(A), &v))) return rc;
+  void *p;
+  if ((rc = dev_ptr(alpha, &p))) return rc;
+  PLA_HIP(hipMemcpy(p, &v, sizeof v, hipMemcpyHostToDevice));
+  return written(alpha);
+}
+
+int PLA_Matrix_infinity_norm(PLA_Obj A, PLA_Obj alpha) {
+  return matrix_norm(A, alpha, true, "PLA_Matrix_infinity_norm");
+}
+
+int PLA_Matrix_one_norm(PLA_Obj A, PLA_Obj alpha) {
+  return matrix_norm(A, alpha, false, "PLA_Matrix_one_norm");
+}
+
 int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y) {
   int rc;
   if ((rc = check_double(A, "PLA_Gemv")) || (rc = check_double(x, "PLA_Gemv")) ||

@@ -324,8 +324,7 @@ int PLA_Device_synchronize(void) {
 }
 
 int PLA_Set_device_nb(int nb) {
-  if (nb < 0 || (nb > 0 && (nb % 32) != 0))
-    return fail(PLA_ERR_INVALID_PARAM, "PLA_Set_device_nb: nb must be 0 or a multiple of 32");
+  if (nb < 0) return fail(PLA_ERR_INVALID_PARAM, "PLA_Set_device_nb: nb = %d", nb);
   pla::ctx().nb_device = nb;
   return PLA_SUCCESS;
 }

@@ -124,6 +124,8 @@ int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D,
 int k_scale2d(hipStream_t s, int m, int n, double alpha, double *D, long ldd);
 int k_fill_drand48(hipStream_t s, int m, int n, double *D, long ldd, uint64_t state0, uint64_t skip);
 int k_lu_leaf(hipStream_t s, double *P, int mp, int w, long lda, int *piv);
+// infinity norm (max absolute row sum) or one norm (max absolute column sum); synchronous
+int k_norm(hipStream_t s, bool inf_norm, int m, int n, const double *A, long lda, double *result);
 int leaf_max_width();
 int trsm_leaf_size();
 

@@ -38,7 +38,10 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // Block tile 128 x 128, 4 waves in a 2 x 2 arrangement, 64 x 64 per wave as a
 // 4 x 4 grid of 16 x 16 MFMA tiles.  K is staged through LDS 16 at a time.
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16];
-// result lane l holds D[4*(l/16)+r][l%16], r = 0..3.
+// result lane l, register r holds D[l/16 + 4r][l%16].  The kernel feeds the
+// B fragment as the A operand and vice versa, so D is the transposed C tile
+// and lane l holds C(l%16, l/16 + 4r): 16 consecutive lanes cover 16
+// consecutive rows of one column (128-byte coalesced loads and stores of C).
 // LOWER: only the lower triangle (row >= col) of C is read or written and
 // tiles entirely above the diagonal exit at once (SYRK).
 // ROLE only separates kernel symbols (the trailing update is ROLE 1) so that
@@ -83,8 +86,8 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
     for (int bj = 0; bj < 4; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gi = i0 + wm * 64 + bi * 16 + lq * 4 + r;
-        const int gj = j0 + wn * 64 + bj * 16 + l16;
+        const int gi = i0 + wm * 64 + bi * 16 + l16;
+        const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
         double v = 0.0;
         if (beta != 0.0 && gi < m && gj < n && (!LOWER || gi >= gj))
           v = beta * C[(long)gj * ldc + gi];
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
       for (int bi = 0; bi < 4; bi++)
 #pragma unroll
         for (int bj = 0; bj < 4; bj++)
-          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[bi], bf[bj], acc[bi][bj], 0, 0, 0);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[bj], af[bi], acc[bi][bj], 0, 0, 0);
     }
     if (kt + 1 < nk) store_tile(cur ^ 1);
     __syncthreads();
@@ -168,8 +171,8 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
     for (int bj = 0; bj < 4; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int gi = i0 + wm * 64 + bi * 16 + lq * 4 + r;
-        const int gj = j0 + wn * 64 + bj * 16 + l16;
+        const int gi = i0 + wm * 64 + bi * 16 + l16;
+        const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
         if (gi < m && gj < n && (!LOWER || gi >= gj)) C[(long)gj * ldc + gi] = acc[bi][bj][r];
       }
 }
@@ -514,6 +517,48 @@ __global__ __launch_bounds__(256) void scale2d_kernel(int m, int n, double alpha
 }
 
 // ---------------------------------------------------------------------------
+// Matrix norms (BLAS2/PLA_Matrix_infinity_norm.c:13, PLA_Matrix_one_norm.c:13):
+// absolute row (column) sums, then their maximum.  Row sums are formed in two
+// passes -- per 256-column chunk, then over chunks in order -- so the result is
+// deterministic; the maximum of non-negative doubles is an integer maximum of
+// their bit patterns.
+constexpr int NORM_CHUNK = 256;
+
+__global__ __launch_bounds__(256) void row_asum_partial_kernel(int m, int n, const double *__restrict__ A,
+                                                               long lda, double *__restrict__ part) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const int j0 = blockIdx.y * NORM_CHUNK, j1 = min(n, j0 + NORM_CHUNK);
+  double s = 0.0;
+  for (int j = j0; j < j1; j++) s += fabs(A[(long)j * lda + i]);
+  part[(long)blockIdx.y * m + i] = s;
+}
+
+__global__ __launch_bounds__(256) void row_asum_max_kernel(int m, int nch, const double *__restrict__ part,
+                                                           unsigned long long *__restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int c = 0; c < nch; c++) s += part[(long)c * m + i];
+  atomicMax(out, (unsigned long long)__double_as_longlong(s));
+}
+
+__global__ __launch_bounds__(256) void col_asum_max_kernel(int m, const double *__restrict__ A, long lda,
+                                                           unsigned long long *__restrict__ out) {
+  __shared__ double red[256];
+  const double *col = A + (long)blockIdx.x * lda;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < m; i += 256) s += fabs(col[i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// ---------------------------------------------------------------------------
 // drand48 fill: element (i, j) takes draw number skip + j*m + i after
 // srand48(seed); value = X * 2^-48 * 2 - 1 (exact in double).  Each thread
 // jumps ahead once (affine LCG map squared log2 times) and then steps
@@ -665,6 +710,31 @@ int k_scale2d(hipStream_t s, int m, int n, double alpha, double *D, long ldd) {
   hipLaunchKernelGGL(scale2d_kernel, dim3(grid1d((long)m * n, 256 * 4)), dim3(256), 0, s, m, n, alpha,
                      D, ldd);
   PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_norm(hipStream_t s, bool inf_norm, int m, int n, const double *A, long lda, double *result) {
+  *result = 0.0;
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  const int nch = (n + NORM_CHUNK - 1) / NORM_CHUNK;
+  const size_t part_bytes = inf_norm ? sizeof(double) * (size_t)nch * m : 0;
+  char *ws = (char *)workspace(part_bytes + 256, 2);
+  if (!ws) return PLA_ERR_NO_MEMORY;
+  unsigned long long *out = (unsigned long long *)(ws + part_bytes);
+  PLA_HIP(hipMemsetAsync(out, 0, sizeof(unsigned long long), s));
+  if (inf_norm) {
+    hipLaunchKernelGGL(row_asum_partial_kernel, dim3((m + 255) / 256, nch), dim3(256), 0, s, m, n, A, lda,
+                       (double *)ws);
+    hipLaunchKernelGGL(row_asum_max_kernel, dim3((m + 255) / 256), dim3(256), 0, s, m, nch,
+                       (const double *)ws, out);
+  } else {
+    hipLaunchKernelGGL(col_asum_max_kernel, dim3(n), dim3(256), 0, s, m, A, lda, out);
+  }
+  PLA_HIP(hipGetLastError());
+  unsigned long long bits = 0;
+  PLA_HIP(hipMemcpyAsync(&bits, out, sizeof bits, hipMemcpyDeviceToHost, s));
+  PLA_HIP(hipStreamSynchronize(s));
+  *result = __builtin_bit_cast(double, bits);
   return PLA_SUCCESS;
 }
 

@@ -105,6 +105,8 @@ _SIGS = {
     "PLA_Trsm": [_i, _i, _i, _i, _obj, _obj, _obj],
     "PLA_Gemv": [_i, _obj, _obj, _obj, _obj, _obj],
     "PLA_Gemm": [_i, _i, _obj, _obj, _obj, _obj, _obj],
+    "PLA_Matrix_infinity_norm": [_obj, _obj],
+    "PLA_Matrix_one_norm": [_obj, _obj],
     "PLA_General_solve": [_obj, _obj, _obj],
     "PLA_Pos_def_solve": [_obj, _obj],
     "PLA_Last_error": [],

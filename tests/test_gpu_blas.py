@@ -163,3 +163,15 @@ def test_copy_and_constants(pla_session):
     assert pla.get_contents(mone)[0, 0] == -1.0 and pla.get_contents(one)[0, 0] == 1.0
     assert pla.get_contents(zero)[0, 0] == 0.0
     pla.free(A, B, mone, zero, one)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (300, 257), (1000, 3), (3, 1000)])
+def test_matrix_norms_vs_numpy(pla_session, m, n):
+    s = pla_session
+    a = np.random.default_rng(m * n).uniform(-1, 1, (m, n))
+    A, nrm = _obj(s, a), s.scalar(0.0)
+    pla.call("PLA_Matrix_infinity_norm", A, nrm)
+    assert abs(pla.get_contents(nrm)[0, 0] - np.abs(a).sum(axis=1).max()) <= 4 * n * EPS * n
+    pla.call("PLA_Matrix_one_norm", A, nrm)
+    assert abs(pla.get_contents(nrm)[0, 0] - np.abs(a).sum(axis=0).max()) <= 4 * m * EPS * m
+    pla.free(A, nrm)
