@@ -265,6 +265,13 @@ int PLA_Device_synchronize(void);
    pla_Environ_set_nb_alg; 0 restores the order: the PLA_OP_PAN_PAN entry if
    pla_Environ_set_nb_alg was called, else the built-in choice for n. */
 int PLA_Set_device_nb(int nb);
+/* Tuning parameters of the device factorizations (an autotuner sets them).
+   PLA_PARAM_LOOKAHEAD: 1 (default) factors panel k+1 on a second stream while
+   step k's trailing update runs; 0 runs everything on one stream.
+   PLA_PARAM_LEAF_ROWS: target rows per workgroup in the panel-LU leaf. */
+#define PLA_PARAM_LOOKAHEAD 1
+#define PLA_PARAM_LEAF_ROWS 2
+int PLA_Set_device_param(int param, int value);
 int PLA_Get_device_nb(int n, int *nb);
 /* Per-launch hipEvent timing of the trailing-update kernels on the library
    stream (the reference keeps per-operation timers in PLA_TIMINGS,

@@ -72,22 +72,52 @@ int trsm_rltn(hipStream_t s, int nrows, int b, const double *L, long ldl, double
   return trsm_rltn(s, nrows, b1, L + (long)b0 * ldl + b0, ldl, X + (long)b0 * ldx, ldx, abort_flag);
 }
 
-int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv) {
-  if (w <= 0) return PLA_SUCCESS;
+struct panel_rec {
+  double *panel;  // panel top-left
+  long lda;
+  int mp, pw;     // panel rows and width
+  int *piv;
+  int *pairs;
+  int nleaves;
+};
+
+// Sub-panel whose first row and column are c0 inside the panel.
+static int lu_panel_rec(hipStream_t s, panel_rec &pr, int c0, int w) {
   const int leaf = leaf_max_width();
-  if (w <= leaf) return k_lu_leaf(s, P, mp, w, lda, piv);
+  const long lda = pr.lda;
+  double *P = pr.panel + c0 * lda + c0;
+  const int mp = pr.mp - c0;
+  if (w <= leaf) {
+    if (pr.nleaves >= 16) return fail(PLA_ERR_INVALID_PARAM, "lu panel: more than 16 leaves");
+    lu_leaf_call c;
+    c.P = P;
+    c.Ppanel = pr.panel + c0;
+    c.lda = lda;
+    c.mp = mp;
+    c.w = w;
+    c.pw = pr.pw;
+    c.pc0 = c0;
+    c.piv = pr.piv + c0;
+    c.pairs = pr.pairs + (size_t)pr.nleaves * leaf_pairs_stride();
+    pr.nleaves++;
+    return k_lu_leaf(s, c);
+  }
   const int w0 = split(w, leaf), w1 = w - w0;
   int rc;
-  if ((rc = lu_panel(s, P, mp, w0, lda, piv))) return rc;
-  double *Pr = P + (long)w0 * lda;  // right columns
-  if ((rc = k_laswp(s, Pr, lda, mp, w1, w0, piv))) return rc;
+  if ((rc = lu_panel_rec(s, pr, c0, w0))) return rc;
+  double *Pr = P + (long)w0 * lda;  // right columns (already interchanged by the leaves)
   if ((rc = trsm_left(s, true, false, true, w0, w1, P, lda, Pr, lda, nullptr))) return rc;
   if ((rc = k_gemm(s, 0, false, false, mp - w0, w1, w0, -1.0, P + w0, lda, Pr, lda, 1.0, Pr + w0, lda,
                    nullptr)))
     return rc;
-  if ((rc = lu_panel(s, Pr + w0, mp - w0, w1, lda, piv + w0))) return rc;
-  // the right half's interchanges also apply to the left half's rows below w0
-  return k_laswp(s, P + w0, lda, mp - w0, w0, w1, piv + w0);
+  return lu_panel_rec(s, pr, c0 + w0, w1);
+}
+
+int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *pairs, int *nleaves) {
+  panel_rec pr{P, lda, mp, w, piv, pairs, 0};
+  const int rc = w > 0 ? lu_panel_rec(s, pr, 0, w) : PLA_SUCCESS;
+  *nleaves = pr.nleaves;
+  return rc;
 }
 
 int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info) {

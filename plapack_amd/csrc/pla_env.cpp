@@ -83,45 +83,28 @@ void *workspace(size_t bytes, int slot) {
 
 static leaf_ws g_leaf;
 
-leaf_ws &leaf_workspace(int G) {
-  if (g_leaf.base && g_leaf.capacity >= G) return g_leaf;
+leaf_ws &leaf_workspace(size_t slot_bytes) {
+  if (g_leaf.base && g_leaf.slot_bytes >= slot_bytes) return g_leaf;
   if (g_leaf.base) {
     if (hipDeviceSynchronize() != hipSuccess) return g_leaf;
     (void)hipFree(g_leaf.base);
     g_leaf = leaf_ws();
   }
-  const size_t W = 32;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    size_t o = off;
-    off += (bytes + 255) & ~(size_t)255;
-    return o;
-  };
-  const size_t o_row = take(sizeof(double) * 2 * G * W);
-  const size_t o_val = take(sizeof(double) * 2 * G);
-  const size_t o_idx = take(sizeof(int) * 2 * G);
-  const size_t o_rowj = take(sizeof(double) * 2 * W);
-  const size_t o_flags = take(sizeof(unsigned) * G);
-  const size_t o_err = take(sizeof(int));
+  const size_t total = slot_bytes + 256;
   void *p = nullptr;
-  if (hipMalloc(&p, off) != hipSuccess) {
+  if (hipMalloc(&p, total) != hipSuccess) {
     fail(PLA_ERR_NO_MEMORY, "panel exchange workspace allocation failed");
     return g_leaf;
   }
-  if (hipMemset(p, 0, off) != hipSuccess) {
+  if (hipMemset(p, 0, total) != hipSuccess) {
     (void)hipFree(p);
     fail(PLA_ERR_DEVICE, "panel exchange workspace initialisation failed");
     return g_leaf;
   }
-  char *b = (char *)p;
   g_leaf.base = p;
-  g_leaf.capacity = G;
-  g_leaf.row = (double *)(b + o_row);
-  g_leaf.val = (double *)(b + o_val);
-  g_leaf.idx = (int *)(b + o_idx);
-  g_leaf.rowj = (double *)(b + o_rowj);
-  g_leaf.flags = (unsigned *)(b + o_flags);
-  g_leaf.err = (int *)(b + o_err);
+  g_leaf.slot_bytes = slot_bytes;
+  g_leaf.slots = (unsigned long long *)p;
+  g_leaf.err = (int *)((char *)p + slot_bytes);
   return g_leaf;
 }
 
@@ -327,6 +310,22 @@ int PLA_Set_device_nb(int nb) {
   if (nb < 0) return fail(PLA_ERR_INVALID_PARAM, "PLA_Set_device_nb: nb = %d", nb);
   pla::ctx().nb_device = nb;
   return PLA_SUCCESS;
+}
+
+int PLA_Set_device_param(int param, int value) {
+  pla::context &c = pla::ctx();
+  switch (param) {
+    case PLA_PARAM_LOOKAHEAD:
+      if (value < 0 || value > 1) return fail(PLA_ERR_INVALID_PARAM, "look-ahead must be 0 or 1");
+      c.lookahead = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_LEAF_ROWS:
+      if (value < 32 || value > 1024) return fail(PLA_ERR_INVALID_PARAM, "leaf rows must be in [32, 1024]");
+      c.leaf_rows = value;
+      return PLA_SUCCESS;
+    default:
+      return fail(PLA_ERR_INVALID_PARAM, "unknown device parameter %d", param);
+  }
 }
 
 int PLA_Get_device_nb(int n, int *nb) {

@@ -64,6 +64,9 @@ struct context {
   int nb_user = 0;                   // set by pla_Environ_set_nb_alg (0 = not set)
   int nb_device = 0;                 // PLA_Set_device_nb (0 = built-in choice)
   int cu_count = 256;
+  // tuning parameters (PLA_Set_device_param)
+  int lookahead = 1;        // factor panel k+1 beside step k's trailing update
+  int leaf_rows = 256;      // target rows per workgroup of the panel leaf
 };
 context &ctx();
 int ensure_device();                 // lazily initialise device + streams
@@ -93,15 +96,11 @@ int scalar_value(PLA_Obj o, double *v);
 // ---- exchange slots of the panel-LU leaf kernel (allocated once, zeroed) ----
 struct leaf_ws {
   void *base = nullptr;
-  int capacity = 0;         // workgroups the slots were sized for
-  double *row = nullptr;    // [2][G][32]
-  double *val = nullptr;    // [2][G]
-  int *idx = nullptr;       // [2][G]
-  double *rowj = nullptr;   // [2][32]
-  unsigned *flags = nullptr;// [G]
-  int *err = nullptr;       // timeout flag
+  size_t slot_bytes = 0;
+  unsigned long long *slots = nullptr;  // tagged exchange words
+  int *err = nullptr;                   // timeout flag
 };
-leaf_ws &leaf_workspace(int G);
+leaf_ws &leaf_workspace(size_t slot_bytes);
 // Returns PLA_ERR_DEVICE if any panel exchange timed out since the last call.
 int leaf_check_error(hipStream_t s);
 
@@ -123,7 +122,21 @@ int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D,
              const int *abort_flag);
 int k_scale2d(hipStream_t s, int m, int n, double alpha, double *D, long ldd);
 int k_fill_drand48(hipStream_t s, int m, int n, double *D, long ldd, uint64_t state0, uint64_t skip);
-int k_lu_leaf(hipStream_t s, double *P, int mp, int w, long lda, int *piv);
+struct lu_leaf_call {
+  double *P;       // leaf top-left
+  double *Ppanel;  // panel column 0 at the leaf's first row
+  long lda;
+  int mp, w;       // rows from the leaf's first row to the panel bottom, leaf width
+  int pw, pc0;     // panel width, leaf's first column inside the panel
+  int *piv;        // w pivot offsets (relative to each row)
+  int *pairs;      // leaf_pairs_stride() ints: count, (dst, src) row pairs from the panel top
+};
+int k_lu_leaf(hipStream_t s, const lu_leaf_call &c);
+// rows of a panel (A = its first row): apply nleaves lists of (dst, src) pairs
+// to columns [0, nleft) and [c1, c2)
+int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, const int *pairs,
+                 int nleaves);
+int leaf_pairs_stride();
 // infinity norm (max absolute row sum) or one norm (max absolute column sum); synchronous
 int k_norm(hipStream_t s, bool inf_norm, int m, int n, const double *A, long lda, double *result);
 int leaf_max_width();
@@ -137,7 +150,10 @@ int trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs,
 int trsm_rltn(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
               const int *abort_flag);
 // Recursive panel LU of an mp x w panel (mp >= w); pivots relative to each row.
-int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv);
+// The leaves apply their interchanges to all w columns of the panel and
+// append their (dst, src) row pairs to `pairs`; returns the number of leaves
+// in *nleaves.
+int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *pairs, int *nleaves);
 // Recursive lower Cholesky of a b x b block; failures go to *info (1-based,
 // offset added).
 int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info);

@@ -178,57 +178,69 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Unblocked panel LU of a tall panel P (mp rows, w <= 32 columns), one
+// Unblocked panel LU of a tall leaf (mp rows, w <= 32 columns) in one
 // persistent grid of G workgroups; workgroup g keeps rows [g*R, g*R+R) in LDS.
-// Per column: local (|value|, index) maximum, publication of the candidate
-// (value, index, whole row) and of the current row j, one cross-workgroup
-// exchange, then every workgroup picks the same winner (largest |value|,
-// ties to the smallest index, as PLA_Iamax.c:108-115), swaps rows in LDS,
-// scales by the reciprocal of the pivot and applies the rank-1 update.
 //
-// Exchange protocol: wave 0 of each workgroup writes its slot with
-// device-scope stores and then releases flags[g] = tag0 + j; readers poll
-// all G flags (bounded spin), acquire, and read slots with device-scope
-// loads.  Slots are double buffered by column parity; one flag per
-// workgroup suffices because no workgroup can run two columns ahead.
+// Rows are never moved while the leaf is factored.  Each row carries its
+// logical position pos[r] (initially its row number); interchanging rows j
+// and p only relabels them (pivot row -> j, the row at j -> p).  Rows at
+// positions < j are the pivot rows already chosen.  Per column j:
+//   1. local candidate: largest |L(r, j)| over rows at positions >= j, ties to
+//      the smallest position (the first maximum, as idamax and the tie rule
+//      of PLA_Iamax.c:108-115);
+//   2. wave 0 publishes its candidate row, then (value, position);
+//   3. every workgroup reads all G (value, position) words, picks the same
+//      winner, reads the winner's row, and applies the relabelling, the
+//      reciprocal scaling (PLA_Local_BLAS1.c:258) and the rank-1 update.
+// At the end rows are written to their final positions, the interchanges
+// are replayed into at most 2w (destination, source) row pairs, and the
+// pairs are applied to the panel's other columns (so the recursive panel
+// needs no separate row-interchange launches).  Workgroup 0 also stores the
+// pairs for the full-width interchange of the panel.
+//
+// Exchange protocol: every 64-bit slot word carries the 32-bit tag of the
+// column it belongs to next to 32 bits of payload (a double is split into two
+// words).  Writers use plain device-scope stores, readers device-scope loads
+// and re-read until every word they need carries the current tag, so no
+// fences are needed.  Two round trips per column: the G small (value,
+// position) records, then the one winning row -- reading every candidate
+// row instead costs several times more (measured on MI355X: 2.5 us vs
+// 8.7 us per exchange at G = 64).  Slots are double buffered by column
+// parity: no workgroup can publish column j+2 before every workgroup has
+// finished reading column j.  Tags only grow across launches.  A bounded
+// spin sets *err and lets every wave exit.
 constexpr int LEAF_W = 32;
 constexpr int LEAF_THREADS = 256;
+constexpr int LEAF_MAXG = 128;
+constexpr int CAND_WORDS = 4;            // value lo/hi, position, pad
+constexpr int ROW_WORDS = 2 * LEAF_W;    // candidate row, lo/hi halves
+constexpr int SLOT_WORDS = CAND_WORDS + ROW_WORDS;
+constexpr int PAIRS_MAX = 2 * LEAF_W;
+constexpr int PAIRS_STRIDE = 1 + 2 * PAIRS_MAX;  // count, then (dst, src) pairs
 
-struct leaf_slots {
-  double *row;      // [2][G][LEAF_W] candidate rows
-  double *val;      // [2][G] signed candidate values
-  int *idx;         // [2][G] candidate global row index (INT_MAX: none)
-  double *rowj;     // [2][LEAF_W] row j before the swap
-  unsigned *flags;  // [G] publication tags
-  int *err;         // set to 1 if a wait timed out
+struct leaf_args {
+  double *P;                // leaf top-left (its row 0 is the leaf's first diagonal row)
+  double *Ppanel;           // panel column 0, at the leaf's first row
+  long lda;
+  int mp, w;                // rows from the leaf's first row to the panel bottom, leaf width
+  int pw, pc0;              // panel width, leaf's first column inside the panel
+  int R;                    // rows per workgroup
+  int *piv;                 // w entries: pivot offset relative to row j
+  int *pairs;               // PAIRS_STRIDE ints, positions relative to the panel's first row
+  unsigned long long *slots;  // [2][G][CAND_WORDS] then [2][G][ROW_WORDS]
+  int *err;
+  unsigned tag0, spin_limit;
 };
 
-__device__ inline void argmax_combine(double &bv, int &bi, double v, int i) {
-  if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
+__device__ inline void argmax3(double &bv, int &bi, int &br, double v, int i, int r) {
+  if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; br = r; }
 }
 
-// Block-wide (|v|, idx) argmax; result valid in every thread.
-__device__ inline void block_argmax(double &bv, int &bi, double *red_v, int *red_i) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    double ov = __shfl_xor(bv, off, 64);
-    int oi = __shfl_xor(bi, off, 64);
-    argmax_combine(bv, bi, ov, oi);
-  }
-  if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
-  __syncthreads();
-  bv = red_v[0];
-  bi = red_i[0];
-  for (int w = 1; w < LEAF_THREADS / 64; w++) argmax_combine(bv, bi, red_v[w], red_i[w]);
-  __syncthreads();
-}
-
-__device__ inline void st_dev(double *p, double v) {
+__device__ inline void st_dev(unsigned long long *p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void st_dev(int *p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline unsigned long long ld_dev(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline double ld_dev(const double *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -236,101 +248,268 @@ __device__ inline double ld_dev(const double *p) {
 __device__ inline int ld_dev(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline unsigned dbl_half(double v, int hi) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return hi ? (unsigned)(b >> 32) : (unsigned)b;
+}
+__device__ inline double dbl_from(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
-__global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(
-    double *__restrict__ P, int mp, int w, long lda, int *__restrict__ piv, int R,
-    leaf_slots s, unsigned tag0, unsigned spin_limit) {
+// Bounded wait: every lane counts the same iterations; the error flag is read
+// by lane 0 and the decision is wave-uniform, so the wave leaves together.
+// Returns true when the wait is abandoned (after setting *err).
+__device__ inline bool wave_spin_fail(unsigned &it, const leaf_args &a) {
+  ++it;
+  const int bail = (it > a.spin_limit) || ((threadIdx.x & 63) == 0 && (it & 63) == 0 && ld_dev(a.err));
+  if (__ballot(bail)) {
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  __builtin_amdgcn_s_sleep(1);
+  return false;
+}
+
+// Wave-wide (|v|, position, row) argmax; result valid in every lane.
+__device__ inline void wave_argmax3(double &bv, int &bi, int &br) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    const int orr = __shfl_xor(br, off, 64);
+    argmax3(bv, bi, br, ov, oi, orr);
+  }
+}
+
+// Apply (dst, src) row pairs to one column with one wave: every lane loads
+// its source before any lane stores (one load instruction, then one store).
+__device__ inline void wave_gather(double *col, int cnt, const int *dst, const int *src) {
+  const int lane = threadIdx.x & 63;
+  if (lane < cnt) {
+    const double v = ld_dev(col + src[lane]);
+    col[dst[lane]] = v;
+  }
+}
+
+__global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
   extern __shared__ double lds[];
-  double *L = lds;                        // L[c*R + r]
-  double *prow = lds + (size_t)w * R;     // pivot row (new row j)
-  double *orow = prow + LEAF_W;           // old row j
-  __shared__ double red_v[LEAF_THREADS / 64];
-  __shared__ int red_i[LEAF_THREADS / 64];
+  const int w = a.w, R = a.R;
+  double *L = lds;                                // L[c*R + r]
+  int *pos = (int *)(lds + (size_t)w * R);        // logical position of local row r
+  __shared__ double red_v[2][LEAF_THREADS / 64];
+  __shared__ int red_i[2][LEAF_THREADS / 64], red_r[2][LEAF_THREADS / 64];
+  __shared__ double prow_s[LEAF_THREADS / 64][LEAF_W];  // each wave's copy of the winning row
+  __shared__ int s_p[LEAF_W];
+  __shared__ int s_dst[PAIRS_MAX], s_src[PAIRS_MAX];
+  __shared__ int s_cnt;
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
   const int r0 = g * R;
-  const int rows = max(0, min(mp - r0, R));
+  const int rows = max(0, min(a.mp - r0, R));
   // a timed-out exchange earlier on this stream: leave the panel untouched
-  if (__hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (ld_dev(a.err)) return;
 
   for (int e = tid; e < rows * w; e += LEAF_THREADS) {
     const int c = e / rows, r = e - c * rows;
-    L[c * R + r] = P[(long)c * lda + r0 + r];
+    L[c * R + r] = a.P[(long)c * a.lda + r0 + r];
   }
+  for (int r = tid; r < rows; r += LEAF_THREADS) pos[r] = r0 + r;
   __syncthreads();
+
+  unsigned long long *cand_slots = a.slots;
+  unsigned long long *row_slots = a.slots + (size_t)2 * LEAF_MAXG * CAND_WORDS;
+  // this thread's candidate for column 0 (every thread owns rows tid, tid+256, ...)
+  double bv = -1.0;
+  int bi = INT_MAX, br = -1;
+  for (int r = tid; r < rows; r += LEAF_THREADS) argmax3(bv, bi, br, fabs(L[r]), pos[r], r);
 
   for (int j = 0; j < w; j++) {
     const int par = j & 1;
-    const unsigned tag = tag0 + (unsigned)j;
-    // local candidate
-    double bv = -1.0;
-    int bi = INT_MAX;
-    for (int r = tid; r < rows; r += LEAF_THREADS)
-      if (r0 + r >= j) argmax_combine(bv, bi, fabs(L[j * R + r]), r0 + r);
-    block_argmax(bv, bi, red_v, red_i);
+    const unsigned long long tag = (unsigned long long)(a.tag0 + (unsigned)j) << 32;
 
-    // publish (wave 0 only, then its lane 0 releases the flag)
-    if (tid < 64) {
-      const bool have = (bi != INT_MAX);
-      if (tid < w && have) st_dev(&s.row[((size_t)par * G + g) * LEAF_W + tid], L[tid * R + (bi - r0)]);
-      if (j >= r0 && j < r0 + rows && tid >= 32 && tid - 32 < w)
-        st_dev(&s.rowj[par * LEAF_W + (tid - 32)], L[(tid - 32) * R + (j - r0)]);
-      if (tid == 0) {
-        st_dev(&s.val[par * G + g], have ? L[j * R + (bi - r0)] : 0.0);
-        st_dev(&s.idx[par * G + g], bi);
-        __hip_atomic_store(&s.flags[g], tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
+    // workgroup candidate: wave reduction, then the 4 wave results (one barrier;
+    // the arrays alternate by column so a wave running ahead cannot clobber them)
+    wave_argmax3(bv, bi, br);
+    if (lane == 0) {
+      red_v[par][wave] = bv;
+      red_i[par][wave] = bi;
+      red_r[par][wave] = br;
     }
-    // wait for every workgroup's publication of column j
-    if (tid < G) {
-      unsigned it = 0;
-      while (__hip_atomic_load(&s.flags[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tag) {
-        if (++it > spin_limit ||
-            ((it & 255) == 0 && __hip_atomic_load(s.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-          __hip_atomic_store(s.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
     __syncthreads();
+    bv = red_v[par][0];
+    bi = red_i[par][0];
+    br = red_r[par][0];
+    for (int v = 1; v < LEAF_THREADS / 64; v++) argmax3(bv, bi, br, red_v[par][v], red_i[par][v], red_r[par][v]);
 
-    // global winner (identical in every workgroup)
+    // publish (wave 0): the candidate row first, then (value, position)
+    if (wave == 0) {
+      const bool have = (br >= 0);
+      const int c = lane >> 1;
+      st_dev(row_slots + ((size_t)par * G + g) * ROW_WORDS + lane,
+             tag | dbl_half((have && c < w) ? L[c * R + br] : 0.0, lane & 1));
+      if (lane < 3) {
+        const unsigned pl = lane < 2 ? dbl_half(have ? L[j * R + br] : 0.0, lane) : (unsigned)bi;
+        st_dev(cand_slots + ((size_t)par * G + g) * CAND_WORDS + lane, tag | pl);
+      }
+    }
+
+    // round trip 1, per wave: every workgroup's (value, position); lane l
+    // holds the records of workgroups l and l + 64, then a wave reduction
     double gv = -1.0;
     int gi = INT_MAX, gw = -1;
-    for (int t = tid; t < G; t += LEAF_THREADS) {
-      const int ci = ld_dev(&s.idx[par * G + t]);
-      if (ci >= j && ci < mp) argmax_combine(gv, gi, fabs(ld_dev(&s.val[par * G + t])), ci);
+    {
+      const unsigned long long *cs = cand_slots + (size_t)par * G * CAND_WORDS;
+      unsigned it = 0;
+      for (;;) {
+        int ok = 1;
+        gv = -1.0;
+        gi = INT_MAX;
+        gw = -1;
+        for (int t = lane; t < G; t += 64) {
+          const unsigned long long w0 = ld_dev(cs + (size_t)t * CAND_WORDS);
+          const unsigned long long w1 = ld_dev(cs + (size_t)t * CAND_WORDS + 1);
+          const unsigned long long w2 = ld_dev(cs + (size_t)t * CAND_WORDS + 2);
+          if (((w0 & w1 & w2) >> 32) != (tag >> 32) || ((w0 | w1 | w2) >> 32) != (tag >> 32)) ok = 0;
+          const int ci = (int)(unsigned)w2;
+          if (ci >= j && ci < a.mp) argmax3(gv, gi, gw, fabs(dbl_from((unsigned)w0, (unsigned)w1)), ci, t);
+        }
+        if (__all(ok)) break;
+        if (wave_spin_fail(it, a)) break;
+      }
+      wave_argmax3(gv, gi, gw);
     }
-    block_argmax(gv, gi, red_v, red_i);
-    gw = (gi != INT_MAX) ? gi / R : -1;  // workgroup owning the winning row
-    const int p = (gi != INT_MAX) ? gi : j;  // all-NaN column: no interchange
-    if (tid < w) {
-      orow[tid] = ld_dev(&s.rowj[par * LEAF_W + tid]);
-      prow[tid] = (gw >= 0) ? ld_dev(&s.row[((size_t)par * G + gw) * LEAF_W + tid]) : orow[tid];
+
+    // round trip 2, per wave: the winner's row (all-NaN column: no winner)
+    {
+      const unsigned long long *rs = row_slots + ((size_t)par * G + (gw >= 0 ? gw : 0)) * ROW_WORDS;
+      unsigned it = 0;
+      for (;;) {
+        int ok = 1;
+        double v = __longlong_as_double(0x7FF8000000000000LL);
+        if (lane < w && gw >= 0) {
+          const unsigned long long lo = ld_dev(rs + 2 * lane), hi = ld_dev(rs + 2 * lane + 1);
+          if ((lo >> 32) != (tag >> 32) || (hi >> 32) != (tag >> 32)) ok = 0;
+          v = dbl_from((unsigned)lo, (unsigned)hi);
+        }
+        if (lane < w) prow_s[wave][lane] = v;
+        if (__all(ok)) break;
+        if (wave_spin_fail(it, a)) break;
+      }
+      __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
-    if (p != j && tid < w) {
-      if (p >= r0 && p < r0 + rows) L[tid * R + (p - r0)] = orow[tid];
-      if (j >= r0 && j < r0 + rows) L[tid * R + (j - r0)] = prow[tid];
-    }
-    __syncthreads();
+
+    const double *prow = prow_s[wave];
+    const int p = (gw >= 0) ? gi : j;
     const double inv = 1.0 / prow[j];
+    // relabel, scale, rank-1 update; the next column's candidate on the way
+    bv = -1.0;
+    bi = INT_MAX;
+    br = -1;
     for (int r = tid; r < rows; r += LEAF_THREADS) {
-      if (r0 + r > j) {
+      int q = pos[r];
+      if (q == p) q = j;
+      else if (q == j) q = p;
+      pos[r] = q;
+      if (q > j) {
         const double l = L[j * R + r] * inv;
         L[j * R + r] = l;
         for (int c = j + 1; c < w; c++) L[c * R + r] = fma(l, -prow[c], L[c * R + r]);
+        if (j + 1 < w) argmax3(bv, bi, br, fabs(L[(j + 1) * R + r]), q, r);
       }
     }
-    if (g == 0 && tid == 0) piv[j] = p - j;
-    __syncthreads();
+    if (tid == 0) {
+      s_p[j] = p;
+      if (g == 0) a.piv[j] = p - j;
+    }
   }
+  __syncthreads();
 
+  // rows to their final positions (a permutation: no two rows collide)
   for (int e = tid; e < rows * w; e += LEAF_THREADS) {
     const int c = e / rows, r = e - c * rows;
-    P[(long)c * lda + r0 + r] = L[c * R + r];
+    a.P[(long)c * a.lda + pos[r]] = L[c * R + r];
+  }
+
+  // replay the interchanges on (position -> source row) entries held by wave 0
+  if (tid < 64) {
+    const int lane = tid;
+    int key = -1, val = -1, cnt = 0;
+    for (int j = 0; j < w; j++) {
+      const int p = s_p[j];
+      if (p == j) continue;
+      unsigned long long mj = __ballot(key == j);
+      if (!mj) {
+        if (lane == cnt) { key = j; val = j; }
+        mj = 1ULL << cnt;
+        cnt++;
+      }
+      unsigned long long mp_ = __ballot(key == p);
+      if (!mp_) {
+        if (lane == cnt) { key = p; val = p; }
+        mp_ = 1ULL << cnt;
+        cnt++;
+      }
+      const int lj = __ffsll((long long)mj) - 1, lp = __ffsll((long long)mp_) - 1;
+      const int vj = __shfl(val, lj, 64), vp = __shfl(val, lp, 64);
+      if (lane == lj) val = vp;
+      if (lane == lp) val = vj;
+    }
+    const bool moved = (key >= 0 && key != val);
+    const unsigned long long mm = __ballot(moved);
+    const int slot = __popcll(mm & ((1ULL << lane) - 1ULL));
+    if (moved) {
+      s_dst[slot] = key;
+      s_src[slot] = val;
+      if (g == 0) {
+        a.pairs[1 + 2 * slot] = key + a.pc0;
+        a.pairs[2 + 2 * slot] = val + a.pc0;
+      }
+    }
+    if (lane == 0) {
+      s_cnt = __popcll(mm);
+      if (g == 0) a.pairs[0] = __popcll(mm);
+    }
+  }
+  __syncthreads();
+
+  // the same interchanges on the panel's other columns, one column per wave
+  const int cnt = s_cnt;
+  const int others = a.pw - w;
+  if (cnt > 0) {
+    for (int q = g * (LEAF_THREADS / 64) + (tid >> 6); q < others; q += G * (LEAF_THREADS / 64)) {
+      const int cc = q < a.pc0 ? q : q + w;
+      wave_gather(a.Ppanel + (long)cc * a.lda, cnt, s_dst, s_src);
+    }
+  }
+}
+
+// Full-width interchanges of one panel: for every column outside the panel
+// (ranges [0, nleft) and [c1, c2)), apply the leaves' (dst, src) row pairs
+// leaf by leaf, one column per wave.  A is the panel's first row.
+__global__ __launch_bounds__(256) void panel_swap_kernel(double *__restrict__ A, long lda, int nleft, int c1,
+                                                         int c2, const int *__restrict__ pairs,
+                                                         int nleaves) {
+  __shared__ int s_pairs[16][PAIRS_STRIDE];
+  for (int e = threadIdx.x; e < nleaves * PAIRS_STRIDE; e += 256) {
+    const int q = e / PAIRS_STRIDE, k = e - q * PAIRS_STRIDE;
+    s_pairs[q][k] = pairs[e];
+  }
+  __syncthreads();
+  const int total = nleft + (c2 - c1);
+  const int lane = threadIdx.x & 63;
+  for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < total; q += gridDim.x * 4) {
+    const int col = q < nleft ? q : c1 + (q - nleft);
+    double *cp = A + (long)col * lda;
+    for (int l = 0; l < nleaves; l++) {
+      const int cnt = s_pairs[l][0];
+      if (lane < cnt) {
+        const double v = ld_dev(cp + s_pairs[l][2 + 2 * lane]);
+        cp[s_pairs[l][1 + 2 * lane]] = v;
+      }
+      // the next leaf's loads (device scope, past the L1) must see these stores
+      __builtin_amdgcn_s_waitcnt(0);
+    }
   }
 }
 
@@ -750,43 +929,69 @@ int k_fill_drand48(hipStream_t s, int m, int n, double *D, long ldd, uint64_t st
   return PLA_SUCCESS;
 }
 
-// Panel leaf: chooses the grid, stages the exchange slots in a workspace and
-// checks the timeout flag lazily (see lu_leaf_error).
+// Panel leaf: chooses the grid (about 256 rows per workgroup, at most
+// LEAF_MAXG workgroups, all co-resident) and draws fresh exchange tags.
 static unsigned g_leaf_tag = 1;
 
-int k_lu_leaf(hipStream_t s, double *P, int mp, int w, long lda, int *piv) {
-  if (mp <= 0 || w <= 0) return PLA_SUCCESS;
-  if (w > LEAF_W) return fail(PLA_ERR_INVALID_PARAM, "lu leaf: w = %d > %d", w, LEAF_W);
+int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
+  if (c.mp <= 0 || c.w <= 0) return PLA_SUCCESS;
+  if (c.w > LEAF_W || c.mp < c.w) return fail(PLA_ERR_INVALID_PARAM, "lu leaf: %d x %d", c.mp, c.w);
   int maxG = ctx().cu_count > 0 ? ctx().cu_count : 256;
-  if (maxG > LEAF_THREADS) maxG = LEAF_THREADS;  // one polling thread per workgroup
-  int G = (mp + 127) / 128;
+  if (maxG > LEAF_MAXG) maxG = LEAF_MAXG;
+  const int target = ctx().leaf_rows > 0 ? ctx().leaf_rows : 256;
+  int G = (c.mp + target - 1) / target;
   if (G > maxG) G = maxG;
   if (G < 1) G = 1;
-  int R = (mp + G - 1) / G;
-  const size_t lds_bytes = ((size_t)w * R + 2 * LEAF_W) * sizeof(double);
+  const int R = (c.mp + G - 1) / G;
+  const size_t lds_bytes = (size_t)c.w * R * sizeof(double) + (size_t)R * sizeof(int);
   if (lds_bytes > 150 * 1024)
-    return fail(PLA_ERR_DIMENSIONS, "lu leaf: panel of %d rows exceeds the LDS budget", mp);
-  leaf_ws &ws = leaf_workspace(maxG);
+    return fail(PLA_ERR_DIMENSIONS, "lu leaf: panel of %d rows exceeds the LDS budget", c.mp);
+  leaf_ws &ws = leaf_workspace((size_t)2 * LEAF_MAXG * SLOT_WORDS * sizeof(unsigned long long));
   if (!ws.base) return PLA_ERR_NO_MEMORY;
-  leaf_slots sl;
-  sl.row = ws.row;
-  sl.val = ws.val;
-  sl.idx = ws.idx;
-  sl.rowj = ws.rowj;
-  sl.flags = ws.flags;
-  sl.err = ws.err;
   if (g_leaf_tag > 0xF0000000u) {  // tags only grow; restart far below wrap-around
-    PLA_HIP(hipMemsetAsync(ws.flags, 0, sizeof(unsigned) * maxG, s));
+    PLA_HIP(hipMemsetAsync(ws.slots, 0, (size_t)2 * LEAF_MAXG * SLOT_WORDS * sizeof(unsigned long long), s));
     g_leaf_tag = 1;
   }
-  const unsigned tag0 = g_leaf_tag;
-  g_leaf_tag += (unsigned)w;
-  hipLaunchKernelGGL(lu_leaf_kernel, dim3(G), dim3(LEAF_THREADS), lds_bytes, s, P, mp, w, lda, piv, R,
-                     sl, tag0, 200000u);
+  leaf_args a;
+  a.P = c.P;
+  a.Ppanel = c.Ppanel;
+  a.lda = c.lda;
+  a.mp = c.mp;
+  a.w = c.w;
+  a.pw = c.pw;
+  a.pc0 = c.pc0;
+  a.R = R;
+  a.piv = c.piv;
+  a.pairs = c.pairs;
+  a.slots = ws.slots;
+  a.err = ws.err;
+  a.tag0 = g_leaf_tag;
+  a.spin_limit = 400000u;
+  static bool lds_attr = false;  // allow more than 64 KiB of dynamic LDS (gfx950 has 160 KiB)
+  if (!lds_attr) {
+    PLA_HIP(hipFuncSetAttribute((const void *)lu_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                150 * 1024));
+    lds_attr = true;
+  }
+  g_leaf_tag += (unsigned)c.w;
+  hipLaunchKernelGGL(lu_leaf_kernel, dim3(G), dim3(LEAF_THREADS), lds_bytes, s, a);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
 
+int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, const int *pairs,
+                 int nleaves) {
+  const int total = nleft + (c2 > c1 ? c2 - c1 : 0);
+  if (total <= 0 || nleaves <= 0) return PLA_SUCCESS;
+  if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel swap: %d leaves", nleaves);
+  const int blocks = (total + 3) / 4;
+  hipLaunchKernelGGL(panel_swap_kernel, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1, c2 > c1 ? c2 : c1,
+                     pairs, nleaves);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int leaf_pairs_stride() { return PAIRS_STRIDE; }
 int leaf_max_width() { return LEAF_W; }
 int trsm_leaf_size() { return TRSM_LEAF; }
 

@@ -33,31 +33,77 @@ static int check_lu_args(PLA_Obj A, PLA_Obj pivots) {
 
 namespace pla {
 
-// Blocked right-looking LU of the m x n column-major device matrix a.
-int lu_factor(hipStream_t s, int m, int n, double *a, long lda, int *piv, int nb) {
+// Blocked right-looking LU of the m x n column-major device matrix a, with
+// one step of look-ahead: panel k+1 is factored on the look-ahead stream as
+// soon as its own columns have received step k's update, while the main
+// stream applies the rest of step k's trailing update.
+//
+// main stream, step k:  wait(panel k) -> full-width interchanges of panel k
+//                       -> A12 <- inv(L11) A12 -> A22[:, next panel] -= A21 A12
+//                       -> signal -> A22[:, rest] -= A21 A12
+// look-ahead stream:    wait(signal) -> factor panel k+1 -> signal(panel k+1)
+int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int nb) {
+  hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
   const int mn = m < n ? m : n;
+  if (nb > 512) nb = 512;
+  if (nb < 1) nb = 1;
+  const int stride = leaf_pairs_stride();
+  int *pairs = (int *)workspace(sizeof(int) * 2 * 16 * stride, 3);
+  if (!pairs) return PLA_ERR_NO_MEMORY;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  if (!ev_ready) {
+    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+  }
   int rc;
+  int nleaves[2] = {0, 0};
+  // panel 0, after everything already queued on the main stream
+  {
+    const int b = mn < nb ? mn : nb;
+    PLA_HIP(hipEventRecord(ev_ready, sM));
+    PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+    if ((rc = lu_panel(sP, a, m, b, lda, piv, pairs, &nleaves[0]))) return rc;
+    PLA_HIP(hipEventRecord(ev_panel, sP));
+  }
   for (int k = 0; k < mn; k += nb) {
     const int b = mn - k < nb ? mn - k : nb;
+    const int par = (k / nb) & 1;
     double *Akk = a + (long)k * lda + k;
-    if ((rc = lu_panel(s, Akk, m - k, b, lda, piv + k))) return rc;
-    if ((rc = k_laswp(s, a + k, lda, m - k, k, b, piv + k))) return rc;
-    const int nr = n - k - b;
-    if (nr > 0) {
-      double *A12 = a + (long)(k + b) * lda + k;
-      if ((rc = k_laswp(s, A12, lda, m - k, nr, b, piv + k))) return rc;
-      if ((rc = trsm_left(s, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
-      const int mr = m - k - b;
-      if (mr > 0) {
-        timing_begin(s, 1);
-        rc = k_gemm(s, 1, false, false, mr, nr, b, -1.0, Akk + b, lda, A12, lda, 1.0, A12 + b, lda,
-                    nullptr);
-        timing_end(s, 1, 2.0 * mr * (double)nr * b);
-        if (rc) return rc;
-      }
+    PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
+    const int nr = n - k - b, mr = m - k - b;
+    if ((rc = k_panel_swap(sM, a + k, lda, k, k + b, n, pairs + (size_t)par * 16 * stride,
+                           nleaves[par])))
+      return rc;
+    if (nr <= 0) continue;
+    double *A12 = a + (long)(k + b) * lda + k;
+    if ((rc = trsm_left(sM, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
+    if (mr <= 0) continue;
+    const int kn = k + b;
+    const int bn = kn < mn ? (mn - kn < nb ? mn - kn : nb) : 0;
+    int done = 0;  // trailing columns already updated
+    if (bn > 0) {
+      timing_begin(sM, 1);
+      rc = k_gemm(sM, 1, false, false, mr, bn, b, -1.0, Akk + b, lda, A12, lda, 1.0, A12 + b, lda, nullptr);
+      timing_end(sM, 1, 2.0 * mr * (double)bn * b);
+      if (rc) return rc;
+      done = bn;
+      PLA_HIP(hipEventRecord(ev_ready, sM));
+      PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+      double *An = a + (long)kn * lda + kn;
+      if ((rc = lu_panel(sP, An, m - kn, bn, lda, piv + kn, pairs + (size_t)(par ^ 1) * 16 * stride,
+                         &nleaves[par ^ 1])))
+        return rc;
+      PLA_HIP(hipEventRecord(ev_panel, sP));
+    }
+    if (nr - done > 0) {
+      timing_begin(sM, 1);
+      rc = k_gemm(sM, 1, false, false, mr, nr - done, b, -1.0, Akk + b, lda, A12 + (long)done * lda, lda,
+                  1.0, A12 + (long)done * lda + b, lda, nullptr);
+      timing_end(sM, 1, 2.0 * mr * (double)(nr - done) * b);
+      if (rc) return rc;
     }
   }
-  return leaf_check_error(s);
+  return leaf_check_error(sM);
 }
 
 }  // namespace pla

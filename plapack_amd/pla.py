@@ -43,6 +43,8 @@ PLA_UNIT_DIAG = 101
 PLA_NONUNIT_DIAG = 102
 PLA_SUCCESS = 0
 PLA_ALIGN_FIRST = -200
+PLA_PARAM_LOOKAHEAD = 1
+PLA_PARAM_LEAF_ROWS = 2
 
 _obj = ctypes.c_void_p
 _pobj = ctypes.POINTER(ctypes.c_void_p)
@@ -114,6 +116,7 @@ _SIGS = {
     "PLA_Obj_fill_drand48": [_obj, ctypes.c_long, ctypes.c_int64],
     "PLA_Device_synchronize": [],
     "PLA_Set_device_nb": [_i],
+    "PLA_Set_device_param": [_i, _i],
     "PLA_Get_device_nb": [_i, _pi],
     "PLA_Kernel_timing": [_i],
     "PLA_Kernel_timing_query": [_i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_d),
