@@ -136,7 +136,9 @@ int choose_nb(int n, int op) {
   if (c.nb_device > 0) return c.nb_device;
   if (c.nb_user > 0) return c.nb_user;
   (void)op;
-  if (n >= 8192) return 256;
+  // measured on MI355X, plain FP64 LU n=32768: nb 256 / 384 / 512 -> 25.8 / 27.1 / 27.9 TF/s
+  if (n >= 16384) return 512;
+  if (n >= 4096) return 256;
   return 128;
 }
 
