@@ -250,6 +250,26 @@ int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B);
 int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B);
 
 /* ---- EXTENSIONS (no counterpart in the reference) ---- */
+/* Mixed-precision factor-and-solve with FP64 iterative refinement.
+   A (n x n MPI_DOUBLE) is left unchanged; an FP32 copy is factored with
+   partial pivoting (pivots as PLA_LU's) by the blocked algorithm of
+   SOLVERS/LU/PLA_LU.c:37-212 with FP64 panels and a low-precision MFMA
+   trailing update (FP32 accumulation):
+     PLA_MIXED_FP16 / PLA_MIXED_BF16     one product of half-precision operands;
+     PLA_MIXED_FP16X3 / PLA_MIXED_BF16X3 operands split into hi + lo halves,
+                                          hi*hi + hi*lo + lo*hi.
+   X (n x nrhs, nrhs <= 16) receives the solution of A X = B, refined in FP64
+   until the normwise backward error
+     max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+   is <= tol or max_iter refinement steps were taken.  *iterations and
+   *backward_error (either may be NULL) report the outcome.  Returns
+   PLA_SUCCESS also when tol was not reached: check *backward_error. */
+#define PLA_MIXED_FP16   1
+#define PLA_MIXED_BF16   2
+#define PLA_MIXED_FP16X3 3
+#define PLA_MIXED_BF16X3 4
+int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision, double tol, int max_iter,
+                 int *iterations, double *backward_error);
 /* Message of the last error returned by any entry point. */
 const char *PLA_Last_error(void);
 /* Device pointer of the object's top-left element (HBM).  Drops the host

@@ -295,8 +295,9 @@ int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x) {
     return fail(PLA_ERR_INVALID_PARAM, "PLA_Trsv: invalid diag");
   double *a, *b;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(x, (void **)&b))) return rc;
-  if ((rc = trsm_left(ctx().stream, uplo == PLA_LOWER_TRIANGULAR, trans != PLA_NO_TRANSPOSE,
-                      diag == PLA_UNIT_DIAG, n, 1, a, ld(A), b, ld(x) > n ? ld(x) : n, nullptr)))
+  // one sync-free blocked solve (HBM-bound: each element of the triangle read once)
+  if ((rc = trsv_sf(ctx().stream, uplo == PLA_LOWER_TRIANGULAR, trans != PLA_NO_TRANSPOSE, diag == PLA_UNIT_DIAG,
+                    n, 1, a, ld(A), b, ld(x) > n ? ld(x) : n, nullptr)))
     return rc;
   return written(x);
 }

@@ -89,8 +89,13 @@ int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B) {
   double *a, *b;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(B, (void **)&b))) return rc;
   hipStream_t s = ctx().stream;
-  if ((rc = trsm_left(s, true, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
-  if ((rc = trsm_left(s, true, true, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  if (B->width <= trsv_max_rhs()) {
+    if ((rc = trsv_sf(s, true, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = trsv_sf(s, true, true, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  } else {
+    if ((rc = trsm_left(s, true, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = trsm_left(s, true, true, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  }
   return written(B);
 }
 

@@ -58,10 +58,10 @@ struct ws_slot {
   void *p = nullptr;
   size_t bytes = 0;
 };
-static ws_slot g_ws[8];
+static ws_slot g_ws[PLA_NUM_WS];
 
-void *workspace(size_t bytes, int slot) {
-  if (slot < 0 || slot >= 8) return nullptr;
+void *workspace(size_t bytes, int slot, bool zero) {
+  if (slot < 0 || slot >= PLA_NUM_WS) return nullptr;
   ws_slot &w = g_ws[slot];
   if (w.bytes >= bytes && w.p) return w.p;
   if (w.p) {
@@ -75,6 +75,12 @@ void *workspace(size_t bytes, int slot) {
   if (hipMalloc(&w.p, want) != hipSuccess) {
     w.p = nullptr;
     fail(PLA_ERR_NO_MEMORY, "workspace allocation of %zu bytes failed", want);
+    return nullptr;
+  }
+  if (zero && hipMemset(w.p, 0, want) != hipSuccess) {
+    (void)hipFree(w.p);
+    w.p = nullptr;
+    fail(PLA_ERR_DEVICE, "workspace initialisation failed");
     return nullptr;
   }
   w.bytes = want;
@@ -106,6 +112,11 @@ leaf_ws &leaf_workspace(size_t slot_bytes) {
   g_leaf.slots = (unsigned long long *)p;
   g_leaf.err = (int *)((char *)p + slot_bytes);
   return g_leaf;
+}
+
+int *leaf_error_word() {
+  leaf_ws &w = leaf_workspace(g_leaf.base ? g_leaf.slot_bytes : 4096);
+  return w.err;
 }
 
 int leaf_check_error(hipStream_t s) {

@@ -71,8 +71,10 @@ struct context {
 context &ctx();
 int ensure_device();                 // lazily initialise device + streams
 
-// Grow-only device workspace (reused across calls, freed at PLA_Finalize).
-void *workspace(size_t bytes, int slot);
+// Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
+// `zero`: clear the memory when it is (re)allocated.
+constexpr int PLA_NUM_WS = 24;
+void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
 
 // ---- object helpers ----
@@ -103,6 +105,8 @@ struct leaf_ws {
 leaf_ws &leaf_workspace(size_t slot_bytes);
 // Returns PLA_ERR_DEVICE if any panel exchange timed out since the last call.
 int leaf_check_error(hipStream_t s);
+// The device error word shared by the bounded waits of every kernel.
+int *leaf_error_word();
 
 // ---- kernel launchers (pla_kernels.hip); all asynchronous on stream s ----
 int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, double alpha,
@@ -141,6 +145,33 @@ int leaf_pairs_stride();
 int k_norm(hipStream_t s, bool inf_norm, int m, int n, const double *A, long lda, double *result);
 int leaf_max_width();
 int trsm_leaf_size();
+
+// ---- mixed precision and FP64-accurate solves (pla_mixed.hip) ----
+int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *pairs,
+                     int nleaves);
+int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long lda,
+            const void *Bh, const void *Bl, long ldb, float *C, long ldc);
+int k_cvt_f64_f32(hipStream_t s, int m, int n, const double *A, long lda, float *W, long ldw);
+int k_cvt_f32_f64(hipStream_t s, int m, int n, const float *W, long ldw, double *P, long ldp);
+int k_panel_out(hipStream_t s, int prec, int rows, int b, const double *P, long ldp, void *Lh, void *Ll,
+                int ldl);
+int k_upanel_out(hipStream_t s, int prec, int b, int nr, const double *U, long ldu, float *W, long ldw,
+                 void *Uh, void *Ul, int ldh);
+// Sync-free blocked triangular solve op(A) X = X, X n x nrhs (nrhs <= 16).
+// Dinv: inverted diagonal blocks (nullptr: computed into a workspace).
+int trsv_sf(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *A, long lda,
+            double *X, long ldx, double *Dinv);
+int trsv_sf_f32(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const float *A, long lda,
+                double *X, long ldx, double *Dinv);
+int k_inv_diag_f32(hipStream_t s, bool lower, bool unit, int n, const float *A, long lda, double *Dinv);
+int trsv_block_size();
+int trsv_max_rhs();
+int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *X, long ldx,
+             const double *B, long ldb, double *R, long ldr, double *rmax);
+int col_amax(hipStream_t s, int n, int nrhs, const double *X, long ldx, double *out);
+int k_permute_rows(hipStream_t s, int n, int nrhs, const int *perm, const double *X, long ldx, double *Y,
+                   long ldy);
+int k_axpy_cols(hipStream_t s, int n, int nrhs, const double *D, long ldd, double *X, long ldx);
 
 // ---- blocked/recursive building blocks (pla_blas.cpp) ----
 // B <- op(T)^{-1} B, T n x n triangular, B n x nrhs.

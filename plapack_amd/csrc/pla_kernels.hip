@@ -487,9 +487,9 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
 // Full-width interchanges of one panel: for every column outside the panel
 // (ranges [0, nleft) and [c1, c2)), apply the leaves' (dst, src) row pairs
 // leaf by leaf, one column per wave.  A is the panel's first row.
-__global__ __launch_bounds__(256) void panel_swap_kernel(double *__restrict__ A, long lda, int nleft, int c1,
-                                                         int c2, const int *__restrict__ pairs,
-                                                         int nleaves) {
+template <typename T>
+__global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long lda, int nleft, int c1, int c2,
+                                                         const int *__restrict__ pairs, int nleaves) {
   __shared__ int s_pairs[16][PAIRS_STRIDE];
   for (int e = threadIdx.x; e < nleaves * PAIRS_STRIDE; e += 256) {
     const int q = e / PAIRS_STRIDE, k = e - q * PAIRS_STRIDE;
@@ -500,11 +500,11 @@ __global__ __launch_bounds__(256) void panel_swap_kernel(double *__restrict__ A,
   const int lane = threadIdx.x & 63;
   for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < total; q += gridDim.x * 4) {
     const int col = q < nleft ? q : c1 + (q - nleft);
-    double *cp = A + (long)col * lda;
+    T *cp = A + (long)col * lda;
     for (int l = 0; l < nleaves; l++) {
       const int cnt = s_pairs[l][0];
       if (lane < cnt) {
-        const double v = ld_dev(cp + s_pairs[l][2 + 2 * lane]);
+        const T v = __hip_atomic_load(cp + s_pairs[l][2 + 2 * lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         cp[s_pairs[l][1 + 2 * lane]] = v;
       }
       // the next leaf's loads (device scope, past the L1) must see these stores
@@ -985,8 +985,19 @@ int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, 
   if (total <= 0 || nleaves <= 0) return PLA_SUCCESS;
   if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel swap: %d leaves", nleaves);
   const int blocks = (total + 3) / 4;
-  hipLaunchKernelGGL(panel_swap_kernel, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1, c2 > c1 ? c2 : c1,
-                     pairs, nleaves);
+  hipLaunchKernelGGL(panel_swap_kernel<double>, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1,
+                     c2 > c1 ? c2 : c1, pairs, nleaves);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *pairs,
+                     int nleaves) {
+  const int total = nleft + (c2 > c1 ? c2 - c1 : 0);
+  if (total <= 0 || nleaves <= 0) return PLA_SUCCESS;
+  if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel swap: %d leaves", nleaves);
+  hipLaunchKernelGGL(panel_swap_kernel<float>, dim3((total + 3) / 4), dim3(256), 0, s, A, lda, nleft, c1,
+                     c2 > c1 ? c2 : c1, pairs, nleaves);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }

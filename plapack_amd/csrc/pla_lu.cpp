@@ -143,8 +143,13 @@ int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
     return rc;
   hipStream_t s = ctx().stream;
   if ((rc = k_laswp(s, b, ld(B), n, B->width, n, piv))) return rc;
-  if ((rc = trsm_left(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
-  if ((rc = trsm_left(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  if (B->width <= trsv_max_rhs()) {
+    if ((rc = trsv_sf(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = trsv_sf(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  } else {
+    if ((rc = trsm_left(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = trsm_left(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+  }
   return written(B);
 }
 

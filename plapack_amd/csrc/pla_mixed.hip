@@ -1,0 +1,693 @@
+// pla_mixed.hip -- mixed-precision LU with FP64 iterative refinement, and the
+// FP64-accurate solve kernels it needs (gfx950).
+//
+// No reference counterpart: this is the precision mode added beside PLA_LU
+// (SOLVERS/LU/PLA_LU.c:13) and PLA_General_solve (SOLVERS/LU/PLA_General.c:67).
+//
+// Factorization.  A (FP64) is copied to an FP32 working matrix W.  The
+// blocked right-looking LU of PLA_LU.c:37-212 runs on W with one step of
+// look-ahead; per step:
+//   panel   W[k:, k:k+b] -> FP64, lu_panel (the FP64 leaf kernels, partial
+//           pivoting identical to PLA_LU), back to W, and a low-precision
+//           row-major copy of L21;
+//   rows    the panel's interchanges on the rest of W (panel_swap, FP32);
+//   U12     W12 -> FP64, inv(L11) W12 by the FP64 recursive trsm, back to W,
+//           and a low-precision column-major copy of U12;
+//   update  W22 -= L21 U12 by hgemm_kernel: FP16 or BF16 MFMA
+//           (v_mfma_f32_16x16x32_{f16,bf16}) with FP32 accumulation;
+//           FP16x3 splits both operands into hi + lo halves and adds
+//           hi*hi + hi*lo + lo*hi (about 22 significant bits).
+// Refinement.  x0 = U\L\(P b) with the FP32 factors in FP64 arithmetic; then
+// r = b - A x in FP64 (chunked sums combined with compensated addition),
+// eta = ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf), d = U\L\(P r), x += d,
+// until eta <= tol or max_iter.
+//
+// Triangular solves (trsv_sf_kernel, also used by PLA_Trsv / the solves of
+// the plain path for up to 16 right-hand sides).  One workgroup per 64-row
+// block in solve order.  Each workgroup accumulates the contributions of the
+// earlier blocks as their solutions appear, then multiplies by the inverted
+// diagonal block.  Workgroups publish their solution blocks as tagged 64-bit
+// words (tag | 32-bit half), so readers need no flags or fences.  A
+// workgroup waits only on workgroups with smaller ids, which were
+// dispatched before it.
+#include "pla_internal.hpp"
+
+#include <climits>
+#include <cstdint>
+#include <vector>
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bhalf8 __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename H>
+struct hvec;
+template <>
+struct hvec<_Float16> {
+  typedef half8 type;
+};
+template <>
+struct hvec<__bf16> {
+  typedef bhalf8 type;
+};
+
+__device__ inline f4v mfma16(half8 a, half8 b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ inline f4v mfma16(bhalf8 a, bhalf8 b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// C (m x n FP32, column-major) -= A (m x k) * B (k x n), A and B in FP16/BF16
+// with k contiguous: row i of A at Ah + i*lda, column j of B at Bh + j*ldb.
+// k is a multiple of 32; lda, ldb multiples of 8.  128 x 128 block tile, 4
+// waves of 64 x 64 as 4 x 4 MFMA 16x16x32 tiles, K staged 32 at a time
+// through double-buffered LDS (rows padded to 40 elements: 16-byte lane
+// reads are bank-conflict free).  MFMA 16x16x32 layout (lane l): A row l%16,
+// k = 8*(l/16)..+7; B column l%16, same k; result lane l holds
+// D[4*(l/16) + r][l%16].
+constexpr int HBM = 128, HBN = 128, HKS = 32, HPAD = 8;
+
+template <typename H, bool SPLIT>
+__global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, const H *__restrict__ Ah,
+                                                       const H *__restrict__ Al, long lda,
+                                                       const H *__restrict__ Bh, const H *__restrict__ Bl,
+                                                       long ldb, float *__restrict__ C, long ldc, int tiles_m,
+                                                       int tiles_n, int vec_ok) {
+  typedef typename hvec<H>::type V;
+  constexpr int NS = SPLIT ? 2 : 1;
+  __shared__ H As[2][NS][HBM][HKS + HPAD];
+  __shared__ H Bs[2][NS][HBN][HKS + HPAD];
+
+  const int ntiles = tiles_m * tiles_n;
+  const int nblk = gridDim.x;  // a multiple of 8
+  const int bid = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+  if (bid >= ntiles) return;
+  constexpr int GROUP = 8;
+  const int group = bid / (GROUP * tiles_n);
+  const int first_m = group * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int within = bid - group * GROUP * tiles_n;
+  const int i0 = (first_m + within % gsz) * HBM, j0 = (within / gsz) * HBN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lq = lane >> 4;
+
+  f4v acc[4][4];
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++) acc[bi][bj] = f4v{0.f, 0.f, 0.f, 0.f};
+
+  V ra[NS][2], rb[NS][2];
+  const V zero = V{};
+  auto load_tile = [&](int kt) {
+    const int kb = kt * HKS;
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int c = tid + 256 * e, row = c >> 2, kc = c & 3;
+      const int gi = i0 + row, gj = j0 + row;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        const H *Ap = s ? Al : Ah;
+        const H *Bp = s ? Bl : Bh;
+        ra[s][e] = gi < m ? *(const V *)(Ap + (long)gi * lda + kb + kc * 8) : zero;
+        rb[s][e] = gj < n ? *(const V *)(Bp + (long)gj * ldb + kb + kc * 8) : zero;
+      }
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const int c = tid + 256 * e, row = c >> 2, kc = c & 3;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        *(V *)&As[buf][s][row][kc * 8] = ra[s][e];
+        *(V *)&Bs[buf][s][row][kc * 8] = rb[s][e];
+      }
+    }
+  };
+
+  const int nk = k / HKS;
+  if (nk > 0) {
+    load_tile(0);
+    store_tile(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tile(kt + 1);
+    V af[NS][4], bf[NS][4];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+#pragma unroll
+      for (int bi = 0; bi < 4; bi++) af[s][bi] = *(const V *)&As[cur][s][wm * 64 + bi * 16 + l16][lq * 8];
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) bf[s][bj] = *(const V *)&Bs[cur][s][wn * 64 + bj * 16 + l16][lq * 8];
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) {
+        acc[bi][bj] = mfma16(af[0][bi], bf[0][bj], acc[bi][bj]);
+        if (SPLIT) {
+          acc[bi][bj] = mfma16(af[0][bi], bf[NS - 1][bj], acc[bi][bj]);
+          acc[bi][bj] = mfma16(af[NS - 1][bi], bf[0][bj], acc[bi][bj]);
+        }
+      }
+    if (kt + 1 < nk) store_tile(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++) {
+      const int row0 = i0 + wm * 64 + bi * 16 + lq * 4;
+      const int col = j0 + wn * 64 + bj * 16 + l16;
+      if (col >= n) continue;
+      float *cp = C + (long)col * ldc + row0;
+      if (vec_ok && row0 + 3 < m) {
+        f4v c4 = *(f4v *)cp;
+        c4 -= acc[bi][bj];
+        *(f4v *)cp = c4;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (row0 + r < m) cp[r] -= acc[bi][bj][r];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Precision conversions.
+__global__ __launch_bounds__(256) void cvt_f64_f32_kernel(int m, int n, const double *__restrict__ A, long lda,
+                                                          float *__restrict__ W, long ldw) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int j = (int)(e / m), i = (int)(e - (long)j * m);
+    W[(long)j * ldw + i] = (float)A[(long)j * lda + i];
+  }
+}
+
+__global__ __launch_bounds__(256) void cvt_f32_f64_kernel(int m, int n, const float *__restrict__ W, long ldw,
+                                                          double *__restrict__ P, long ldp) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int j = (int)(e / m), i = (int)(e - (long)j * m);
+    P[(long)j * ldp + i] = (double)W[(long)j * ldw + i];
+  }
+}
+
+template <typename H, bool SPLIT>
+__device__ inline void to_low(float v, H *hi, H *lo, long idx) {
+  const H h = (H)v;
+  hi[idx] = h;
+  if (SPLIT) lo[idx] = (H)(v - (float)h);
+}
+
+// Rows of a factored panel below its diagonal block (P: FP64, rows x b, ldp)
+// as a row-major low-precision copy Lh[i*ldl + c], rounded through FP32 as
+// stored in W; columns b..ldl-1 are zero.
+template <typename H, bool SPLIT>
+__global__ __launch_bounds__(256) void panel_out_kernel(int rows, int b, const double *__restrict__ P, long ldp,
+                                                        H *__restrict__ Lh, H *__restrict__ Ll, int ldl) {
+  const long total = (long)rows * ldl;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = (int)(e / ldl), c = (int)(e - (long)i * ldl);
+    const float v = c < b ? (float)P[(long)c * ldp + i] : 0.f;
+    to_low<H, SPLIT>(v, Lh, Ll, (long)i * ldl + c);
+  }
+}
+
+// U12 (FP64, b x nr, ldu) back into W (FP32) and a column-major low-precision
+// copy Uh[j*ldh + r] (rows b..ldh-1 zero).
+template <typename H, bool SPLIT>
+__global__ __launch_bounds__(256) void upanel_out_kernel(int b, int nr, const double *__restrict__ U, long ldu,
+                                                         float *__restrict__ W, long ldw, H *__restrict__ Uh,
+                                                         H *__restrict__ Ul, int ldh) {
+  const long total = (long)nr * ldh;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int j = (int)(e / ldh), r = (int)(e - (long)j * ldh);
+    float v = 0.f;
+    if (r < b) {
+      v = (float)U[(long)j * ldu + r];
+      W[(long)j * ldw + r] = v;
+    }
+    to_low<H, SPLIT>(v, Uh, Ul, (long)j * ldh + r);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves.  Effective matrix M = op(A) (TRANS: A^T); it is lower
+// triangular when LOWER != TRANS.  TB-row blocks.
+constexpr int TB = 64;
+constexpr int MAXRHS = 16;
+
+template <typename T, bool LOWER, bool TRANS>
+__device__ inline double op_elem(const T *A, long lda, int r, int c) {
+  return TRANS ? (double)A[(long)r * lda + c] : (double)A[(long)c * lda + r];
+}
+
+// Inverse of every diagonal block of M, as FP64 column-major TB x TB blocks.
+template <typename T, bool LOWER, bool TRANS, bool UNIT>
+__global__ __launch_bounds__(TB) void inv_diag_kernel(int n, const T *__restrict__ A, long lda,
+                                                      double *__restrict__ Dinv) {
+  constexpr bool LOWEFF = (LOWER != TRANS);
+  __shared__ double M[TB][TB + 1];
+  __shared__ double X[TB][TB + 1];  // X[i][c]: column c of the inverse
+  const int blk = blockIdx.x, r0 = blk * TB, bs = min(TB, n - r0);
+  const int c = threadIdx.x;
+  for (int i = 0; i < TB; i++) {
+    double v = 0.0;
+    if (i < bs && c < bs) {
+      const bool in_tri = LOWEFF ? (c <= i) : (c >= i);
+      if (in_tri) v = (UNIT && c == i) ? 1.0 : op_elem<T, LOWER, TRANS>(A, lda, r0 + i, r0 + c);
+    }
+    M[i][c] = (i < bs && c < bs) ? v : (i == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  // column c of the inverse: M x = e_c
+  if (LOWEFF) {
+    for (int i = 0; i < TB; i++) {
+      double t = (i == c) ? 1.0 : 0.0;
+      if (i > c)
+        for (int q = c; q < i; q++) t = fma(-M[i][q], X[q][c], t);
+      X[i][c] = (i < c) ? 0.0 : t / M[i][i];
+    }
+  } else {
+    for (int i = TB - 1; i >= 0; i--) {
+      double t = (i == c) ? 1.0 : 0.0;
+      if (i < c)
+        for (int q = i + 1; q <= c; q++) t = fma(-M[i][q], X[q][c], t);
+      X[i][c] = (i > c) ? 0.0 : t / M[i][i];
+    }
+  }
+  __syncthreads();
+  double *D = Dinv + (size_t)blk * TB * TB;
+  for (int i = 0; i < TB; i++) D[(size_t)c * TB + i] = X[i][c];
+}
+
+__device__ inline void st_dev64(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long ld_dev64(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Solve M X = X in place (n x nrhs, nrhs <= 16).  xs: exchange words
+// [nblk][TB][MAXRHS][2]; tag unique per launch.
+template <typename T, bool LOWER, bool TRANS>
+__global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *__restrict__ A, long lda,
+                                                      const double *__restrict__ Dinv, double *__restrict__ X,
+                                                      long ldx, unsigned long long *__restrict__ xs, unsigned tag32,
+                                                      unsigned spin_limit, int *__restrict__ err) {
+  constexpr bool LOWEFF = (LOWER != TRANS);
+  __shared__ double tile[TB][TB + 1];
+  __shared__ double accs[TB][MAXRHS];
+  __shared__ double xb[TB][MAXRHS];
+  const int nblk = (n + TB - 1) / TB;
+  const int b = blockIdx.x;
+  const int blk = LOWEFF ? b : nblk - 1 - b;
+  const int r0 = blk * TB, bs = min(TB, n - r0);
+  const int tid = threadIdx.x;
+  const unsigned long long tag = (unsigned long long)tag32 << 32;
+  const int row = tid & 63, g = tid >> 6;  // row of the block, right-hand-side group
+
+  for (int e = tid; e < TB * MAXRHS; e += 256) {
+    const int i = e / MAXRHS, q = e - i * MAXRHS;
+    accs[i][q] = (i < bs && q < nrhs) ? X[(long)q * ldx + r0 + i] : 0.0;
+  }
+  bool failed = false;
+  for (int s = 0; s < b; s++) {
+    const int iblk = LOWEFF ? s : nblk - 1 - s;
+    const int c0 = iblk * TB, cs = min(TB, n - c0);
+    __syncthreads();  // previous tile and xb consumed
+    // M[r0 + i][c0 + c]; TRANS reads along c, NoTrans along i (both contiguous)
+    for (int e = tid; e < TB * TB; e += 256) {
+      int i, c;
+      if (TRANS) { c = e & 63; i = e >> 6; }
+      else { i = e & 63; c = e >> 6; }
+      tile[i][c] = (i < bs && c < cs) ? op_elem<T, LOWER, TRANS>(A, lda, r0 + i, c0 + c) : 0.0;
+    }
+    // block iblk's solution (tagged words)
+    const unsigned long long *src = xs + (size_t)iblk * TB * MAXRHS * 2;
+    unsigned it = 0;
+    for (;;) {
+      int ok = 1;
+      for (int e = tid; e < cs * nrhs; e += 256) {
+        const int c = e / nrhs, q = e - c * nrhs;
+        const unsigned long long lo = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2);
+        const unsigned long long hi = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2 + 1);
+        if ((lo & 0xFFFFFFFF00000000ULL) != tag || (hi & 0xFFFFFFFF00000000ULL) != tag) ok = 0;
+        xb[c][q] = __longlong_as_double((long long)(((hi & 0xFFFFFFFFULL) << 32) | (lo & 0xFFFFFFFFULL)));
+      }
+      if (__syncthreads_and(ok)) break;
+      ++it;
+      const int bail = (it > spin_limit) || (tid == 0 && (it & 63) == 0 &&
+                                               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (__syncthreads_or(bail)) {
+        failed = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (failed) break;
+    if (row < bs) {
+      for (int q = g; q < nrhs; q += 4) {
+        double t = accs[row][q];
+        for (int c = 0; c < cs; c++) t = fma(-tile[row][c], xb[c][q], t);
+        accs[row][q] = t;
+      }
+    }
+  }
+  if (failed && tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  // x_blk = inv(M_bb) acc
+  const double *D = Dinv + (size_t)blk * TB * TB;
+  unsigned long long *dst = xs + (size_t)blk * TB * MAXRHS * 2;
+  if (row < bs) {
+    for (int q = g; q < nrhs; q += 4) {
+      double t = 0.0;
+      for (int c = 0; c < bs; c++) t = fma(D[(size_t)c * TB + row], accs[c][q], t);
+      X[(long)q * ldx + r0 + row] = t;
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
+      st_dev64(dst + ((size_t)row * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
+      st_dev64(dst + ((size_t)row * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Residual R = B - A X (FP64, nrhs <= 16): per 512-column chunk a plain FMA
+// sum, then the chunks combined in order with compensated (Neumaier)
+// addition; max |R| per right-hand side into rmax (as ordered bits).
+constexpr int RCH = 512;
+
+__global__ __launch_bounds__(256) void resid_partial_kernel(int n, int nrhs, const double *__restrict__ A,
+                                                            long lda, const double *__restrict__ X, long ldx,
+                                                            double *__restrict__ part) {
+  __shared__ double xs[RCH][MAXRHS];
+  const int ch = blockIdx.y, j0 = ch * RCH, j1 = min(n, j0 + RCH);
+  for (int e = threadIdx.x; e < RCH * MAXRHS; e += 256) {
+    const int j = e / MAXRHS, q = e - j * MAXRHS;
+    xs[j][q] = (j0 + j < j1 && q < nrhs) ? X[(long)q * ldx + j0 + j] : 0.0;
+  }
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s[MAXRHS];
+#pragma unroll
+  for (int q = 0; q < MAXRHS; q++) s[q] = 0.0;
+  for (int j = j0; j < j1; j++) {
+    const double a = A[(long)j * lda + i];
+#pragma unroll
+    for (int q = 0; q < MAXRHS; q++)
+      if (q < nrhs) s[q] = fma(a, xs[j - j0][q], s[q]);
+  }
+  for (int q = 0; q < nrhs; q++) part[((size_t)ch * nrhs + q) * n + i] = s[q];
+}
+
+__global__ __launch_bounds__(256) void resid_final_kernel(int n, int nrhs, int nch, const double *__restrict__ part,
+                                                          const double *__restrict__ B, long ldb,
+                                                          double *__restrict__ R, long ldr,
+                                                          unsigned long long *__restrict__ rmax) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int q = blockIdx.y;
+  if (i >= n) return;
+  double s = 0.0, comp = 0.0;
+  for (int c = 0; c < nch; c++) {
+    const double v = part[((size_t)c * nrhs + q) * n + i];
+    const double t = s + v;
+    comp += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  const double bq = B[(long)q * ldb + i];
+  const double r = (bq - s) - comp;
+  R[(long)q * ldr + i] = r;
+  atomicMax(rmax + q, (unsigned long long)__double_as_longlong(fabs(r)));
+}
+
+// max |X(:, q)| per column into out (as ordered bits)
+__global__ __launch_bounds__(256) void col_amax_kernel(int n, const double *__restrict__ X, long ldx,
+                                                       unsigned long long *__restrict__ out) {
+  __shared__ double red[256];
+  const int q = blockIdx.y;
+  double m = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) m = fmax(m, fabs(X[(long)q * ldx + i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out + q, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// Y(i, q) = X(perm[i], q)
+__global__ __launch_bounds__(256) void permute_rows_kernel(int n, int nrhs, const int *__restrict__ perm,
+                                                           const double *__restrict__ X, long ldx,
+                                                           double *__restrict__ Y, long ldy) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int src = perm[i];
+  for (int q = 0; q < nrhs; q++) Y[(long)q * ldy + i] = X[(long)q * ldx + src];
+}
+
+// X += D
+__global__ __launch_bounds__(256) void axpy_cols_kernel(int n, int nrhs, const double *__restrict__ D, long ldd,
+                                                        double *__restrict__ X, long ldx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  for (int q = 0; q < nrhs; q++) X[(long)q * ldx + i] += D[(long)q * ldd + i];
+}
+
+inline unsigned grid1(long total) {
+  long g = (total + 1023) / 1024;
+  if (g < 1) g = 1;
+  if (g > 1L << 20) g = 1L << 20;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+namespace pla {
+
+// ---- launchers ---------------------------------------------------------------
+
+int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long lda,
+            const void *Bh, const void *Bl, long ldb, float *C, long ldc) {
+  if (m <= 0 || n <= 0 || k <= 0) return PLA_SUCCESS;
+  if (k % HKS || lda % 8 || ldb % 8) return fail(PLA_ERR_INVALID_PARAM, "hgemm: k %% 32 / ld %% 8");
+  const int tm = (m + HBM - 1) / HBM, tn = (n + HBN - 1) / HBN;
+  const int nblk = ((tm * tn + 7) / 8) * 8;
+  const int vec_ok = (ldc % 4 == 0) && (((uintptr_t)C & 15) == 0);
+  dim3 grid(nblk), blk(256);
+  switch (prec) {
+    case PLA_MIXED_FP16:
+      hipLaunchKernelGGL((hgemm_kernel<_Float16, false>), grid, blk, 0, s, m, n, k, (const _Float16 *)Ah,
+                         (const _Float16 *)Al, lda, (const _Float16 *)Bh, (const _Float16 *)Bl, ldb, C, ldc, tm,
+                         tn, vec_ok);
+      break;
+    case PLA_MIXED_FP16X3:
+      hipLaunchKernelGGL((hgemm_kernel<_Float16, true>), grid, blk, 0, s, m, n, k, (const _Float16 *)Ah,
+                         (const _Float16 *)Al, lda, (const _Float16 *)Bh, (const _Float16 *)Bl, ldb, C, ldc, tm,
+                         tn, vec_ok);
+      break;
+    case PLA_MIXED_BF16:
+      hipLaunchKernelGGL((hgemm_kernel<__bf16, false>), grid, blk, 0, s, m, n, k, (const __bf16 *)Ah,
+                         (const __bf16 *)Al, lda, (const __bf16 *)Bh, (const __bf16 *)Bl, ldb, C, ldc, tm, tn,
+                         vec_ok);
+      break;
+    case PLA_MIXED_BF16X3:
+      hipLaunchKernelGGL((hgemm_kernel<__bf16, true>), grid, blk, 0, s, m, n, k, (const __bf16 *)Ah,
+                         (const __bf16 *)Al, lda, (const __bf16 *)Bh, (const __bf16 *)Bl, ldb, C, ldc, tm, tn,
+                         vec_ok);
+      break;
+    default:
+      return fail(PLA_ERR_INVALID_PARAM, "hgemm: precision %d", prec);
+  }
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_cvt_f64_f32(hipStream_t s, int m, int n, const double *A, long lda, float *W, long ldw) {
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(cvt_f64_f32_kernel, dim3(grid1((long)m * n)), dim3(256), 0, s, m, n, A, lda, W, ldw);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_cvt_f32_f64(hipStream_t s, int m, int n, const float *W, long ldw, double *P, long ldp) {
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(cvt_f32_f64_kernel, dim3(grid1((long)m * n)), dim3(256), 0, s, m, n, W, ldw, P, ldp);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+static bool is_bf16(int prec) { return prec == PLA_MIXED_BF16 || prec == PLA_MIXED_BF16X3; }
+static bool is_split(int prec) { return prec == PLA_MIXED_FP16X3 || prec == PLA_MIXED_BF16X3; }
+
+int k_panel_out(hipStream_t s, int prec, int rows, int b, const double *P, long ldp, void *Lh, void *Ll,
+                int ldl) {
+  if (rows <= 0) return PLA_SUCCESS;
+  dim3 grid(grid1((long)rows * ldl)), blk(256);
+#define PLA_PO(H, SP) \
+  hipLaunchKernelGGL((panel_out_kernel<H, SP>), grid, blk, 0, s, rows, b, P, ldp, (H *)Lh, (H *)Ll, ldl)
+  if (is_bf16(prec)) {
+    if (is_split(prec)) PLA_PO(__bf16, true); else PLA_PO(__bf16, false);
+  } else {
+    if (is_split(prec)) PLA_PO(_Float16, true); else PLA_PO(_Float16, false);
+  }
+#undef PLA_PO
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_upanel_out(hipStream_t s, int prec, int b, int nr, const double *U, long ldu, float *W, long ldw, void *Uh,
+                 void *Ul, int ldh) {
+  if (nr <= 0) return PLA_SUCCESS;
+  dim3 grid(grid1((long)nr * ldh)), blk(256);
+#define PLA_UO(H, SP) \
+  hipLaunchKernelGGL((upanel_out_kernel<H, SP>), grid, blk, 0, s, b, nr, U, ldu, W, ldw, (H *)Uh, (H *)Ul, ldh)
+  if (is_bf16(prec)) {
+    if (is_split(prec)) PLA_UO(__bf16, true); else PLA_UO(__bf16, false);
+  } else {
+    if (is_split(prec)) PLA_UO(_Float16, true); else PLA_UO(_Float16, false);
+  }
+#undef PLA_UO
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+// ---- sync-free triangular solve --------------------------------------------
+
+static unsigned g_trsv_tag = 1;
+
+template <typename T>
+static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const T *A, long lda,
+                        double *X, long ldx, double *Dinv) {
+  if (n <= 0 || nrhs <= 0) return PLA_SUCCESS;
+  if (nrhs > MAXRHS) return fail(PLA_ERR_INVALID_PARAM, "trsv: %d right-hand sides (max %d)", nrhs, MAXRHS);
+  const int nblk = (n + TB - 1) / TB;
+  unsigned long long *xs =
+      (unsigned long long *)workspace(sizeof(unsigned long long) * nblk * TB * MAXRHS * 2, 5, true);
+  if (!xs) return PLA_ERR_NO_MEMORY;
+  int *err = leaf_error_word();
+  if (!Dinv) {
+    Dinv = (double *)workspace(sizeof(double) * (size_t)nblk * TB * TB, 6);
+    if (!Dinv) return PLA_ERR_NO_MEMORY;
+#define PLA_INV(LO, TR, UN) \
+  hipLaunchKernelGGL((inv_diag_kernel<T, LO, TR, UN>), dim3(nblk), dim3(TB), 0, s, n, A, lda, Dinv)
+    if (lower && !trans && unit) PLA_INV(true, false, true);
+    else if (lower && !trans && !unit) PLA_INV(true, false, false);
+    else if (lower && trans && unit) PLA_INV(true, true, true);
+    else if (lower && trans && !unit) PLA_INV(true, true, false);
+    else if (!lower && !trans && unit) PLA_INV(false, false, true);
+    else if (!lower && !trans && !unit) PLA_INV(false, false, false);
+    else if (!lower && trans && unit) PLA_INV(false, true, true);
+    else PLA_INV(false, true, false);
+#undef PLA_INV
+    PLA_HIP(hipGetLastError());
+  }
+  if (g_trsv_tag > 0xF0000000u) {
+    PLA_HIP(hipMemsetAsync(xs, 0, sizeof(unsigned long long) * nblk * TB * MAXRHS * 2, s));
+    g_trsv_tag = 1;
+  }
+  const unsigned tag = g_trsv_tag++;
+#define PLA_TSF(LO, TR)                                                                                  \
+  hipLaunchKernelGGL((trsv_sf_kernel<T, LO, TR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, ldx, \
+                     xs, tag, 2000000u, err)
+  if (lower && !trans) PLA_TSF(true, false);
+  else if (lower && trans) PLA_TSF(true, true);
+  else if (!lower && !trans) PLA_TSF(false, false);
+  else PLA_TSF(false, true);
+#undef PLA_TSF
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int trsv_sf(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *A, long lda, double *X,
+            long ldx, double *Dinv) {
+  return trsv_sf_impl<double>(s, lower, trans, unit, n, nrhs, A, lda, X, ldx, Dinv);
+}
+int trsv_sf_f32(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const float *A, long lda,
+                double *X, long ldx, double *Dinv) {
+  return trsv_sf_impl<float>(s, lower, trans, unit, n, nrhs, A, lda, X, ldx, Dinv);
+}
+
+int k_inv_diag_f32(hipStream_t s, bool lower, bool unit, int n, const float *A, long lda, double *Dinv) {
+  const int nblk = (n + TB - 1) / TB;
+  if (n <= 0) return PLA_SUCCESS;
+  if (lower && unit)
+    hipLaunchKernelGGL((inv_diag_kernel<float, true, false, true>), dim3(nblk), dim3(TB), 0, s, n, A, lda, Dinv);
+  else if (lower)
+    hipLaunchKernelGGL((inv_diag_kernel<float, true, false, false>), dim3(nblk), dim3(TB), 0, s, n, A, lda, Dinv);
+  else if (unit)
+    hipLaunchKernelGGL((inv_diag_kernel<float, false, false, true>), dim3(nblk), dim3(TB), 0, s, n, A, lda, Dinv);
+  else
+    hipLaunchKernelGGL((inv_diag_kernel<float, false, false, false>), dim3(nblk), dim3(TB), 0, s, n, A, lda, Dinv);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int trsv_block_size() { return TB; }
+int trsv_max_rhs() { return MAXRHS; }
+
+// ---- residual and vector norms -----------------------------------------------
+
+// R = B - A X; rmax[q] = max |R(:, q)| (double).  Synchronous.
+int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *X, long ldx,
+             const double *B, long ldb, double *R, long ldr, double *rmax) {
+  if (nrhs > MAXRHS) return fail(PLA_ERR_INVALID_PARAM, "residual: %d right-hand sides", nrhs);
+  const int nch = (n + RCH - 1) / RCH;
+  double *part = (double *)workspace(sizeof(double) * (size_t)nch * nrhs * n + 256, 7);
+  unsigned long long *mx = (unsigned long long *)workspace(sizeof(unsigned long long) * MAXRHS, 8);
+  if (!part || !mx) return PLA_ERR_NO_MEMORY;
+  PLA_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long) * MAXRHS, s));
+  hipLaunchKernelGGL(resid_partial_kernel, dim3((n + 255) / 256, nch), dim3(256), 0, s, n, nrhs, A, lda, X, ldx,
+                     part);
+  hipLaunchKernelGGL(resid_final_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nrhs, nch, part, B, ldb,
+                     R, ldr, mx);
+  PLA_HIP(hipGetLastError());
+  unsigned long long h[MAXRHS];
+  PLA_HIP(hipMemcpyAsync(h, mx, sizeof(unsigned long long) * nrhs, hipMemcpyDeviceToHost, s));
+  PLA_HIP(hipStreamSynchronize(s));
+  for (int q = 0; q < nrhs; q++) rmax[q] = __builtin_bit_cast(double, h[q]);
+  return PLA_SUCCESS;
+}
+
+int col_amax(hipStream_t s, int n, int nrhs, const double *X, long ldx, double *out) {
+  if (nrhs > MAXRHS) return fail(PLA_ERR_INVALID_PARAM, "col_amax: %d columns", nrhs);
+  unsigned long long *mx = (unsigned long long *)workspace(sizeof(unsigned long long) * MAXRHS, 9);
+  if (!mx) return PLA_ERR_NO_MEMORY;
+  PLA_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long) * MAXRHS, s));
+  int gx = (n + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(col_amax_kernel, dim3(gx, nrhs), dim3(256), 0, s, n, X, ldx, mx);
+  PLA_HIP(hipGetLastError());
+  unsigned long long h[MAXRHS];
+  PLA_HIP(hipMemcpyAsync(h, mx, sizeof(unsigned long long) * nrhs, hipMemcpyDeviceToHost, s));
+  PLA_HIP(hipStreamSynchronize(s));
+  for (int q = 0; q < nrhs; q++) out[q] = __builtin_bit_cast(double, h[q]);
+  return PLA_SUCCESS;
+}
+
+int k_permute_rows(hipStream_t s, int n, int nrhs, const int *perm, const double *X, long ldx, double *Y,
+                   long ldy) {
+  hipLaunchKernelGGL(permute_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, nrhs, perm, X, ldx, Y, ldy);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_axpy_cols(hipStream_t s, int n, int nrhs, const double *D, long ldd, double *X, long ldx) {
+  hipLaunchKernelGGL(axpy_cols_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, nrhs, D, ldd, X, ldx);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+}  // namespace pla

@@ -1,0 +1,204 @@
+// pla_mixed_lu.cpp -- PLA_LU_mixed: LU in FP32 storage with a low-precision
+// MFMA trailing update, then FP64 iterative refinement (extension; no
+// reference counterpart).  The kernels and the scheme are described in
+// pla_mixed.hip.
+#include "pla_internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+using namespace pla;
+
+namespace pla {
+
+static bool prec_split(int prec) { return prec == PLA_MIXED_FP16X3 || prec == PLA_MIXED_BF16X3; }
+
+// Workspace slots used here (the others belong to pla_kernels / pla_mixed).
+enum { WS_W = 10, WS_P64, WS_U64, WS_LH, WS_LL, WS_UH, WS_UL, WS_PERM, WS_DINVL, WS_DINVU, WS_D, WS_R, WS_PAIRS };
+
+// Blocked right-looking LU of the n x n FP32 matrix W (ldw) with one step
+// of look-ahead, as lu_factor (pla_lu.cpp) but with FP64 panels and
+// triangular solves and a low-precision trailing update.
+int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *piv, int nb) {
+  hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
+  nb = (nb / 32) * 32;
+  if (nb < 32) nb = 32;
+  if (nb > 512) nb = 512;
+  const int nbp = nb;  // low-precision copies are padded to nb columns/rows (a multiple of 32)
+  const int esz = 2;   // bytes of one FP16 / BF16 element
+  const bool split = prec_split(prec);
+  const int stride = leaf_pairs_stride();
+  double *P64 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_P64);
+  double *U64 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_U64);
+  char *Lh = (char *)workspace((size_t)2 * n * nbp * esz, WS_LH);
+  char *Ll = split ? (char *)workspace((size_t)2 * n * nbp * esz, WS_LL) : nullptr;
+  char *Uh = (char *)workspace((size_t)n * nbp * esz, WS_UH);
+  char *Ul = split ? (char *)workspace((size_t)n * nbp * esz, WS_UL) : nullptr;
+  int *pairs = (int *)workspace(sizeof(int) * 2 * 16 * stride, WS_PAIRS);
+  if (!P64 || !U64 || !Lh || !Uh || !pairs || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
+  const size_t lbuf = (size_t)n * nbp * esz;  // one parity of the L copies
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  if (!ev_ready) {
+    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+  }
+  int nleaves[2] = {0, 0};
+  int rc;
+  // panel at column k, on the look-ahead stream
+  auto panel = [&](int k, int b, int par) -> int {
+    const int mp = n - k;
+    float *Wk = W + (long)k * ldw + k;
+    int r;
+    if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
+    if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + (size_t)par * 16 * stride, &nleaves[par]))) return r;
+    if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
+    return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
+  };
+  {
+    const int b = n < nb ? n : nb;
+    PLA_HIP(hipEventRecord(ev_ready, sM));
+    PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+    if ((rc = panel(0, b, 0))) return rc;
+    PLA_HIP(hipEventRecord(ev_panel, sP));
+  }
+  for (int k = 0; k < n; k += nb) {
+    const int b = n - k < nb ? n - k : nb;
+    const int par = (k / nb) & 1;
+    PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
+    if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, pairs + (size_t)par * 16 * stride, nleaves[par])))
+      return rc;
+    const int nr = n - k - b;
+    if (nr <= 0) continue;
+    float *W12 = W + (long)(k + b) * ldw + k;
+    // U12 = inv(L11) W12 in FP64 (L11: top of the FP64 panel, still in P64)
+    if ((rc = k_cvt_f32_f64(sM, b, nr, W12, ldw, U64, b))) return rc;
+    if ((rc = trsm_left(sM, true, false, true, b, nr, P64, n - k, U64, b, nullptr))) return rc;
+    if ((rc = k_upanel_out(sM, prec, b, nr, U64, b, W12, ldw, Uh, split ? Ul : nullptr, nbp))) return rc;
+    const int mr = nr;
+    const void *L_h = Lh + par * lbuf;
+    const void *L_l = split ? Ll + par * lbuf : nullptr;
+    const int kn = k + b;
+    const int bn = kn < n ? (n - kn < nb ? n - kn : nb) : 0;
+    int done = 0;
+    if (bn > 0) {
+      timing_begin(sM, 3);
+      rc = k_hgemm(sM, prec, mr, bn, nbp, L_h, L_l, nbp, Uh, split ? Ul : nullptr, nbp, W12 + b, ldw);
+      timing_end(sM, 3, 2.0 * mr * (double)bn * b);
+      if (rc) return rc;
+      done = bn;
+      PLA_HIP(hipEventRecord(ev_ready, sM));
+      PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+      if ((rc = panel(kn, bn, par ^ 1))) return rc;
+      PLA_HIP(hipEventRecord(ev_panel, sP));
+    }
+    if (nr - done > 0) {
+      timing_begin(sM, 3);
+      rc = k_hgemm(sM, prec, mr, nr - done, nbp, L_h, L_l, nbp, Uh + (size_t)done * nbp * esz,
+                   split ? Ul + (size_t)done * nbp * esz : nullptr, nbp, W12 + (long)done * ldw + b, ldw);
+      timing_end(sM, 3, 2.0 * mr * (double)(nr - done) * b);
+      if (rc) return rc;
+    }
+  }
+  return leaf_check_error(sM);
+}
+
+}  // namespace pla
+
+extern "C" {
+
+int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision, double tol, int max_iter,
+                 int *iterations, double *backward_error) {
+  if (!valid(A) || !valid(B) || !valid(X) || !valid(pivots))
+    return fail(PLA_ERR_INVALID_OBJECT, "PLA_LU_mixed: invalid object");
+  if (A->base->datatype != MPI_DOUBLE || B->base->datatype != MPI_DOUBLE || X->base->datatype != MPI_DOUBLE)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_LU_mixed: A, B and X must be MPI_DOUBLE");
+  if (pivots->base->datatype != MPI_INT)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_LU_mixed: pivots must be of type MPI_INT");
+  const int n = A->length, nrhs = B->width;
+  if (A->width != n || B->length != n || X->length != n || X->width != nrhs || pivots->length != n ||
+      pivots->width != 1)
+    return fail(PLA_ERR_DIMENSIONS, "PLA_LU_mixed: A %dx%d, B %dx%d, X %dx%d, pivots %dx%d", A->length, A->width,
+                B->length, B->width, X->length, X->width, pivots->length, pivots->width);
+  if (nrhs < 1 || nrhs > trsv_max_rhs())
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_LU_mixed: 1 to %d right-hand sides", trsv_max_rhs());
+  if (precision < PLA_MIXED_FP16 || precision > PLA_MIXED_BF16X3)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_LU_mixed: unknown precision %d", precision);
+  if (max_iter < 0) return fail(PLA_ERR_INVALID_PARAM, "PLA_LU_mixed: max_iter < 0");
+  if (n == 0) return PLA_SUCCESS;
+  hipStream_t s = ctx().stream;
+  double *a, *b, *x;
+  int *piv;
+  int rc;
+  if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(B, (void **)&b)) || (rc = dev_ptr(X, (void **)&x)) ||
+      (rc = dev_ptr(pivots, (void **)&piv)))
+    return rc;
+  const long lda = ld(A), ldb = ld(B), ldx = ld(X);
+
+  // factor a single-precision copy
+  const long ldw = n;
+  float *W = (float *)workspace(sizeof(float) * (size_t)n * n, WS_W);
+  if (!W) return PLA_ERR_NO_MEMORY;
+  if ((rc = k_cvt_f64_f32(s, n, n, a, lda, W, ldw))) return rc;
+  if ((rc = lu_factor_mixed(s, precision, n, W, ldw, piv, choose_nb(n, PLA_OP_PAN_PAN)))) return rc;
+
+  // row permutation of the interchanges (position i holds original row perm[i])
+  std::vector<int> hp(n), perm(n);
+  PLA_HIP(hipMemcpyAsync(hp.data(), piv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  PLA_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n; i++) perm[i] = i;
+  for (int j = 0; j < n; j++) {
+    const int p = j + hp[j];
+    if (p < 0 || p >= n) return fail(PLA_ERR_DEVICE, "PLA_LU_mixed: pivot %d out of range", j);
+    const int t = perm[j];
+    perm[j] = perm[p];
+    perm[p] = t;
+  }
+  const int nblk = (n + trsv_block_size() - 1) / trsv_block_size();
+  const size_t dsz = sizeof(double) * (size_t)nblk * trsv_block_size() * trsv_block_size();
+  int *dperm = (int *)workspace(sizeof(int) * n, WS_PERM);
+  double *DinvL = (double *)workspace(dsz, WS_DINVL);
+  double *DinvU = (double *)workspace(dsz, WS_DINVU);
+  double *D = (double *)workspace(sizeof(double) * (size_t)n * nrhs, WS_D);
+  double *R = (double *)workspace(sizeof(double) * (size_t)n * nrhs, WS_R);
+  if (!dperm || !DinvL || !DinvU || !D || !R) return PLA_ERR_NO_MEMORY;
+  PLA_HIP(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  if ((rc = k_inv_diag_f32(s, true, true, n, W, ldw, DinvL))) return rc;
+  if ((rc = k_inv_diag_f32(s, false, false, n, W, ldw, DinvU))) return rc;
+
+  auto solve = [&](const double *rhs, long ldr, double *out, long ldo) -> int {
+    int r;
+    if ((r = k_permute_rows(s, n, nrhs, dperm, rhs, ldr, out, ldo))) return r;
+    if ((r = trsv_sf_f32(s, true, false, true, n, nrhs, W, ldw, out, ldo, DinvL))) return r;
+    return trsv_sf_f32(s, false, false, false, n, nrhs, W, ldw, out, ldo, DinvU);
+  };
+
+  double anorm = 0.0;
+  if ((rc = k_norm(s, true, n, n, a, lda, &anorm))) return rc;
+  std::vector<double> bmax(nrhs), xmax(nrhs), rmax(nrhs);
+  if ((rc = col_amax(s, n, nrhs, b, ldb, bmax.data()))) return rc;
+  if ((rc = solve(b, ldb, x, ldx))) return rc;
+  int it = 0;
+  double eta = INFINITY;
+  for (;;) {
+    if ((rc = residual(s, n, nrhs, a, lda, x, ldx, b, ldb, R, n, rmax.data()))) return rc;
+    if ((rc = col_amax(s, n, nrhs, x, ldx, xmax.data()))) return rc;
+    eta = 0.0;
+    for (int q = 0; q < nrhs; q++) {
+      const double den = anorm * xmax[q] + bmax[q];
+      const double e = den > 0.0 ? rmax[q] / den : (rmax[q] > 0.0 ? INFINITY : 0.0);
+      if (!(e <= eta)) eta = e;  // NaN propagates
+    }
+    if (eta <= tol || it >= max_iter) break;
+    if ((rc = solve(R, n, D, n))) return rc;
+    if ((rc = k_axpy_cols(s, n, nrhs, D, n, x, ldx))) return rc;
+    it++;
+  }
+  PLA_HIP(hipStreamSynchronize(s));
+  if ((rc = leaf_check_error(s))) return rc;
+  if (iterations) *iterations = it;
+  if (backward_error) *backward_error = eta;
+  if ((rc = written(X))) return rc;
+  return written(pivots);
+}
+
+}  // extern "C"

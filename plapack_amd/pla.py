@@ -43,6 +43,10 @@ PLA_UNIT_DIAG = 101
 PLA_NONUNIT_DIAG = 102
 PLA_SUCCESS = 0
 PLA_ALIGN_FIRST = -200
+PLA_MIXED_FP16 = 1
+PLA_MIXED_BF16 = 2
+PLA_MIXED_FP16X3 = 3
+PLA_MIXED_BF16X3 = 4
 PLA_PARAM_LOOKAHEAD = 1
 PLA_PARAM_LEAF_ROWS = 2
 
@@ -111,6 +115,7 @@ _SIGS = {
     "PLA_Matrix_one_norm": [_obj, _obj],
     "PLA_General_solve": [_obj, _obj, _obj],
     "PLA_Pos_def_solve": [_obj, _obj],
+    "PLA_LU_mixed": [_obj, _obj, _obj, _obj, _i, _d, _i, _pi, ctypes.POINTER(_d)],
     "PLA_Last_error": [],
     "PLA_Obj_device_buffer": [_obj, _pobj],
     "PLA_Obj_fill_drand48": [_obj, ctypes.c_long, ctypes.c_int64],

@@ -120,18 +120,24 @@ def test_trsm_right_lower_trans_vs_oracle(pla_session, ora):
     pla.free(L, X, al)
 
 
-def test_trsv_vs_oracle(pla_session, ora):
+@pytest.mark.parametrize("uplo", [pla.PLA_LOWER_TRIANGULAR, pla.PLA_UPPER_TRIANGULAR])
+@pytest.mark.parametrize("trans", [pla.PLA_NO_TRANSPOSE, pla.PLA_TRANSPOSE])
+@pytest.mark.parametrize("diag", [pla.PLA_UNIT_DIAG, pla.PLA_NONUNIT_DIAG])
+@pytest.mark.parametrize("n", [1, 63, 333, 2000])
+def test_trsv_vs_oracle(pla_session, ora, uplo, trans, diag, n):
+    """The sync-free blocked solve (64-row blocks, inverted diagonal blocks)."""
     s = pla_session
-    rng = np.random.default_rng(4)
-    n = 333
+    rng = np.random.default_rng(4 + n)
     t = rng.uniform(-1, 1, (n, n)) / n + np.eye(n)
     x = rng.uniform(-1, 1, n)
     T = _obj(s, t)
     X = s.mvector(n)
     pla.set_contents(X, x)
-    pla.call("PLA_Trsv", pla.PLA_UPPER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG, T, X)
+    pla.call("PLA_Trsv", uplo, trans, diag, T, X)
     got = pla.get_contents(X)[:, 0]
-    ref = ora.trsm_left("U", "N", "N", t, x)[:, 0]
+    ref = ora.trsm_left("L" if uplo == pla.PLA_LOWER_TRIANGULAR else "U",
+                        "N" if trans == pla.PLA_NO_TRANSPOSE else "T",
+                        "U" if diag == pla.PLA_UNIT_DIAG else "N", t, x)[:, 0]
     assert np.abs(got - ref).max() <= 64 * n * EPS
     pla.free(T, X)
 
