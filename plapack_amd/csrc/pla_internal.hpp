@@ -136,19 +136,20 @@ struct lu_leaf_call {
   int *pairs;      // leaf_pairs_stride() ints: count, (dst, src) row pairs from the panel top
 };
 int k_lu_leaf(hipStream_t s, const lu_leaf_call &c);
-// rows of a panel (A = its first row): apply nleaves lists of (dst, src) pairs
-// to columns [0, nleft) and [c1, c2)
-int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, const int *pairs,
-                 int nleaves);
+// Compose the leaves' (dst, src) row lists of one panel into one list (comp,
+// panel_comp_stride() ints) and apply it to the rows of a panel (A = its first
+// row) in columns [0, nleft) and [c1, c2).
+int k_panel_compose(hipStream_t s, const int *pairs, int nleaves, int *comp);
+int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, const int *comp);
 int leaf_pairs_stride();
+int panel_comp_stride();
 // infinity norm (max absolute row sum) or one norm (max absolute column sum); synchronous
 int k_norm(hipStream_t s, bool inf_norm, int m, int n, const double *A, long lda, double *result);
 int leaf_max_width();
 int trsm_leaf_size();
 
 // ---- mixed precision and FP64-accurate solves (pla_mixed.hip) ----
-int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *pairs,
-                     int nleaves);
+int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *comp);
 int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long lda,
             const void *Bh, const void *Bl, long ldb, float *C, long ldc);
 int k_cvt_f64_f32(hipStream_t s, int m, int n, const double *A, long lda, float *W, long ldw);

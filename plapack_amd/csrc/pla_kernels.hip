@@ -484,31 +484,91 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
   }
 }
 
+// Composite interchange of one panel: the leaves' (dst, src) row lists,
+// applied one after the other, are replayed on a position -> source-row map
+// kept in an LDS hash table (one workgroup), and the positions whose source
+// changed become one list of at most 2*nb (dst, src) pairs: count, then pairs.
+constexpr int COMP_MAX = 1024;  // 2 * the largest panel width
+constexpr int HASH_SZ = 4096;
+
+__device__ inline int hslot(int key) { return (int)(((unsigned)key * 2654435761u) >> 20); }  // 12 bits
+
+__global__ __launch_bounds__(256) void panel_compose_kernel(const int *__restrict__ pairs, int nleaves,
+                                                            int *__restrict__ out) {
+  __shared__ int hk[HASH_SZ], hv[HASH_SZ];
+  __shared__ int val[PAIRS_MAX];
+  __shared__ int ncomp;
+  for (int e = threadIdx.x; e < HASH_SZ; e += 256) hk[e] = -1;
+  if (threadIdx.x == 0) ncomp = 0;
+  __syncthreads();
+  const int t = threadIdx.x;
+  for (int q = 0; q < nleaves; q++) {
+    const int *pl = pairs + (size_t)q * PAIRS_STRIDE;
+    const int cnt = pl[0];
+    if (t < cnt) {  // new[dst] = old[src]: read every source first
+      const int src = pl[2 + 2 * t];
+      int h = hslot(src), v = src;
+      for (int probe = 0; probe < HASH_SZ; probe++, h = (h + 1) & (HASH_SZ - 1)) {
+        const int key = hk[h];
+        if (key == src) { v = hv[h]; break; }
+        if (key == -1) break;
+      }
+      val[t] = v;
+    }
+    __syncthreads();
+    if (t < cnt) {  // then insert or update the destinations (distinct within a leaf)
+      const int dst = pl[1 + 2 * t];
+      int h = hslot(dst);
+      for (int probe = 0; probe < HASH_SZ; probe++, h = (h + 1) & (HASH_SZ - 1)) {
+        const int prev = atomicCAS(&hk[h], -1, dst);
+        if (prev == -1 || prev == dst) { hv[h] = val[t]; break; }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < HASH_SZ; e += 256) {
+    const int key = hk[e];
+    if (key >= 0 && hv[e] != key) {
+      const int slot = atomicAdd(&ncomp, 1);
+      if (slot < COMP_MAX) {
+        out[1 + 2 * slot] = key;
+        out[2 + 2 * slot] = hv[e];
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0) out[0] = min(ncomp, COMP_MAX);
+}
+
 // Full-width interchanges of one panel: for every column outside the panel
-// (ranges [0, nleft) and [c1, c2)), apply the leaves' (dst, src) row pairs
-// leaf by leaf, one column per wave.  A is the panel's first row.
+// (ranges [0, nleft) and [c1, c2)) apply the composite (dst, src) list, one
+// column per wave: every lane loads all its sources before any lane stores.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long lda, int nleft, int c1, int c2,
-                                                         const int *__restrict__ pairs, int nleaves) {
-  __shared__ int s_pairs[16][PAIRS_STRIDE];
-  for (int e = threadIdx.x; e < nleaves * PAIRS_STRIDE; e += 256) {
-    const int q = e / PAIRS_STRIDE, k = e - q * PAIRS_STRIDE;
-    s_pairs[q][k] = pairs[e];
+                                                         const int *__restrict__ comp) {
+  __shared__ int sd[COMP_MAX], ss[COMP_MAX];
+  const int cnt = comp[0];
+  for (int e = threadIdx.x; e < cnt; e += 256) {
+    sd[e] = comp[1 + 2 * e];
+    ss[e] = comp[2 + 2 * e];
   }
   __syncthreads();
   const int total = nleft + (c2 - c1);
   const int lane = threadIdx.x & 63;
+  constexpr int PER = COMP_MAX / 64;
   for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < total; q += gridDim.x * 4) {
     const int col = q < nleft ? q : c1 + (q - nleft);
     T *cp = A + (long)col * lda;
-    for (int l = 0; l < nleaves; l++) {
-      const int cnt = s_pairs[l][0];
-      if (lane < cnt) {
-        const T v = __hip_atomic_load(cp + s_pairs[l][2 + 2 * lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        cp[s_pairs[l][1 + 2 * lane]] = v;
-      }
-      // the next leaf's loads (device scope, past the L1) must see these stores
-      __builtin_amdgcn_s_waitcnt(0);
+    T v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int p = lane + 64 * i;
+      if (p < cnt) v[i] = cp[ss[p]];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int p = lane + 64 * i;
+      if (p < cnt) cp[sd[p]] = v[i];
     }
   }
 }
@@ -979,28 +1039,36 @@ int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
   return PLA_SUCCESS;
 }
 
-int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, const int *pairs,
-                 int nleaves) {
-  const int total = nleft + (c2 > c1 ? c2 - c1 : 0);
-  if (total <= 0 || nleaves <= 0) return PLA_SUCCESS;
-  if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel swap: %d leaves", nleaves);
-  const int blocks = (total + 3) / 4;
-  hipLaunchKernelGGL(panel_swap_kernel<double>, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1,
-                     c2 > c1 ? c2 : c1, pairs, nleaves);
+int k_panel_compose(hipStream_t s, const int *pairs, int nleaves, int *comp) {
+  if (nleaves <= 0) return PLA_SUCCESS;
+  if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel compose: %d leaves", nleaves);
+  hipLaunchKernelGGL(panel_compose_kernel, dim3(1), dim3(256), 0, s, pairs, nleaves, comp);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
 
-int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *pairs,
-                     int nleaves) {
+template <typename T>
+static int panel_swap_launch(hipStream_t s, T *A, long lda, int nleft, int c1, int c2, const int *comp) {
   const int total = nleft + (c2 > c1 ? c2 - c1 : 0);
-  if (total <= 0 || nleaves <= 0) return PLA_SUCCESS;
-  if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel swap: %d leaves", nleaves);
-  hipLaunchKernelGGL(panel_swap_kernel<float>, dim3((total + 3) / 4), dim3(256), 0, s, A, lda, nleft, c1,
-                     c2 > c1 ? c2 : c1, pairs, nleaves);
+  if (total <= 0) return PLA_SUCCESS;
+  int blocks = (total + 3) / 4;
+  const int cap = 16 * (ctx().cu_count > 0 ? ctx().cu_count : 256);
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(panel_swap_kernel<T>, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1, c2 > c1 ? c2 : c1,
+                     comp);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
+
+int k_panel_swap(hipStream_t s, double *A, long lda, int nleft, int c1, int c2, const int *comp) {
+  return panel_swap_launch<double>(s, A, lda, nleft, c1, c2, comp);
+}
+
+int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *comp) {
+  return panel_swap_launch<float>(s, A, lda, nleft, c1, c2, comp);
+}
+
+int panel_comp_stride() { return 1 + 2 * COMP_MAX; }
 
 int leaf_pairs_stride() { return PAIRS_STRIDE; }
 int leaf_max_width() { return LEAF_W; }

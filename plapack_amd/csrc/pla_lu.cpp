@@ -48,8 +48,10 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
   if (nb > 512) nb = 512;
   if (nb < 1) nb = 1;
   const int stride = leaf_pairs_stride();
+  const int cstride = panel_comp_stride();
   int *pairs = (int *)workspace(sizeof(int) * 2 * 16 * stride, 3);
-  if (!pairs) return PLA_ERR_NO_MEMORY;
+  int *comp = (int *)workspace(sizeof(int) * 2 * cstride, 4);
+  if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
   static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
@@ -63,6 +65,7 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
     PLA_HIP(hipEventRecord(ev_ready, sM));
     PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
     if ((rc = lu_panel(sP, a, m, b, lda, piv, pairs, &nleaves[0]))) return rc;
+    if ((rc = k_panel_compose(sP, pairs, nleaves[0], comp))) return rc;
     PLA_HIP(hipEventRecord(ev_panel, sP));
   }
   for (int k = 0; k < mn; k += nb) {
@@ -71,9 +74,7 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
     double *Akk = a + (long)k * lda + k;
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
     const int nr = n - k - b, mr = m - k - b;
-    if ((rc = k_panel_swap(sM, a + k, lda, k, k + b, n, pairs + (size_t)par * 16 * stride,
-                           nleaves[par])))
-      return rc;
+    if ((rc = k_panel_swap(sM, a + k, lda, k, k + b, n, comp + (size_t)par * cstride))) return rc;
     if (nr <= 0) continue;
     double *A12 = a + (long)(k + b) * lda + k;
     if ((rc = trsm_left(sM, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
@@ -92,6 +93,9 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
       double *An = a + (long)kn * lda + kn;
       if ((rc = lu_panel(sP, An, m - kn, bn, lda, piv + kn, pairs + (size_t)(par ^ 1) * 16 * stride,
                          &nleaves[par ^ 1])))
+        return rc;
+      if ((rc = k_panel_compose(sP, pairs + (size_t)(par ^ 1) * 16 * stride, nleaves[par ^ 1],
+                                comp + (size_t)(par ^ 1) * cstride)))
         return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }

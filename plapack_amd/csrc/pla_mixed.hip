@@ -299,43 +299,67 @@ __device__ inline unsigned long long ld_dev64(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Solve M X = X in place (n x nrhs, nrhs <= 16).  xs: exchange words
+// Solve M X = X in place (n x nrhs, nrhs <= NR <= 16).  xs: exchange words
 // [nblk][TB][MAXRHS][2]; tag unique per launch.
-template <typename T, bool LOWER, bool TRANS>
+//
+// Thread t owns row t%64 of its block and the columns c = t/64 (mod 4) of
+// every tile; its partial sums stay in registers across tiles and the four
+// partials of a row are combined once at the end.  Per step the next tile is
+// loaded into registers while the current one is used, and the solution
+// block it needs is read together with the tile (one round trip when the
+// block is already published).
+template <typename T, bool LOWER, bool TRANS, int NR>
 __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *__restrict__ A, long lda,
                                                       const double *__restrict__ Dinv, double *__restrict__ X,
                                                       long ldx, unsigned long long *__restrict__ xs, unsigned tag32,
                                                       unsigned spin_limit, int *__restrict__ err) {
   constexpr bool LOWEFF = (LOWER != TRANS);
   __shared__ double tile[TB][TB + 1];
-  __shared__ double accs[TB][MAXRHS];
-  __shared__ double xb[TB][MAXRHS];
+  __shared__ double xb[TB][NR];
+  __shared__ double red[4][TB][NR];
   const int nblk = (n + TB - 1) / TB;
   const int b = blockIdx.x;
   const int blk = LOWEFF ? b : nblk - 1 - b;
   const int r0 = blk * TB, bs = min(TB, n - r0);
   const int tid = threadIdx.x;
   const unsigned long long tag = (unsigned long long)tag32 << 32;
-  const int row = tid & 63, g = tid >> 6;  // row of the block, right-hand-side group
+  const int row = tid & 63, cq = tid >> 6;
 
-  for (int e = tid; e < TB * MAXRHS; e += 256) {
-    const int i = e / MAXRHS, q = e - i * MAXRHS;
-    accs[i][q] = (i < bs && q < nrhs) ? X[(long)q * ldx + r0 + i] : 0.0;
-  }
-  bool failed = false;
-  for (int s = 0; s < b; s++) {
+  double part[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) part[q] = 0.0;
+
+  // tile of step s in registers: 16 elements per thread
+  T treg[16];
+  auto load_tile = [&](int s) {
     const int iblk = LOWEFF ? s : nblk - 1 - s;
     const int c0 = iblk * TB, cs = min(TB, n - c0);
-    __syncthreads();  // previous tile and xb consumed
-    // M[r0 + i][c0 + c]; TRANS reads along c, NoTrans along i (both contiguous)
-    for (int e = tid; e < TB * TB; e += 256) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int e = tid + 256 * k;
       int i, c;
       if (TRANS) { c = e & 63; i = e >> 6; }
       else { i = e & 63; c = e >> 6; }
-      tile[i][c] = (i < bs && c < cs) ? op_elem<T, LOWER, TRANS>(A, lda, r0 + i, c0 + c) : 0.0;
+      treg[k] = (i < bs && c < cs) ? (TRANS ? A[(long)(r0 + i) * lda + c0 + c] : A[(long)(c0 + c) * lda + r0 + i])
+                                   : (T)0;
     }
-    // block iblk's solution (tagged words)
+  };
+  bool failed = false;
+  if (b > 0) load_tile(0);
+  for (int s = 0; s < b; s++) {
+    const int iblk = LOWEFF ? s : nblk - 1 - s;
+    const int cs = min(TB, n - iblk * TB);
     const unsigned long long *src = xs + (size_t)iblk * TB * MAXRHS * 2;
+    __syncthreads();  // tile and xb of the previous step consumed
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int e = tid + 256 * k;
+      int i, c;
+      if (TRANS) { c = e & 63; i = e >> 6; }
+      else { i = e & 63; c = e >> 6; }
+      tile[i][c] = (double)treg[k];
+    }
+    if (s + 1 < b) load_tile(s + 1);  // next tile in flight during this step
     unsigned it = 0;
     for (;;) {
       int ok = 1;
@@ -357,57 +381,76 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
       __builtin_amdgcn_s_sleep(1);
     }
     if (failed) break;
-    if (row < bs) {
-      for (int q = g; q < nrhs; q += 4) {
-        double t = accs[row][q];
-        for (int c = 0; c < cs; c++) t = fma(-tile[row][c], xb[c][q], t);
-        accs[row][q] = t;
-      }
+    for (int c = cq; c < cs; c += 4) {
+      const double a = tile[row][c];
+#pragma unroll
+      for (int q = 0; q < NR; q++) part[q] = fma(-a, xb[c][q], part[q]);
     }
   }
   if (failed && tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  // x_blk = inv(M_bb) acc
+#pragma unroll
+  for (int q = 0; q < NR; q++) red[cq][row][q] = part[q];
+  __syncthreads();
+  // acc = rhs + partials; x_blk = inv(M_bb) acc (xb reused for acc)
+  for (int e = tid; e < TB * NR; e += 256) {
+    const int i = e / NR, q = e - i * NR;
+    double v = 0.0;
+    if (i < bs && q < nrhs) v = X[(long)q * ldx + r0 + i] + ((red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]));
+    xb[i][q] = v;
+  }
+  __syncthreads();
   const double *D = Dinv + (size_t)blk * TB * TB;
   unsigned long long *dst = xs + (size_t)blk * TB * MAXRHS * 2;
-  if (row < bs) {
-    for (int q = g; q < nrhs; q += 4) {
-      double t = 0.0;
-      for (int c = 0; c < bs; c++) t = fma(D[(size_t)c * TB + row], accs[c][q], t);
-      X[(long)q * ldx + r0 + row] = t;
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
-      st_dev64(dst + ((size_t)row * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
-      st_dev64(dst + ((size_t)row * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
-    }
+  for (int e = tid; e < bs * nrhs; e += 256) {
+    const int i = e % bs, q = e / bs;
+    double t = 0.0;
+    for (int c = 0; c < bs; c++) t = fma(D[(size_t)c * TB + i], xb[c][q], t);
+    X[(long)q * ldx + r0 + i] = t;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
+    st_dev64(dst + ((size_t)i * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
+    st_dev64(dst + ((size_t)i * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
   }
 }
 
 // ---------------------------------------------------------------------------
-// Residual R = B - A X (FP64, nrhs <= 16): per 512-column chunk a plain FMA
-// sum, then the chunks combined in order with compensated (Neumaier)
+// Residual R = B - A X (FP64, nrhs <= NR <= 16): per 512-column chunk a plain
+// FMA sum, then the chunks combined in order with compensated (Neumaier)
 // addition; max |R| per right-hand side into rmax (as ordered bits).
 constexpr int RCH = 512;
 
+template <int NR>
 __global__ __launch_bounds__(256) void resid_partial_kernel(int n, int nrhs, const double *__restrict__ A,
                                                             long lda, const double *__restrict__ X, long ldx,
                                                             double *__restrict__ part) {
-  __shared__ double xs[RCH][MAXRHS];
+  __shared__ double xs[RCH][NR];
   const int ch = blockIdx.y, j0 = ch * RCH, j1 = min(n, j0 + RCH);
-  for (int e = threadIdx.x; e < RCH * MAXRHS; e += 256) {
-    const int j = e / MAXRHS, q = e - j * MAXRHS;
+  for (int e = threadIdx.x; e < RCH * NR; e += 256) {
+    const int j = e / NR, q = e - j * NR;
     xs[j][q] = (j0 + j < j1 && q < nrhs) ? X[(long)q * ldx + j0 + j] : 0.0;
   }
   __syncthreads();
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double s[MAXRHS];
+  double s[NR];
 #pragma unroll
-  for (int q = 0; q < MAXRHS; q++) s[q] = 0.0;
-  for (int j = j0; j < j1; j++) {
-    const double a = A[(long)j * lda + i];
+  for (int q = 0; q < NR; q++) s[q] = 0.0;
+  const double *ap = A + (long)j0 * lda + i;
+  int j = j0;
+  for (; j + 4 <= j1; j += 4, ap += 4 * lda) {
+    const double a0 = ap[0], a1 = ap[lda], a2 = ap[2 * lda], a3 = ap[3 * lda];
 #pragma unroll
-    for (int q = 0; q < MAXRHS; q++)
-      if (q < nrhs) s[q] = fma(a, xs[j - j0][q], s[q]);
+    for (int q = 0; q < NR; q++) {
+      s[q] = fma(a0, xs[j - j0][q], s[q]);
+      s[q] = fma(a1, xs[j - j0 + 1][q], s[q]);
+      s[q] = fma(a2, xs[j - j0 + 2][q], s[q]);
+      s[q] = fma(a3, xs[j - j0 + 3][q], s[q]);
+    }
+  }
+  for (; j < j1; j++, ap += lda) {
+    const double a0 = ap[0];
+#pragma unroll
+    for (int q = 0; q < NR; q++) s[q] = fma(a0, xs[j - j0][q], s[q]);
   }
   for (int q = 0; q < nrhs; q++) part[((size_t)ch * nrhs + q) * n + i] = s[q];
 }
@@ -568,6 +611,22 @@ int k_upanel_out(hipStream_t s, int prec, int b, int nr, const double *U, long l
 
 static unsigned g_trsv_tag = 1;
 
+template <typename T, int NR>
+static int trsv_launch(hipStream_t s, bool lower, bool trans, int n, int nrhs, const T *A, long lda, double *X,
+                       long ldx, const double *Dinv, unsigned long long *xs, unsigned tag, int *err) {
+  const int nblk = (n + TB - 1) / TB;
+#define PLA_TSF(LO, TR)                                                                                      \
+  hipLaunchKernelGGL((trsv_sf_kernel<T, LO, TR, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, \
+                     ldx, xs, tag, 2000000u, err)
+  if (lower && !trans) PLA_TSF(true, false);
+  else if (lower && trans) PLA_TSF(true, true);
+  else if (!lower && !trans) PLA_TSF(false, false);
+  else PLA_TSF(false, true);
+#undef PLA_TSF
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
 template <typename T>
 static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const T *A, long lda,
                         double *X, long ldx, double *Dinv) {
@@ -599,16 +658,9 @@ static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n,
     g_trsv_tag = 1;
   }
   const unsigned tag = g_trsv_tag++;
-#define PLA_TSF(LO, TR)                                                                                  \
-  hipLaunchKernelGGL((trsv_sf_kernel<T, LO, TR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, ldx, \
-                     xs, tag, 2000000u, err)
-  if (lower && !trans) PLA_TSF(true, false);
-  else if (lower && trans) PLA_TSF(true, true);
-  else if (!lower && !trans) PLA_TSF(false, false);
-  else PLA_TSF(false, true);
-#undef PLA_TSF
-  PLA_HIP(hipGetLastError());
-  return PLA_SUCCESS;
+  if (nrhs <= 1) return trsv_launch<T, 1>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  if (nrhs <= 4) return trsv_launch<T, 4>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  return trsv_launch<T, MAXRHS>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
 }
 
 int trsv_sf(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *A, long lda, double *X,
@@ -649,8 +701,13 @@ int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const do
   unsigned long long *mx = (unsigned long long *)workspace(sizeof(unsigned long long) * MAXRHS, 8);
   if (!part || !mx) return PLA_ERR_NO_MEMORY;
   PLA_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long) * MAXRHS, s));
-  hipLaunchKernelGGL(resid_partial_kernel, dim3((n + 255) / 256, nch), dim3(256), 0, s, n, nrhs, A, lda, X, ldx,
-                     part);
+  const dim3 rg((n + 255) / 256, nch);
+  if (nrhs <= 1)
+    hipLaunchKernelGGL(resid_partial_kernel<1>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
+  else if (nrhs <= 4)
+    hipLaunchKernelGGL(resid_partial_kernel<4>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
+  else
+    hipLaunchKernelGGL(resid_partial_kernel<MAXRHS>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
   hipLaunchKernelGGL(resid_final_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nrhs, nch, part, B, ldb,
                      R, ldr, mx);
   PLA_HIP(hipGetLastError());

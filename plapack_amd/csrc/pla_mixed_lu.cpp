@@ -14,7 +14,8 @@ namespace pla {
 static bool prec_split(int prec) { return prec == PLA_MIXED_FP16X3 || prec == PLA_MIXED_BF16X3; }
 
 // Workspace slots used here (the others belong to pla_kernels / pla_mixed).
-enum { WS_W = 10, WS_P64, WS_U64, WS_LH, WS_LL, WS_UH, WS_UL, WS_PERM, WS_DINVL, WS_DINVU, WS_D, WS_R, WS_PAIRS };
+enum { WS_W = 10, WS_P64, WS_U64, WS_LH, WS_LL, WS_UH, WS_UL, WS_PERM, WS_DINVL, WS_DINVU, WS_D, WS_R, WS_PAIRS,
+       WS_COMP };
 
 // Blocked right-looking LU of the n x n FP32 matrix W (ldw) with one step
 // of look-ahead, as lu_factor (pla_lu.cpp) but with FP64 panels and
@@ -35,7 +36,9 @@ int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *pi
   char *Uh = (char *)workspace((size_t)n * nbp * esz, WS_UH);
   char *Ul = split ? (char *)workspace((size_t)n * nbp * esz, WS_UL) : nullptr;
   int *pairs = (int *)workspace(sizeof(int) * 2 * 16 * stride, WS_PAIRS);
-  if (!P64 || !U64 || !Lh || !Uh || !pairs || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
+  const int cstride = panel_comp_stride();
+  int *comp = (int *)workspace(sizeof(int) * 2 * cstride, WS_COMP);
+  if (!P64 || !U64 || !Lh || !Uh || !pairs || !comp || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
   const size_t lbuf = (size_t)n * nbp * esz;  // one parity of the L copies
   static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
   if (!ev_ready) {
@@ -51,6 +54,8 @@ int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *pi
     int r;
     if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
     if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + (size_t)par * 16 * stride, &nleaves[par]))) return r;
+    if ((r = k_panel_compose(sP, pairs + (size_t)par * 16 * stride, nleaves[par], comp + (size_t)par * cstride)))
+      return r;
     if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
     return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
   };
@@ -65,8 +70,7 @@ int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *pi
     const int b = n - k < nb ? n - k : nb;
     const int par = (k / nb) & 1;
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
-    if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, pairs + (size_t)par * 16 * stride, nleaves[par])))
-      return rc;
+    if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, comp + (size_t)par * cstride))) return rc;
     const int nr = n - k - b;
     if (nr <= 0) continue;
     float *W12 = W + (long)(k + b) * ldw + k;
