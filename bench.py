@@ -1,17 +1,25 @@
-"""Benchmark: plain FP64 PLA_LU with partial pivoting plus the 1-RHS solve,
-on one MI355X, through the C ABI (libpla_mi355x.so).
+"""Benchmark: FP64-accurate LU on one MI355X -- mixed precision vs plain FP64,
+n = 65536 (the metric's configuration), through the C ABI (libpla_mi355x.so).
 
-One step = restore A from a pristine device copy, PLA_LU(A, pivots), then
-apply the interchanges to b = A*1 and do the unit-lower and upper triangular
-solves -- the factor-and-solve path of SOLVERS/LU/PLA_General.c:67-111 with
-A already resident in HBM.  Throughput counts the LU work 2n^3/3 per step.
+mixed step   PLA_LU_mixed: FP32 copy of A, LU with FP64 panels and an FP16x3
+             MFMA trailing update, then FP64 iterative refinement of one
+             right-hand side until the normwise backward error is <= 1e-14.
+plain step   restore A from a pristine HBM copy, PLA_LU (FP64 MFMA trailing
+             update), interchanges + unit-lower + upper solves of b
+             (SOLVERS/LU/PLA_General.c:67-111).
+Both count the LU work 2n^3/3 per step; A is resident in HBM before timing.
+`value` is the mixed path's FP64-accurate TFLOP/s (the backward error of
+every timed step is checked); `plain_fp64` holds the plain path on the same
+matrix, and `speedup_vs_plain` their ratio.
 
-Prints ONE JSON line.  The roofline object is the trailing-update GEMM (the
-dominant kernel): its algorithmic flops over its hipEvent-timed launches on
-the library stream.  The CPU baseline is the oracle's blocked LU (a scalar C
-restatement of the reference algorithm) on a bounded sample size.
+Prints ONE JSON line.  The roofline objects are the dominant kernels of each
+path (trailing-update GEMMs): their algorithmic flops over their hipEvent-
+timed launches on the stream that launches them.  The CPU baseline is the
+oracle's blocked LU (a scalar C restatement of the reference algorithm) on a
+bounded sample size.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -23,7 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "FP64-accurate LU TFLOP/s, mixed vs plain FP64, n=65536; batched solves/s"
-FP64_MFMA_PEAK_TFLOPS = 78.6
+FP64_MFMA_PEAK = 78.6
+FP16_MFMA_PEAK = 2500.0
+PREC_NAMES = {1: "FP16", 2: "BF16", 3: "FP16x3", 4: "BF16x3"}
 
 
 def cpu_baseline(n):
@@ -33,9 +43,27 @@ def cpu_baseline(n):
     t = time.perf_counter()
     ora.lu_blocked(a, nb=128)
     dt = time.perf_counter() - t
-    return {"value": 2.0 / 3.0 * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+    return {"value": round(2.0 / 3.0 * n ** 3 / dt / 1e12, 6), "unit": "TFLOP/s", "cores": 1, "kind": "port",
             "sample": f"oracle blocked LU (nb=128, scalar C, netlib loop order) on n={n}, "
                       f"drand48 seed 0; {dt:.1f} s"}
+
+
+def timing_query(pla, role):
+    launches, ms, fl = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+    pla.call("PLA_Kernel_timing_query", role, ctypes.byref(launches), ctypes.byref(ms), ctypes.byref(fl))
+    return launches.value, ms.value, fl.value
+
+
+def roofline(kernel, launches, ms, flops, peak, mfma_per_flop=1):
+    achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else None
+    out = {"kernel": kernel, "bound": "mfma", "achieved": round(achieved, 3) if achieved else None,
+           "peak": peak, "unit": "TFLOP/s",
+           "frac": round(mfma_per_flop * achieved / peak, 4) if achieved else None,
+           "traffic": None, "launches": launches,
+           "avg_launch_ms": round(ms / max(1, launches), 4)}
+    if mfma_per_flop != 1:
+        out["mfma_products_per_flop"] = mfma_per_flop
+    return out
 
 
 def main():
@@ -43,8 +71,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--mode", choices=["mixed", "plain"], default="mixed")
+    ap.add_argument("--prec", type=int, default=3, help="1 FP16, 2 BF16, 3 FP16x3, 4 BF16x3")
+    ap.add_argument("--tol", type=float, default=1e-14)
     ap.add_argument("--nb", type=int, default=0, help="block size (0: library's choice)")
+    ap.add_argument("--plain-steps", type=int, default=1, help="plain FP64 steps timed beside mixed (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=4096, help="oracle sample size (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -55,76 +87,126 @@ def main():
     from plapack_amd import pla
 
     n = args.n
+    work = 2.0 / 3.0 * n ** 3
+    data = ("synthetic: A(i,j) = drand48()*2-1 from srand48(0), column-major (TEST/create_problem.c), "
+            "b = A*1, generated on the GPU")
     with pla.Session() as s:
         pla.call("PLA_Set_device_nb", args.nb)
-        import ctypes
         nbv = ctypes.c_int()
         pla.call("PLA_Get_device_nb", n, ctypes.byref(nbv))
-        A0, A = s.matrix(n, n), s.matrix(n, n)
+        A0 = s.matrix(n, n)
         pla.call("PLA_Obj_fill_drand48", A0, 0, 0)
         ones, b, x, r = s.mvector(n), s.mvector(n), s.mvector(n), s.mvector(n)
         P = s.mvector(n, dtype=pla.MPI_INT)
-        one, zero, mone = s.scalar(1.0), s.scalar(0.0), s.scalar(-1.0)
+        one, zero, mone, nrm = s.scalar(1.0), s.scalar(0.0), s.scalar(-1.0), s.scalar(0.0)
         pla.call("PLA_Obj_set_to_one", ones)
-        pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, one, A0, ones, zero, b)
-
-        def step():
-            pla.call("PLA_Copy", A0, A)
-            pla.call("PLA_LU", A, P)
-            pla.call("PLA_Copy", b, x)
-            pla.call("PLA_Apply_pivots_to_rows", x, P)
-            pla.call("PLA_Trsv", pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_UNIT_DIAG, A, x)
-            pla.call("PLA_Trsv", pla.PLA_UPPER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG,
-                     A, x)
-
-        for _ in range(args.warmup):
-            step()
-        pla.call("PLA_Device_synchronize")
-        pla.call("PLA_Kernel_timing", 1)
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            step()
-            print(f"step {i} done", file=sys.stderr, flush=True)
-        pla.call("PLA_Device_synchronize")
-        elapsed = time.perf_counter() - t0
-        launches, kms, kfl = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
-        pla.call("PLA_Kernel_timing_query", 1, ctypes.byref(launches), ctypes.byref(kms),
-                 ctypes.byref(kfl))
-        pla.call("PLA_Kernel_timing", 0)
-
-        # accuracy of the last step (FP64 on the GPU): normwise backward error
-        # eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) and HPL's eta / (n eps)
-        pla.call("PLA_Copy", b, r)
-        pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, mone, A0, x, one, r)
-        nrm = s.scalar(0.0)
+        pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, one, A0, ones, zero, b)   # b = A*1
         pla.call("PLA_Matrix_infinity_norm", A0, nrm)
         anorm = float(pla.get_contents(nrm)[0, 0])
-        xv, rv, bv = (pla.get_contents(o)[:, 0] for o in (x, r, b))
-        eta = float(np.abs(rv).max() / (anorm * np.abs(xv).max() + np.abs(bv).max()))
-        xerr = float(np.abs(xv - 1.0).max())
-        pla.free(A0, A, ones, b, x, r, P, one, zero, mone, nrm)
 
-    ms = elapsed / args.steps * 1e3
-    work = 2.0 / 3.0 * n ** 3
-    value = work / (ms * 1e-3) / 1e12
-    achieved = kfl.value / (kms.value * 1e-3) / 1e12 if kms.value > 0 else None
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: A(i,j) = drand48()*2-1 from srand48(0), column-major "
-                "(TEST/create_problem.c), b = A*1, generated on the GPU",
-        "config": {"workload": f"plain FP64 PLA_LU (partial pivoting) + 1-RHS solve, n={n}",
-                   "n": n, "nb": nbv.value, "rhs": 1, "template": "1x1"},
-        "roofline": {"kernel": "dgemm_kernel<NN, ROLE 1> (A22 -= A21*A12)", "bound": "mfma",
-                     "achieved": round(achieved, 3) if achieved else None,
-                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 4) if achieved else None,
-                     "traffic": None, "launches": launches.value,
-                     "avg_launch_ms": round(kms.value / max(1, launches.value), 4)},
-        "accuracy": {"backward_error_inf": eta, "hpl_scaled_residual": eta / (n * 2.0 ** -52),
-                     "x_minus_1_inf": xerr},
-    }
+        def backward_error():
+            pla.call("PLA_Copy", b, r)
+            pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, mone, A0, x, one, r)
+            xv, rv, bv = (pla.get_contents(o)[:, 0] for o in (x, r, b))
+            return (float(np.abs(rv).max() / (anorm * np.abs(xv).max() + np.abs(bv).max())),
+                    float(np.abs(xv - 1.0).max()))
+
+        def run_plain(steps, warmup):
+            A = s.matrix(n, n)
+
+            def step():
+                pla.call("PLA_Copy", A0, A)
+                pla.call("PLA_LU", A, P)
+                pla.call("PLA_Copy", b, x)
+                pla.call("PLA_Apply_pivots_to_rows", x, P)
+                pla.call("PLA_Trsv", pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_UNIT_DIAG, A, x)
+                pla.call("PLA_Trsv", pla.PLA_UPPER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG,
+                         A, x)
+            for _ in range(warmup):
+                step()
+            pla.call("PLA_Device_synchronize")
+            pla.call("PLA_Kernel_timing", 1)
+            t0 = time.perf_counter()
+            for i in range(steps):
+                step()
+                print(f"plain step {i} done", file=sys.stderr, flush=True)
+            pla.call("PLA_Device_synchronize")
+            el = time.perf_counter() - t0
+            rl = roofline("dgemm_kernel<NN, ROLE 1> (A22 -= A21*A12, FP64)", *timing_query(pla, 1),
+                          FP64_MFMA_PEAK)
+            pla.call("PLA_Kernel_timing", 0)
+            pla.free(A)
+            eta, xerr = backward_error()
+            ms = el / steps * 1e3
+            return {"value": round(work / (ms * 1e-3) / 1e12, 3), "ms_per_step": round(ms, 2),
+                    "steps": steps, "warmup": warmup, "roofline": rl,
+                    "backward_error_inf": eta, "hpl_scaled_residual": eta / (n * 2.0 ** -52),
+                    "x_minus_1_inf": xerr}
+
+        if args.mode == "plain":
+            res = run_plain(args.steps, args.warmup)
+            out = {"metric": METRIC, "value": res["value"], "unit": "TFLOP/s", "n_gpus": 1,
+                   "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+                   "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                   "data": data,
+                   "config": {"workload": f"plain FP64 PLA_LU (partial pivoting) + 1-RHS solve, n={n}",
+                              "n": n, "nb": nbv.value, "rhs": 1, "template": "1x1"},
+                   "roofline": res["roofline"],
+                   "accuracy": {k: res[k] for k in ("backward_error_inf", "hpl_scaled_residual",
+                                                    "x_minus_1_inf")}}
+        else:
+            X = s.matrix(n, 1)
+            bmat = s.matrix(n, 1)
+            pla.call("PLA_Copy", b, bmat)
+            it, be = ctypes.c_int(), ctypes.c_double()
+
+            def step():
+                pla.call("PLA_LU_mixed", A0, P, bmat, X, args.prec, args.tol, 30, ctypes.byref(it),
+                         ctypes.byref(be))
+                if not be.value <= args.tol:
+                    raise RuntimeError(f"refinement did not reach {args.tol}: {be.value}")
+            for _ in range(args.warmup):
+                step()
+            pla.call("PLA_Device_synchronize")
+            pla.call("PLA_Kernel_timing", 1)
+            t0 = time.perf_counter()
+            errs, iters = [], []
+            for i in range(args.steps):
+                step()
+                errs.append(be.value)
+                iters.append(it.value)
+                print(f"mixed step {i} done: {it.value} refinement steps, backward error {be.value:.3e}",
+                      file=sys.stderr, flush=True)
+            pla.call("PLA_Device_synchronize")
+            el = time.perf_counter() - t0
+            split = 3 if args.prec in (3, 4) else 1
+            rl = roofline(f"hgemm_kernel<{PREC_NAMES[args.prec]}> (W22 -= L21*U12, FP32 accumulate)",
+                          *timing_query(pla, 3), FP16_MFMA_PEAK, mfma_per_flop=split)
+            pla.call("PLA_Kernel_timing", 0)
+            pla.call("PLA_Copy", X, x)
+            eta, xerr = backward_error()   # recomputed independently (GEMV on A0)
+            ms = el / args.steps * 1e3
+            value = work / (ms * 1e-3) / 1e12
+            pla.free(X, bmat)
+            plain = run_plain(args.plain_steps, 0) if args.plain_steps > 0 else None
+            out = {"metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
+                   "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
+                   "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                   "dtype": f"{PREC_NAMES[args.prec].lower()} MFMA / f32 / f64 (FP64-refined)",
+                   "data": data,
+                   "config": {"workload": f"mixed-precision PLA_LU_mixed ({PREC_NAMES[args.prec]} MFMA trailing "
+                                          f"update, FP64 panels, FP32 factors) + FP64 iterative refinement to "
+                                          f"backward error <= {args.tol}, n={n}, 1 RHS",
+                              "n": n, "nb": nbv.value, "rhs": 1, "template": "1x1",
+                              "precision": PREC_NAMES[args.prec]},
+                   "roofline": rl,
+                   "accuracy": {"backward_error_inf": max(errs), "refinement_steps": iters,
+                                "backward_error_recomputed": eta, "x_minus_1_inf": xerr}}
+            if plain:
+                out["plain_fp64"] = plain
+                out["speedup_vs_plain"] = round(value / plain["value"], 3)
+        pla.free(A0, ones, b, x, r, P, one, zero, mone, nrm)
+
     if args.cpu_n > 0:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)
