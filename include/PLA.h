@@ -270,6 +270,49 @@ int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B);
 #define PLA_MIXED_BF16X3 4
 int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision, double tol, int max_iter,
                  int *iterations, double *backward_error);
+/* Batched factor-and-solve of `count` independent SPD systems of uneven sizes
+   A_i X_i = B_i (the mathematics of PLA_Pos_def_solve, SOLVERS/Cholesky/
+   PLA_Pos_def.c:44-99, for thousands of small systems at once).  One
+   persistent kernel solves every system; a plan captures it in a hipGraph
+   and replays it.
+   Storage (all device pointers, HBM): A_i is n[i] x n[i] column-major with
+   leading dimension n[i] at A + a_off[i]; B_i is n[i] x nrhs (ld n[i]) at
+   B + b_off[i]; NULL offsets mean packed back to back in index order.  Only
+   the lower triangle of each A_i is read.  nrhs <= 8.
+   mode PLA_BATCHED_FP64:    A_i is overwritten by L_i (lower), B_i by X_i.
+   mode PLA_BATCHED_FP32_IR: A_i is left unchanged; FP32 factors, then FP64
+     refinement until max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf +
+     ||b_j||_inf) <= tol or max_iter steps; B_i is overwritten by X_i.
+   OR-ing PLA_BATCHED_AUTOTUNE into mode times every kernel variant (block
+   size, workgroups per CU) on scratch copies and keeps the fastest.
+   info[i] (device, int): 0, or the 1-based index of the first non-positive
+   pivot of system i (then B_i is left unchanged).  iters[i] / berr[i]
+   (device, may be NULL): refinement steps and final backward error
+   (FP32_IR only).  The plan keeps the pointers: execute replays on them. */
+#define PLA_BATCHED_FP64     1
+#define PLA_BATCHED_FP32_IR  2
+#define PLA_BATCHED_AUTOTUNE 0x100
+typedef struct PLA_Batched_plan_struct *PLA_Batched_plan;
+int PLA_Batched_plan_create(int count, const int *n, const long *a_off, const long *b_off, int nrhs,
+                            int mode, double tol, int max_iter, double *A, double *B, int *info,
+                            int *iters, double *berr, PLA_Batched_plan *plan);
+/* Enqueue one replay of the plan's graph on the library stream (asynchronous;
+   PLA_Device_synchronize waits for it). */
+int PLA_Batched_plan_execute(PLA_Batched_plan plan);
+/* Autotune an existing plan (see PLA_BATCHED_AUTOTUNE); *ms_best may be NULL. */
+int PLA_Batched_plan_autotune(PLA_Batched_plan plan, double *ms_best);
+/* The plan's block size and workgroups per CU. */
+int PLA_Batched_plan_config(PLA_Batched_plan plan, int *nb, int *workgroups_per_cu);
+int PLA_Batched_plan_free(PLA_Batched_plan *plan);
+/* create + execute + wait + free */
+int PLA_Chol_batched(int count, const int *n, const long *a_off, const long *b_off, int nrhs, int mode,
+                     double tol, int max_iter, double *A, double *B, int *info, int *iters, double *berr);
+/* OBJ/PLA_Obj_management.c:15 -- attach a caller's DEVICE (HBM) buffer with
+   leading dimension ldim to obj, which must view its whole base object.
+   Objects here always own a buffer, so the old one is released (the
+   reference requires a base without a buffer).  user_buffer != 0: the
+   buffer is not freed by PLA_Obj_free. */
+int PLA_Obj_attach_buffer(PLA_Obj obj, void *buffer, int local_ldim, int user_buffer);
 /* Message of the last error returned by any entry point. */
 const char *PLA_Last_error(void);
 /* Device pointer of the object's top-left element (HBM).  Drops the host

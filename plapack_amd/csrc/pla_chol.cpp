@@ -20,33 +20,87 @@ namespace pla {
 
 // Returns PLA_SUCCESS and sets *info to 0 or to the 1-based index of the
 // first non-positive pivot.
-int chol_factor(hipStream_t s, int n, double *a, long lda, int nb, int *info_out) {
-  int *info = (int *)workspace(sizeof(int), 0);
+//
+// One step of look-ahead, as in the LU: the column block of step k+1 is
+// factored (potrf + trsm) on the look-ahead stream as soon as step k's
+// update of those columns is done, while the main stream applies the rest of
+// step k's SYRK.
+//
+// main stream, step k:  wait(panel k) -> lower(A22[:bn, :bn]) -= A21 A21^T,
+//                       A22[bn:, :bn] -= A21[bn:] A21[:bn]^T -> signal
+//                       -> lower(A22[bn:, bn:]) -= A21[bn:] A21[bn:]^T
+// look-ahead stream:    wait(signal) -> back up + factor A11 -> A21 inv(L11)^T
+int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out) {
+  hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
+  if (nb > 512) nb = 512;
+  if (nb < 1) nb = 1;
+  // info: first failure index, written by the panels on sP.  seen: the
+  // main stream's snapshot of info taken when panel k is done and panel k+1
+  // not yet started, so step k's update runs exactly when the reference's
+  // would (every panel <= k succeeded), whatever panel k+1 does meanwhile.
+  int *info = (int *)workspace(2 * sizeof(int), 0);
+  int *seen = info + 1;
   double *backup = (double *)workspace(sizeof(double) * (size_t)nb * nb, 1);
   if (!info || !backup) return PLA_ERR_NO_MEMORY;
-  PLA_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  if (!ev_ready) {
+    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+  }
+  PLA_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int), sM));
   int rc;
-  for (int k = 0; k < n; k += nb) {
+  // factor the column block that starts at diagonal position k (on sP)
+  auto panel = [&](int k) -> int {
     const int b = n - k < nb ? n - k : nb;
     double *Akk = a + (long)k * lda + k;
-    if ((rc = k_copy2d(s, b, b, Akk, lda, backup, b, info))) return rc;
-    if ((rc = potrf_block(s, b, Akk, lda, k, info))) return rc;
+    int rc2;
+    if ((rc2 = k_copy2d(sP, b, b, Akk, lda, backup, b, info))) return rc2;
+    if ((rc2 = potrf_block(sP, b, Akk, lda, k, info))) return rc2;
     const int r = n - k - b;
-    if (r > 0) {
-      if ((rc = trsm_rltn(s, r, b, Akk, lda, Akk + b, lda, info))) return rc;
-      timing_begin(s, 2);
-      rc = k_syrk_lower(s, 1, r, b, -1.0, Akk + b, lda, 1.0, Akk + (long)b * lda + b, lda, info);
-      timing_end(s, 2, (double)r * (double)(r + 1) * b);
+    if (r > 0 && (rc2 = trsm_rltn(sP, r, b, Akk, lda, Akk + b, lda, info))) return rc2;
+    return PLA_SUCCESS;
+  };
+  PLA_HIP(hipEventRecord(ev_ready, sM));
+  PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+  if ((rc = panel(0))) return rc;
+  PLA_HIP(hipEventRecord(ev_panel, sP));
+  for (int k = 0; k < n; k += nb) {
+    const int b = n - k < nb ? n - k : nb;
+    PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
+    const int r = n - k - b;
+    if (r <= 0) continue;
+    PLA_HIP(hipMemcpyAsync(seen, info, sizeof(int), hipMemcpyDeviceToDevice, sM));
+    double *A21 = a + (long)k * lda + k + b;
+    double *A22 = a + (long)(k + b) * lda + k + b;
+    const int bn = r < nb ? r : nb;
+    timing_begin(sM, 2);
+    rc = k_syrk_lower(sM, 1, bn, b, -1.0, A21, lda, 1.0, A22, lda, seen);
+    if (!rc && r > bn)
+      rc = k_gemm(sM, 1, false, true, r - bn, bn, b, -1.0, A21 + bn, lda, A21, lda, 1.0, A22 + bn, lda,
+                  seen);
+    timing_end(sM, 2, (double)bn * (bn + 1) * b + 2.0 * (r - bn) * (double)bn * b);
+    if (rc) return rc;
+    PLA_HIP(hipEventRecord(ev_ready, sM));
+    PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+    if ((rc = panel(k + b))) return rc;
+    PLA_HIP(hipEventRecord(ev_panel, sP));
+    if (r > bn) {
+      const int rr = r - bn;
+      timing_begin(sM, 2);
+      rc = k_syrk_lower(sM, 1, rr, b, -1.0, A21 + bn, lda, 1.0, A22 + (long)bn * lda + bn, lda, seen);
+      timing_end(sM, 2, (double)rr * (rr + 1) * b);
       if (rc) return rc;
     }
   }
   int h_info = 0;
-  PLA_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, s));
-  PLA_HIP(hipStreamSynchronize(s));
+  PLA_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, sM));
+  PLA_HIP(hipStreamSynchronize(sM));
   if (h_info > 0) {
+    // the failed block's backup is the last one taken: later panels abort
+    // on the info word before copying
     const int k = ((h_info - 1) / nb) * nb;
     const int b = n - k < nb ? n - k : nb;
-    if ((rc = k_copy2d(s, b, b, backup, b, a + (long)k * lda + k, lda, nullptr))) return rc;
+    if ((rc = k_copy2d(sM, b, b, backup, b, a + (long)k * lda + k, lda, nullptr))) return rc;
   }
   *info_out = h_info;
   return PLA_SUCCESS;

@@ -26,6 +26,7 @@ struct pla_base {
   void *host;             // pinned mirror or nullptr
   bool host_dirty;        // mirror may hold data newer than the device copy
   int number_of_views;    // reference count (OBJ/PLA_Obj.c:2019)
+  int user_buffer;        // dev belongs to the caller (PLA_Obj_attach_buffer)
   PLA_Template templ;
 };
 

@@ -60,7 +60,7 @@ int written(PLA_Obj o) {
 
 static void detach(pla_base *b) {
   if (--b->number_of_views > 0) return;
-  if (b->dev) (void)hipFree(b->dev);
+  if (b->dev && !b->user_buffer) (void)hipFree(b->dev);
   if (b->host) (void)hipHostFree(b->host);
   free(b);
 }
@@ -311,6 +311,35 @@ int PLA_Obj_device_buffer(PLA_Obj obj, void **device_buffer) {
     b->host = nullptr;
     b->host_dirty = false;
   }
+  return PLA_SUCCESS;
+}
+
+// OBJ/PLA_Obj_management.c:15-42.  The reference attaches a buffer to a base
+// object that has none; every object created here already owns an HBM
+// buffer, so attaching replaces it (the old buffer and any host mirror are
+// released).  `buffer` is a device pointer; with user_buffer != 0 it stays
+// the caller's and PLA_Obj_free leaves it alone.
+int PLA_Obj_attach_buffer(PLA_Obj obj, void *buffer, int local_ldim, int user_buffer) {
+  if (!valid(obj) || !buffer) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Obj_attach_buffer: invalid argument");
+  pla_base *b = obj->base;
+  if (obj->row0 || obj->col0 || obj->length != b->length || obj->width != b->width)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Obj_attach_buffer: obj must view its whole base object");
+  if (obj->fixed) return fail(PLA_ERR_INVALID_PARAM, "PLA_Obj_attach_buffer: obj is a constant");
+  if (local_ldim < (b->length > 0 ? b->length : 1))
+    return fail(PLA_ERR_DIMENSIONS, "PLA_Obj_attach_buffer: ldim %d < %d rows", local_ldim, b->length);
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, buffer) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Obj_attach_buffer: buffer is not device (HBM) memory");
+  }
+  PLA_HIP(hipStreamSynchronize(ctx().stream));
+  if (b->dev && !b->user_buffer) (void)hipFree(b->dev);
+  if (b->host) (void)hipHostFree(b->host);
+  b->host = nullptr;
+  b->host_dirty = false;
+  b->dev = buffer;
+  b->ldim = local_ldim;
+  b->user_buffer = user_buffer ? 1 : 0;
   return PLA_SUCCESS;
 }
 

@@ -49,6 +49,9 @@ PLA_MIXED_FP16X3 = 3
 PLA_MIXED_BF16X3 = 4
 PLA_PARAM_LOOKAHEAD = 1
 PLA_PARAM_LEAF_ROWS = 2
+PLA_BATCHED_FP64 = 1
+PLA_BATCHED_FP32_IR = 2
+PLA_BATCHED_AUTOTUNE = 0x100
 
 _obj = ctypes.c_void_p
 _pobj = ctypes.POINTER(ctypes.c_void_p)
@@ -116,8 +119,15 @@ _SIGS = {
     "PLA_General_solve": [_obj, _obj, _obj],
     "PLA_Pos_def_solve": [_obj, _obj],
     "PLA_LU_mixed": [_obj, _obj, _obj, _obj, _i, _d, _i, _pi, ctypes.POINTER(_d)],
+    "PLA_Batched_plan_create": [_i, _pi, _p, _p, _i, _i, _d, _i, _p, _p, _p, _p, _p, _pobj],
+    "PLA_Batched_plan_execute": [_obj],
+    "PLA_Batched_plan_autotune": [_obj, ctypes.POINTER(_d)],
+    "PLA_Batched_plan_config": [_obj, _pi, _pi],
+    "PLA_Batched_plan_free": [_pobj],
+    "PLA_Chol_batched": [_i, _pi, _p, _p, _i, _i, _d, _i, _p, _p, _p, _p, _p],
     "PLA_Last_error": [],
     "PLA_Obj_device_buffer": [_obj, _pobj],
+    "PLA_Obj_attach_buffer": [_obj, _p, _i, _i],
     "PLA_Obj_fill_drand48": [_obj, ctypes.c_long, ctypes.c_int64],
     "PLA_Device_synchronize": [],
     "PLA_Set_device_nb": [_i],

@@ -172,3 +172,29 @@ def test_reference_style_c_driver(tmp_path):
     out = subprocess.run([exe], input="2 1\n128\n128\n1\n1\n0\n-1\n", capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 3 and "1x1" in out.stdout
+
+
+def test_attach_device_buffer(pla_session, ora):
+    """PLA_Obj_attach_buffer (OBJ/PLA_Obj_management.c:15) with a device buffer:
+    PLA_LU on an object whose storage is the top-left block of another."""
+    s = pla_session
+    big = ora.uniform(300, 260, seed=9)
+    X = pla.from_numpy(s, big)
+    ptr = ctypes.c_void_p()
+    pla.call("PLA_Obj_device_buffer", X, ctypes.byref(ptr))
+    Y = s.matrix(200, 200)
+    P = s.mvector(200, dtype=pla.MPI_INT)
+    pla.call("PLA_Obj_attach_buffer", Y, ptr, 300, 1)
+    pla.call("PLA_LU", Y, P)
+    lu_ref, piv_ref = ora.lu_blocked(big[:200, :200], nb=128)
+    out = pla.get_contents(X)
+    assert np.array_equal(pla.get_contents(P)[:, 0], piv_ref)
+    assert np.allclose(out[:200, :200], lu_ref, atol=1e-10)
+    assert np.array_equal(out[200:, :], big[200:, :]) and np.array_equal(out[:, 200:], big[:, 200:])
+    pla.free(Y, P)                      # the attached buffer stays X's
+    assert np.allclose(pla.get_contents(X)[:200, :200], lu_ref, atol=1e-10)
+    host = np.zeros(10)
+    Z = s.matrix(10, 1)
+    assert pla.lib().PLA_Obj_attach_buffer(Z, host.ctypes.data_as(ctypes.c_void_p), 10, 1) < 0
+    assert pla.lib().PLA_Obj_attach_buffer(Z, ptr, 5, 1) < 0     # ldim < rows
+    pla.free(X, Z)
