@@ -313,6 +313,26 @@ int PLA_Chol_batched(int count, const int *n, const long *a_off, const long *b_o
    reference requires a base without a buffer).  user_buffer != 0: the
    buffer is not freed by PLA_Obj_free. */
 int PLA_Obj_attach_buffer(PLA_Obj obj, void *buffer, int local_ldim, int user_buffer);
+/* Mixed-precision Cholesky factor-and-solve with FP64 refinement.  Only the
+   lower triangle of A (n x n MPI_DOUBLE, symmetric positive definite) is read,
+   and A is left unchanged.  An FP32 copy is factored by the blocked algorithm
+   of SOLVERS/Cholesky/PLA_Chol.c:83-234 with nb = 256: FP64 diagonal blocks
+   and panels, and the trailing lower(A22) -= L21 L21^T on FP8 MFMA --
+     PLA_MIXED_FP8: L21 in OCP E4M3 with one power-of-two scale per 256 x 256
+                    tile, exact products, FP32 accumulation.
+   If the FP8-updated factorization meets a non-positive pivot it is repeated
+   for A + shift I with a growing shift (reported by PLA_Mixed_stats).
+   X (n x nrhs, nrhs <= 16) is then refined in FP64 against A: classic
+   refinement, switching to GMRES-IR preconditioned by the factors when a
+   step fails to halve the normwise backward error, until it is <= tol or
+   max_iter corrections were applied.  PLA_LU_mixed refines the same way. */
+#define PLA_MIXED_FP8    5
+int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, int max_iter,
+                   int *iterations, double *backward_error);
+/* Statistics of the last PLA_LU_mixed / PLA_Chol_mixed call: corrections
+   applied, GMRES steps in total, corrections computed by GMRES, and the
+   diagonal shift of the factored matrix (0 for LU).  Any pointer may be NULL. */
+int PLA_Mixed_stats(int *outer, int *gmres_steps, int *gmres_corrections, double *shift);
 /* Message of the last error returned by any entry point. */
 const char *PLA_Last_error(void);
 /* Device pointer of the object's top-left element (HBM).  Drops the host
@@ -339,7 +359,8 @@ int PLA_Get_device_nb(int n, int *nb);
 /* Per-launch hipEvent timing of the trailing-update kernels on the library
    stream (the reference keeps per-operation timers in PLA_TIMINGS,
    OBJ/PLA_Timings.c:17-125).  role 1: PLA_LU's A22 -= A21 A12 GEMM;
-   role 2: PLA_Chol's lower(A22) -= A21 A21^T SYRK.  PLA_Kernel_timing(on)
+   role 2: PLA_Chol's lower(A22) -= A21 A21^T SYRK; role 3: PLA_LU_mixed's
+   low-precision trailing GEMM; role 4: PLA_Chol_mixed's FP8 SYRK.  PLA_Kernel_timing(on)
    clears the counters; the query waits for the recorded events and returns
    the launch count, summed kernel milliseconds and summed algorithmic flops. */
 int PLA_Kernel_timing(int on);

@@ -186,3 +186,77 @@ def lu_scaled_residual(a, lu, piv):
     na = np.abs(np.asarray(a)).sum(axis=0).max()
     n = max(a.shape)
     return r / (n * na * np.finfo(np.float64).eps)
+
+
+# ---- FP8 (OCP E4M3) reference rounding --------------------------------------
+# Restated from the OCP 8-bit floating point specification (OFP8, E4M3):
+# 1 sign, 4 exponent (bias 7), 3 mantissa bits; normals 2^(E-7) (1 + m/8) for
+# E = 1..15 (E = 15, m = 7 is NaN, so the largest value is 448); subnormals
+# 2^-6 (m/8).  Rounding is to nearest, ties to even.  No counterpart in the
+# reference; pinned in tests/test_oracle.py by the specification's encodings.
+
+E4M3_MAX = 448.0
+
+
+def e4m3_round(x):
+    """Round to the nearest E4M3 value (ties to even); |x| must be <= 448."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.abs(x)
+    if np.any(a > E4M3_MAX):
+        raise ValueError("e4m3_round: |x| > 448")
+    e = np.floor(np.log2(np.where(a > 0, a, 1.0)))
+    e = np.maximum(e, -6.0)                 # subnormal range shares the 2^-9 quantum
+    # correct log2 rounding at binade edges
+    e = np.where(a >= np.exp2(e + 1), e + 1, e)
+    e = np.where((a < np.exp2(e)) & (e > -6), e - 1, e)
+    q = np.exp2(e - 3)                      # spacing of the binade
+    r = np.rint(a / q) * q                  # exact: a / q is scaled by a power of two
+    return np.copysign(r, x)
+
+
+def e4m3_encode(v):
+    """Bytes of E4M3 values (v must be exactly representable)."""
+    v = np.asarray(v, dtype=np.float64)
+    sign = (np.signbit(v)).astype(np.uint8) << 7
+    a = np.abs(v)
+    out = np.zeros(v.shape, dtype=np.uint8)
+    sub = a < 2.0 ** -6
+    m_sub = np.rint(a / 2.0 ** -9).astype(np.int64)
+    e = np.floor(np.log2(np.where(sub, 1.0, a))).astype(np.int64)
+    m = np.rint((np.where(sub, 1.0, a) / np.exp2(e) - 1.0) * 8).astype(np.int64)
+    norm = ((e + 7) << 3) | m
+    out = np.where(sub, m_sub, norm).astype(np.uint8) | sign
+    if not np.array_equal(e4m3_decode(out), v):
+        raise ValueError("e4m3_encode: value not representable")
+    return out
+
+
+def e4m3_decode(b):
+    b = np.asarray(b, dtype=np.uint8).astype(np.int64)
+    s = np.where(b & 0x80, -1.0, 1.0)
+    E = (b >> 3) & 0xF
+    m = b & 7
+    val = np.where(E == 0, np.exp2(-6.0) * m / 8.0, np.exp2(E - 7.0) * (1.0 + m / 8.0))
+    return s * val
+
+
+def fp8_quantize_tiles(p, tile=256, width=256):
+    """Quantization of an R x b panel (b <= width) in tiles of `tile` rows:
+    per tile the smallest e with max|tile| <= 448 * 2^e, values rounded to
+    FP32 then to E4M3.  Returns (bytes R x width, row-major; exponents)."""
+    p = np.asarray(p, dtype=np.float64)
+    R, b = p.shape
+    q = np.zeros((R, width), dtype=np.uint8)
+    exps = []
+    for t0 in range(0, R, tile):
+        blk = p[t0:t0 + tile]
+        amax = np.abs(blk).max() if blk.size else 0.0
+        if amax > 0:
+            f, e2 = np.frexp(amax / E4M3_MAX)
+            ex = int(e2 - 1 if f == 0.5 else e2)
+        else:
+            ex = 0
+        exps.append(ex)
+        scaled = (blk * 2.0 ** -ex).astype(np.float32).astype(np.float64)
+        q[t0:t0 + tile, :b] = e4m3_encode(e4m3_round(scaled))
+    return q, np.array(exps, dtype=np.int32)

@@ -48,7 +48,6 @@ struct bsys {
 };
 
 constexpr int BT = 256;      // threads per workgroup
-constexpr int BQ = 8;        // row tiles of 16 per wave per pass (4 waves: 512 rows)
 constexpr int MAX_RHS = 8;
 
 template <typename LT> struct mfma16;
@@ -97,7 +96,10 @@ __device__ inline void wait_stores_and_sync() {
 
 // Left-looking blocked Cholesky of system d into L (ld n).  Returns 0, or the
 // 1-based index of the first non-positive pivot (as PLA_Chol).
-template <typename LT, int NB>
+// BQ: row tiles of 16 per wave per pass over the block column (4 waves cover
+// 64 BQ rows per pass); more tiles reuse each L fragment more, fewer leave
+// registers for a second workgroup per CU.
+template <typename LT, int NB, int BQ>
 __device__ int factor_system(const bsys &d, const batched_args &a, LT *L, LT *P, LT *rsq) {
   using M = mfma16<LT>;
   typedef typename M::v4 v4;
@@ -119,26 +121,47 @@ __device__ int factor_system(const bsys &d, const batched_args &a, LT *L, LT *P,
       for (int q = 0; q < BQ; q++)
 #pragma unroll
         for (int c = 0; c < CT; c++) acc[q][c] = v4{0, 0, 0, 0};
-      if (Q > 0) {
-        for (int p = 0; p < k0; p += 4) {
-          const LT *Lc = L + (long)(p + (lane >> 4)) * ld;
-          LT yb[CT], xa[BQ];
+      if (Q > 0 && k0 > 0) {
+        // k0 is a multiple of NB (>= 16): groups of U = 4 MFMA k-steps (16
+        // columns of L), the next group's fragments in flight while the
+        // current group is multiplied
+        constexpr int U = 4;
+        LT xa0[U][BQ], yb0[U][CT], xa1[U][BQ], yb1[U][CT];
+        auto load = [&](int p0, LT (&xa)[U][BQ], LT (&yb)[U][CT]) {
 #pragma unroll
-          for (int c = 0; c < CT; c++) {
-            const int j = k0 + c * 16 + l16;
-            yb[c] = j < k0 + w ? Lc[j] : (LT)0;
-          }
+          for (int u = 0; u < U; u++) {
+            const LT *Lc = L + (long)(p0 + 4 * u + (lane >> 4)) * ld;
 #pragma unroll
-          for (int q = 0; q < BQ; q++) {
-            const int i = k0 + (rtb + 4 * q) * 16 + l16;
-            xa[q] = (q < Q && i < n) ? Lc[i] : (LT)0;
-          }
-#pragma unroll
-          for (int q = 0; q < BQ; q++)
-            if (q < Q) {
-#pragma unroll
-              for (int c = 0; c < CT; c++) acc[q][c] = M::op(yb[c], xa[q], acc[q][c]);
+            for (int c = 0; c < CT; c++) {
+              const int j = k0 + c * 16 + l16;
+              yb[u][c] = j < k0 + w ? Lc[j] : (LT)0;
             }
+#pragma unroll
+            for (int q = 0; q < BQ; q++) {
+              const int i = k0 + (rtb + 4 * q) * 16 + l16;
+              xa[u][q] = (q < Q && i < n) ? Lc[i] : (LT)0;
+            }
+          }
+        };
+        auto mul = [&](LT (&xa)[U][BQ], LT (&yb)[U][CT]) {
+#pragma unroll
+          for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < BQ; q++)
+              if (q < Q) {
+#pragma unroll
+                for (int c = 0; c < CT; c++) acc[q][c] = M::op(yb[u][c], xa[u][q], acc[q][c]);
+              }
+        };
+        const int ng = k0 / (4 * U);
+        load(0, xa0, yb0);
+        for (int g = 0; g < ng; g += 2) {
+          if (g + 1 < ng) load((g + 1) * 4 * U, xa1, yb1);
+          mul(xa0, yb0);
+          if (g + 1 < ng) {
+            if (g + 2 < ng) load((g + 2) * 4 * U, xa0, yb0);
+            mul(xa1, yb1);
+          }
         }
       }
       // lane holds C(i = (rtb+4q)*16 + l16, j = c*16 + row(lane, r)), local indices
@@ -194,10 +217,17 @@ __device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
       const int i = lane & 15, r = wave * 4 + (lane >> 4);
       const bool act = r < nrhs && i < jb;
       double v = act ? y[r * RS + j0 + i] : 0.0;
-      for (int jj = 0; jj < jb; jj++) {
-        if (act && i == jj) v = v / (double)L[(long)(j0 + jj) * ld + j0 + jj];
-        const double yj = __shfl(v, (lane & 48) | jj);
-        if (act && i > jj) v -= (double)L[(long)(j0 + jj) * ld + j0 + i] * yj;
+      double lrow[16];  // row i of the diagonal block, loaded before the dependent chain
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++)
+        lrow[jj] = (act && jj <= i) ? (double)L[(long)(j0 + jj) * ld + j0 + i] : 1.0;
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) {
+        if (jj < jb) {
+          if (act && i == jj) v = v / lrow[jj];
+          const double yj = __shfl(v, (lane & 48) | jj);
+          if (act && i > jj) v -= lrow[jj] * yj;
+        }
       }
       if (act) y[r * RS + j0 + i] = v;
     }
@@ -225,10 +255,17 @@ __device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
       const int i = lane & 15, r = wave * 4 + (lane >> 4);
       const bool act = r < nrhs && i < jb;
       double v = act ? y[r * RS + j0 + i] : 0.0;
-      for (int jj = jb - 1; jj >= 0; jj--) {
-        if (act && i == jj) v = v / (double)L[(long)(j0 + jj) * ld + j0 + jj];
-        const double xj = __shfl(v, (lane & 48) | jj);
-        if (act && i < jj) v -= (double)L[(long)(j0 + i) * ld + j0 + jj] * xj;
+      double lcol[16];  // column i of the diagonal block (row i of its transpose)
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++)
+        lcol[jj] = (act && jj >= i && jj < jb) ? (double)L[(long)(j0 + i) * ld + j0 + jj] : 1.0;
+#pragma unroll
+      for (int jj = 15; jj >= 0; jj--) {
+        if (jj < jb) {
+          if (act && i == jj) v = v / lcol[jj];
+          const double xj = __shfl(v, (lane & 48) | jj);
+          if (act && i < jj) v -= lcol[jj] * xj;
+        }
       }
       if (act) y[r * RS + j0 + i] = v;
     }
@@ -365,7 +402,7 @@ __device__ double residual_system(const double *Ad, const double *Bd, int n, int
   return eta;
 }
 
-template <typename LT, int NB, bool IR>
+template <typename LT, int NB, int BQ, bool IR>
 __global__ __launch_bounds__(BT) void batched_chol_kernel(batched_args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_idx;
@@ -390,7 +427,7 @@ __global__ __launch_bounds__(BT) void batched_chol_kernel(batched_args a) {
       continue;
     }
     LT *L = IR ? (LT *)(a.L32 + d.a_off) : (LT *)(a.A + d.a_off);
-    const int fail = factor_system<LT, NB>(d, a, L, (LT *)smem, s_rsq);
+    const int fail = factor_system<LT, NB, BQ>(d, a, L, (LT *)smem, s_rsq);
     if (tid == 0) a.info[d.id] = fail;
     if (fail) {
       __syncthreads();
@@ -454,22 +491,24 @@ __global__ void batched_reset_kernel(unsigned *counter) {
 // ---- kernel selection ----
 typedef void (*kfn)(batched_args);
 struct kvariant {
-  int mode, nb;
+  int mode, nb, bq;
   kfn fn;
   int lt_size;
 };
 const kvariant KVARIANTS[] = {
-    {PLA_BATCHED_FP64, 16, batched_chol_kernel<double, 16, false>, 8},
-    {PLA_BATCHED_FP64, 32, batched_chol_kernel<double, 32, false>, 8},
-    {PLA_BATCHED_FP32_IR, 16, batched_chol_kernel<float, 16, true>, 4},
-    {PLA_BATCHED_FP32_IR, 32, batched_chol_kernel<float, 32, true>, 4},
-    {PLA_BATCHED_FP32_IR, 64, batched_chol_kernel<float, 64, true>, 4},
+    {PLA_BATCHED_FP64, 16, 4, batched_chol_kernel<double, 16, 4, false>, 8},
+    {PLA_BATCHED_FP64, 16, 8, batched_chol_kernel<double, 16, 8, false>, 8},
+    {PLA_BATCHED_FP64, 32, 4, batched_chol_kernel<double, 32, 4, false>, 8},
+    {PLA_BATCHED_FP32_IR, 16, 8, batched_chol_kernel<float, 16, 8, true>, 4},
+    {PLA_BATCHED_FP32_IR, 32, 4, batched_chol_kernel<float, 32, 4, true>, 4},
+    {PLA_BATCHED_FP32_IR, 32, 8, batched_chol_kernel<float, 32, 8, true>, 4},
+    {PLA_BATCHED_FP32_IR, 64, 4, batched_chol_kernel<float, 64, 4, true>, 4},
 };
 constexpr size_t LDS_MAX = 160 * 1024 - 4096;  // dynamic part; the rest is static
 
-const kvariant *find_variant(int mode, int nb) {
+const kvariant *find_variant(int mode, int nb, int bq) {
   for (const auto &v : KVARIANTS)
-    if (v.mode == mode && v.nb == nb) return &v;
+    if (v.mode == mode && v.nb == nb && v.bq == bq) return &v;
   return nullptr;
 }
 
@@ -479,8 +518,11 @@ size_t lds_bytes(const kvariant &v, int rs, int nrhs) {
   return std::max(panel, vec);
 }
 
-// tuned (nb, workgroups per CU) by problem signature, for the process lifetime
-std::map<std::tuple<int, int, int, long, long, int>, std::pair<int, int>> g_tuned;
+struct bconfig {
+  int nb, bq, wpc;  // block size, row tiles per wave, workgroups per CU
+};
+// tuned configuration by problem signature, for the process lifetime
+std::map<std::tuple<int, int, int, long, long, int>, bconfig> g_tuned;
 
 }  // namespace
 
@@ -496,7 +538,7 @@ struct PLA_Batched_plan_struct {
   bsys *d_sys = nullptr;
   unsigned *d_counter = nullptr;
   float *d_L32 = nullptr;
-  int nb = 0, wpc = 0;  // chosen block size and workgroups per CU
+  int nb = 0, bq = 0, wpc = 0;  // chosen block size, row tiles per wave, workgroups per CU
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -548,7 +590,7 @@ int build_graph(PLA_Batched_plan p) {
     PLA_HIP(hipGraphDestroy(p->graph));
     p->graph = nullptr;
   }
-  const kvariant *v = find_variant(p->mode, p->nb);
+  const kvariant *v = find_variant(p->mode, p->nb, p->bq);
   if (!v) return fail(PLA_ERR_INVALID_PARAM, "batched: no kernel for nb = %d", p->nb);
   if (max_wpc(*v, p->rs, p->nrhs) < p->wpc)
     return fail(PLA_ERR_INVALID_PARAM, "batched: %d workgroups per CU do not fit", p->wpc);
@@ -567,13 +609,13 @@ int build_graph(PLA_Batched_plan p) {
   return PLA_SUCCESS;
 }
 
-// Candidates (nb, workgroups per CU) that fit this plan's sizes.
-std::vector<std::pair<int, int>> candidates(PLA_Batched_plan p) {
-  std::vector<std::pair<int, int>> c;
+// Candidate configurations that fit this plan's sizes.
+std::vector<bconfig> candidates(PLA_Batched_plan p) {
+  std::vector<bconfig> c;
   for (const auto &v : KVARIANTS) {
     if (v.mode != p->mode) continue;
     const int m = max_wpc(v, p->rs, p->nrhs);
-    for (int w = 1; w <= std::min(m, 2); w++) c.push_back({v.nb, w});
+    for (int w = 1; w <= std::min(m, 2); w++) c.push_back({v.nb, v.bq, w});
   }
   return c;
 }
@@ -649,16 +691,19 @@ int PLA_Batched_plan_create(int count, const int *n, const long *a_off, const lo
   }
   const auto key = std::make_tuple(p->mode, count, nrhs, p->sum_n, p->sum_n2, p->nmax);
   auto it = g_tuned.find(key);
+  bconfig cfg = cand[0];
   if (it != g_tuned.end()) {
-    p->nb = it->second.first;
-    p->wpc = it->second.second;
+    cfg = it->second;
   } else {
-    // untuned default: the largest block size that fits, one workgroup per CU
-    p->nb = 0;
+    // untuned default: the largest block size that fits at one workgroup per
+    // CU, with the most row tiles per wave
     for (auto &c : cand)
-      if (c.second == 1) p->nb = std::max(p->nb, c.first);
-    p->wpc = 1;
+      if (c.wpc == 1 && (c.nb > cfg.nb || (c.nb == cfg.nb && c.bq > cfg.bq))) cfg = c;
+    cfg.wpc = 1;
   }
+  p->nb = cfg.nb;
+  p->bq = cfg.bq;
+  p->wpc = cfg.wpc;
   if ((rc = build_graph(p))) {
     PLA_Batched_plan_free(&p);
     return rc;
@@ -701,10 +746,10 @@ int PLA_Batched_plan_autotune(PLA_Batched_plan p, double *ms_best) {
   if (e == hipSuccess) e = hipEventCreate(&e1);
   int rc = PLA_SUCCESS;
   double best = 1e300;
-  std::pair<int, int> best_c{p->nb, p->wpc};
+  bconfig best_c{p->nb, p->bq, p->wpc};
   if (e != hipSuccess) rc = hip_fail(e, "PLA_Batched_plan_autotune: scratch allocation");
   for (size_t ci = 0; ci < cand.size() && rc == PLA_SUCCESS; ci++) {
-    const kvariant *v = find_variant(p->mode, cand[ci].first);
+    const kvariant *v = find_variant(p->mode, cand[ci].nb, cand[ci].bq);
     (void)max_wpc(*v, p->rs, p->nrhs);  // sets the kernel's dynamic LDS limit
     double t_min = 1e300;
     for (int rep = 0; rep < 2 && rc == PLA_SUCCESS; rep++) {
@@ -715,7 +760,7 @@ int PLA_Batched_plan_autotune(PLA_Batched_plan p, double *ms_best) {
         rc = hip_fail(e, "PLA_Batched_plan_autotune");
         break;
       }
-      rc = launch_into(p, *v, cand[ci].second, As ? As : p->A, Bs, info_s, iters_s, berr_s, s);
+      rc = launch_into(p, *v, cand[ci].wpc, As ? As : p->A, Bs, info_s, iters_s, berr_s, s);
       if (rc) break;
       e = hipEventRecord(e1, s);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -740,8 +785,9 @@ int PLA_Batched_plan_autotune(PLA_Batched_plan p, double *ms_best) {
   (void)hipFree(iters_s);
   (void)hipFree(berr_s);
   if (rc) return rc;
-  p->nb = best_c.first;
-  p->wpc = best_c.second;
+  p->nb = best_c.nb;
+  p->bq = best_c.bq;
+  p->wpc = best_c.wpc;
   g_tuned[std::make_tuple(p->mode, p->count, p->nrhs, p->sum_n, p->sum_n2, p->nmax)] = best_c;
   if (ms_best) *ms_best = best;
   return build_graph(p);

@@ -13,8 +13,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 
 struct pla_base {
@@ -74,7 +76,15 @@ int ensure_device();                 // lazily initialise device + streams
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
-constexpr int PLA_NUM_WS = 24;
+constexpr int PLA_NUM_WS = 48;
+// slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
+// residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
+// 29-33 GMRES, 34-42 mixed Cholesky
+enum {
+  WS_SRES_PART = 24, WS_SRES_MAX, WS_ONES, WS_NORM_R, WS_MDOT,
+  WS_GM_V = 29, WS_GM_W, WS_GM_H, WS_GM_T, WS_GM_Z,
+  WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R
+};
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
 
@@ -194,6 +204,48 @@ int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info
 // ---- kernel timing (hipEvents on the library stream, per role) ----
 // role 1: trailing update of PLA_LU (GEMM); role 2: trailing update of PLA_Chol (SYRK)
 bool timing_on();
+// ---- refinement (pla_krylov.hip, pla_refine.cpp, pla_fp8.hip) ----
+int sym_residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *X, long ldx,
+                 const double *B, long ldb, double *R, long ldr, double *rmax, bool abs_mode = false);
+int sym_inf_norm(hipStream_t s, int n, const double *A, long lda, double *out);
+int k_fill(hipStream_t s, long count, double v, double *X);
+int k_mdot(hipStream_t s, int n, int nrhs, int k, const double *V, long vstride, long ldv, const double *W,
+           long ldw, double *out);
+int k_maxpy(hipStream_t s, int n, int nrhs, int k, const double *V, long vstride, long ldv, const double *h,
+            double alpha, double *W, long ldw);
+int k_scale_cols(hipStream_t s, int n, int nrhs, double *W, long ldw, const double *f, bool inv);
+int k_inv_diag_f32_op(hipStream_t s, bool lower, bool trans, bool unit, int n, const float *A, long lda,
+                      double *Dinv);
+int fp8_tile_rows();
+int k_fp8_quantize(hipStream_t s, int R, int b, const double *P, long ldp, unsigned char *Lq, int *exps);
+int k_fp8_syrk_lower(hipStream_t s, int R, const unsigned char *Lq, const int *exps, float *C, long ldc,
+                     const int *abort_flag);
+int k_cvt_lower_shift(hipStream_t s, int n, const double *A, long lda, double shift, float *W, long ldw);
+
+// Refinement of X for A X = B with a low-precision factorization M:
+// residual(X, B, R, rmax) computes R = B - A X in FP64 (B == nullptr: -A X;
+// rmax == nullptr: no norms, asynchronous); precond(Y, Z) computes
+// Z = M^{-1} Y.  Classic refinement first; with gmres_fallback, GMRES-IR
+// (restart steps, relative tolerance gmres_tol) once a classic step fails
+// to halve the backward error.
+struct refine_ops {
+  int n = 0, nrhs = 0;
+  int restart = 50;
+  double gmres_tol = 1e-10;
+  std::function<int(const double *X, long ldx, const double *B, long ldb, double *R, long ldr, double *rmax)>
+      residual;
+  std::function<int(const double *Y, long ldy, double *Z, long ldz)> precond;
+};
+struct refine_stats {
+  int outer = 0;        // refinement steps (corrections applied)
+  int inner = 0;        // GMRES steps in total
+  int gmres_outer = 0;  // corrections computed by GMRES
+  double eta = INFINITY;
+};
+int refine(hipStream_t s, const refine_ops &op, double anorm, const double *B, long ldb, double *X, long ldx,
+           double tol, int max_iter, bool gmres_fallback, refine_stats *st);
+void set_last_refine(const refine_stats &st, double shift);
+
 void timing_begin(hipStream_t s, int role);
 void timing_end(hipStream_t s, int role, double flops);
 

@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void resid_final_kernel(int n, int nrhs, int n
     comp += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
     s = t;
   }
-  const double bq = B[(long)q * ldb + i];
+  const double bq = B ? B[(long)q * ldb + i] : 0.0;
   const double r = (bq - s) - comp;
   R[(long)q * ldr + i] = r;
   atomicMax(rmax + q, (unsigned long long)__double_as_longlong(fabs(r)));
@@ -687,12 +687,33 @@ int k_inv_diag_f32(hipStream_t s, bool lower, bool unit, int n, const float *A, 
   return PLA_SUCCESS;
 }
 
+// inverse diagonal blocks of op(A) for trsv_sf_f32 with the same (lower, trans, unit)
+int k_inv_diag_f32_op(hipStream_t s, bool lower, bool trans, bool unit, int n, const float *A, long lda,
+                      double *Dinv) {
+  const int nblk = (n + TB - 1) / TB;
+  if (n <= 0) return PLA_SUCCESS;
+#define PLA_INV(LO, TR, UN) \
+  hipLaunchKernelGGL((inv_diag_kernel<float, LO, TR, UN>), dim3(nblk), dim3(TB), 0, s, n, A, lda, Dinv)
+  if (lower && !trans && unit) PLA_INV(true, false, true);
+  else if (lower && !trans && !unit) PLA_INV(true, false, false);
+  else if (lower && trans && unit) PLA_INV(true, true, true);
+  else if (lower && trans && !unit) PLA_INV(true, true, false);
+  else if (!lower && !trans && unit) PLA_INV(false, false, true);
+  else if (!lower && !trans && !unit) PLA_INV(false, false, false);
+  else if (!lower && trans && unit) PLA_INV(false, true, true);
+  else PLA_INV(false, true, false);
+#undef PLA_INV
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
 int trsv_block_size() { return TB; }
 int trsv_max_rhs() { return MAXRHS; }
 
 // ---- residual and vector norms -----------------------------------------------
 
-// R = B - A X; rmax[q] = max |R(:, q)| (double).  Synchronous.
+// R = B - A X (B == nullptr: R = -A X); rmax[q] = max |R(:, q)| (double),
+// synchronous; rmax == nullptr: asynchronous, no norms.
 int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const double *X, long ldx,
              const double *B, long ldb, double *R, long ldr, double *rmax) {
   if (nrhs > MAXRHS) return fail(PLA_ERR_INVALID_PARAM, "residual: %d right-hand sides", nrhs);
@@ -711,6 +732,7 @@ int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const do
   hipLaunchKernelGGL(resid_final_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nrhs, nch, part, B, ldb,
                      R, ldr, mx);
   PLA_HIP(hipGetLastError());
+  if (!rmax) return PLA_SUCCESS;
   unsigned long long h[MAXRHS];
   PLA_HIP(hipMemcpyAsync(h, mx, sizeof(unsigned long long) * nrhs, hipMemcpyDeviceToHost, s));
   PLA_HIP(hipStreamSynchronize(s));

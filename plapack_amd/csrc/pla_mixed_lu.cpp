@@ -162,9 +162,7 @@ int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision,
   int *dperm = (int *)workspace(sizeof(int) * n, WS_PERM);
   double *DinvL = (double *)workspace(dsz, WS_DINVL);
   double *DinvU = (double *)workspace(dsz, WS_DINVU);
-  double *D = (double *)workspace(sizeof(double) * (size_t)n * nrhs, WS_D);
-  double *R = (double *)workspace(sizeof(double) * (size_t)n * nrhs, WS_R);
-  if (!dperm || !DinvL || !DinvU || !D || !R) return PLA_ERR_NO_MEMORY;
+  if (!dperm || !DinvL || !DinvU) return PLA_ERR_NO_MEMORY;
   PLA_HIP(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
   if ((rc = k_inv_diag_f32(s, true, true, n, W, ldw, DinvL))) return rc;
   if ((rc = k_inv_diag_f32(s, false, false, n, W, ldw, DinvU))) return rc;
@@ -178,29 +176,20 @@ int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision,
 
   double anorm = 0.0;
   if ((rc = k_norm(s, true, n, n, a, lda, &anorm))) return rc;
-  std::vector<double> bmax(nrhs), xmax(nrhs), rmax(nrhs);
-  if ((rc = col_amax(s, n, nrhs, b, ldb, bmax.data()))) return rc;
-  if ((rc = solve(b, ldb, x, ldx))) return rc;
-  int it = 0;
-  double eta = INFINITY;
-  for (;;) {
-    if ((rc = residual(s, n, nrhs, a, lda, x, ldx, b, ldb, R, n, rmax.data()))) return rc;
-    if ((rc = col_amax(s, n, nrhs, x, ldx, xmax.data()))) return rc;
-    eta = 0.0;
-    for (int q = 0; q < nrhs; q++) {
-      const double den = anorm * xmax[q] + bmax[q];
-      const double e = den > 0.0 ? rmax[q] / den : (rmax[q] > 0.0 ? INFINITY : 0.0);
-      if (!(e <= eta)) eta = e;  // NaN propagates
-    }
-    if (eta <= tol || it >= max_iter) break;
-    if ((rc = solve(R, n, D, n))) return rc;
-    if ((rc = k_axpy_cols(s, n, nrhs, D, n, x, ldx))) return rc;
-    it++;
-  }
+  refine_ops op;
+  op.n = n;
+  op.nrhs = nrhs;
+  op.residual = [&](const double *Xv, long ldxv, const double *Bv, long ldbv, double *Rv, long ldr, double *rmax) {
+    return residual(s, n, nrhs, a, lda, Xv, ldxv, Bv, ldbv, Rv, ldr, rmax);
+  };
+  op.precond = solve;
+  refine_stats st;
+  if ((rc = refine(s, op, anorm, b, ldb, x, ldx, tol, max_iter, true, &st))) return rc;
   PLA_HIP(hipStreamSynchronize(s));
   if ((rc = leaf_check_error(s))) return rc;
-  if (iterations) *iterations = it;
-  if (backward_error) *backward_error = eta;
+  set_last_refine(st, 0.0);
+  if (iterations) *iterations = st.outer;
+  if (backward_error) *backward_error = st.eta;
   if ((rc = written(X))) return rc;
   return written(pivots);
 }

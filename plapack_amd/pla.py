@@ -47,6 +47,7 @@ PLA_MIXED_FP16 = 1
 PLA_MIXED_BF16 = 2
 PLA_MIXED_FP16X3 = 3
 PLA_MIXED_BF16X3 = 4
+PLA_MIXED_FP8 = 5
 PLA_PARAM_LOOKAHEAD = 1
 PLA_PARAM_LEAF_ROWS = 2
 PLA_BATCHED_FP64 = 1
@@ -119,6 +120,8 @@ _SIGS = {
     "PLA_General_solve": [_obj, _obj, _obj],
     "PLA_Pos_def_solve": [_obj, _obj],
     "PLA_LU_mixed": [_obj, _obj, _obj, _obj, _i, _d, _i, _pi, ctypes.POINTER(_d)],
+    "PLA_Chol_mixed": [_obj, _obj, _obj, _i, _d, _i, _pi, ctypes.POINTER(_d)],
+    "PLA_Mixed_stats": [_pi, _pi, _pi, ctypes.POINTER(_d)],
     "PLA_Batched_plan_create": [_i, _pi, _p, _p, _i, _i, _d, _i, _p, _p, _p, _p, _p, _pobj],
     "PLA_Batched_plan_execute": [_obj],
     "PLA_Batched_plan_autotune": [_obj, ctypes.POINTER(_d)],

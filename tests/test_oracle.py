@@ -224,3 +224,42 @@ def test_lu_and_chol_solve(ora):
     l, info = ora.chol_blocked(s, nb=16)
     x = ora.chol_solve(np.tril(l), s @ np.ones(80))
     assert info == 0 and np.abs(x[:, 0] - 1).max() < 1e-10
+
+
+# ---- E4M3 rounding of the oracle, pinned by the OCP OFP8 specification ----
+
+def test_e4m3_encodings_from_specification(ora):
+    # (value, byte): max normal, min normal, min subnormal, 1, -2, 0.1 -> 0.1015625
+    cases = [(448.0, 0x7E), (2.0 ** -6, 0x08), (2.0 ** -9, 0x01), (1.0, 0x38), (-2.0, 0xC0),
+             (0.1015625, 0x1D), (0.0, 0x00), (240.0, 0x77), (-448.0, 0xFE)]
+    for v, byte in cases:
+        assert int(ora.e4m3_encode(np.array([v]))[0]) == byte, v
+        assert ora.e4m3_decode(np.array([byte], np.uint8))[0] == v
+
+
+def test_e4m3_rounding_ties_and_subnormals(ora):
+    r = ora.e4m3_round
+    assert r(0.1) == 0.1015625
+    assert r(1.0625) == 1.0 and r(1.1875) == 1.25           # ties to even mantissa
+    assert r(0.001) == 2.0 ** -9 and r(0.0009) == 0.0        # subnormal grid 2^-9
+    assert r(2.0 ** -10) == 0.0 and r(3 * 2.0 ** -10) == 2 * 2.0 ** -9  # ties to even
+    assert r(440.0) == 448.0 and r(-3.3) == -3.25
+    # every representable value maps to itself; midpoints go to the even neighbour
+    vals = np.unique(np.abs(ora.e4m3_decode(np.arange(256, dtype=np.uint8))))
+    vals = vals[np.isfinite(vals)]
+    vals = vals[vals <= 448]
+    assert np.array_equal(r(vals), vals)
+    mids = (vals[:-1] + vals[1:]) / 2
+    got = r(mids)
+    lo_even = (ora.e4m3_encode(vals[:-1]) & 1) == 0
+    assert np.array_equal(got, np.where(lo_even, vals[:-1], vals[1:]))
+
+
+def test_fp8_tile_quantization_scales(ora):
+    rng = np.random.default_rng(0)
+    p = rng.uniform(-1, 1, (600, 200)) * np.repeat([3e-3, 17.0, 1e5], [256, 256, 88])[:, None]
+    q, ex = ora.fp8_quantize_tiles(p)
+    assert q.shape == (600, 256) and ex.tolist() == [-17, -4, 8]
+    deq = ora.e4m3_decode(q[:, :200]) * np.exp2(np.repeat(ex, [256, 256, 88]))[:, None]
+    assert np.abs(deq - p).max() <= 2.0 ** -4 * np.abs(p).max()     # unit roundoff 2^-4
+    assert (q[:, 200:] == 0).all()
