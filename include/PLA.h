@@ -351,9 +351,21 @@ int PLA_Set_device_nb(int nb);
 /* Tuning parameters of the device factorizations (an autotuner sets them).
    PLA_PARAM_LOOKAHEAD: 1 (default) factors panel k+1 on a second stream while
    step k's trailing update runs; 0 runs everything on one stream.
-   PLA_PARAM_LEAF_ROWS: target rows per workgroup in the panel-LU leaf. */
+   PLA_PARAM_LEAF_ROWS: target rows per workgroup in the panel-LU leaf.
+   PLA_PARAM_PANEL_CUS: CUs reserved for the LU panel factorization while the
+   trailing update runs on the others (a multiple of 32 -- the same number on
+   every shader engine of every XCD -- below the device's CU count; 0 lets the
+   panel share every CU).
+   PLA_PARAM_LEAF_WIDTH: panel-LU leaf width, 8, 16 or 32 (0: the widest
+   that fits).
+   PLA_PARAM_LEAF_KIND: 0 (default) keeps the leaf's rows in registers, one
+   wave per workgroup and no LDS, so it runs beside the trailing update; 1
+   keeps them in LDS, four waves per workgroup. */
 #define PLA_PARAM_LOOKAHEAD 1
 #define PLA_PARAM_LEAF_ROWS 2
+#define PLA_PARAM_PANEL_CUS 3
+#define PLA_PARAM_LEAF_WIDTH 4
+#define PLA_PARAM_LEAF_KIND 5
 int PLA_Set_device_param(int param, int value);
 int PLA_Get_device_nb(int n, int *nb);
 /* Per-launch hipEvent timing of the trailing-update kernels on the library

@@ -79,16 +79,19 @@ struct panel_rec {
   int *piv;
   int *pairs;
   int nleaves;
+  int leaf;       // leaf width
+  int gcap;       // workgroup cap of a leaf launch
 };
 
 // Sub-panel whose first row and column are c0 inside the panel.
 static int lu_panel_rec(hipStream_t s, panel_rec &pr, int c0, int w) {
-  const int leaf = leaf_max_width();
+  const int leaf = pr.leaf;
   const long lda = pr.lda;
   double *P = pr.panel + c0 * lda + c0;
   const int mp = pr.mp - c0;
   if (w <= leaf) {
-    if (pr.nleaves >= 16) return fail(PLA_ERR_INVALID_PARAM, "lu panel: more than 16 leaves");
+    if (pr.nleaves >= LU_MAX_LEAVES)
+      return fail(PLA_ERR_INVALID_PARAM, "lu panel: more than %d leaves", LU_MAX_LEAVES);
     lu_leaf_call c;
     c.P = P;
     c.Ppanel = pr.panel + c0;
@@ -99,6 +102,8 @@ static int lu_panel_rec(hipStream_t s, panel_rec &pr, int c0, int w) {
     c.pc0 = c0;
     c.piv = pr.piv + c0;
     c.pairs = pr.pairs + (size_t)pr.nleaves * leaf_pairs_stride();
+    c.gcap = pr.gcap;
+    c.W = pr.leaf;
     pr.nleaves++;
     return k_lu_leaf(s, c);
   }
@@ -113,8 +118,16 @@ static int lu_panel_rec(hipStream_t s, panel_rec &pr, int c0, int w) {
   return lu_panel_rec(s, pr, c0 + w0, w1);
 }
 
-int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *pairs, int *nleaves) {
-  panel_rec pr{P, lda, mp, w, piv, pairs, 0};
+int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *pairs, int *nleaves, int gcap) {
+  // gcap counts CUs; a register-leaf wave takes one SIMD, four per CU
+  const bool regs = ctx().leaf_kind == 0;
+  int leaf = ctx().leaf_width > 0 ? ctx().leaf_width
+                                  : (regs ? rleaf_width_for(mp, gcap * 4) : leaf_width_for(mp, gcap));
+  if (leaf <= 0) return fail(PLA_ERR_DIMENSIONS, "lu panel: %d rows are too many for the panel leaf", mp);
+  if (regs) gcap *= 4;
+  // the leaves of one panel are recorded for the composite interchange
+  while ((w + leaf - 1) / leaf > LU_MAX_LEAVES && leaf < leaf_max_width()) leaf *= 2;
+  panel_rec pr{P, lda, mp, w, piv, pairs, 0, leaf, gcap};
   const int rc = w > 0 ? lu_panel_rec(s, pr, 0, w) : PLA_SUCCESS;
   *nleaves = pr.nleaves;
   return rc;

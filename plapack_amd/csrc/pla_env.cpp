@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 int PLA_ERROR_CHECKING = 1;            // OBJ/PLA_Diagnostics.c:16-19
 int PLA_CHECK_PARAMETERS = 1;
@@ -87,6 +88,76 @@ void *workspace(size_t bytes, int slot, bool zero) {
   return w.p;
 }
 
+// CU-masked stream pair for the LU look-ahead.  Logical CU i of a mask lives
+// on XCD i % 8 and shader engine (i / 8) % 4 of it, so bits [0, k) with k a
+// multiple of 32 give every shader engine of every XCD k / 32 CUs.  Uneven
+// counts per shader engine are avoided: the panel leaf needs all of its
+// workgroups resident at once, and workgroups are spread evenly over the
+// shader engines whatever CUs each one has free.
+static int make_masked_streams(context &c, int k) {
+  if (c.masked_cus == k && c.stream_panel) return PLA_SUCCESS;
+  if (c.stream_panel) {
+    PLA_HIP(hipStreamSynchronize(c.stream_panel));
+    PLA_HIP(hipStreamSynchronize(c.stream_trail));
+    PLA_HIP(hipStreamDestroy(c.stream_panel));
+    PLA_HIP(hipStreamDestroy(c.stream_trail));
+    c.stream_panel = c.stream_trail = nullptr;
+    c.masked_cus = 0;
+  }
+  const int words = (c.cu_count + 31) / 32;
+  std::vector<uint32_t> mp(words, 0u), mt(words, 0u);
+  for (int i = 0; i < c.cu_count; i++) {
+    if (i < k) mp[i / 32] |= 1u << (i % 32);
+    else mt[i / 32] |= 1u << (i % 32);
+  }
+  PLA_HIP(hipExtStreamCreateWithCUMask(&c.stream_panel, (uint32_t)words, mp.data()));
+  PLA_HIP(hipExtStreamCreateWithCUMask(&c.stream_trail, (uint32_t)words, mt.data()));
+  c.masked_cus = k;
+  return PLA_SUCCESS;
+}
+
+static hipEvent_t g_ev_join[3];
+
+int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, int *gcap) {
+  context &c = g_ctx;
+  *gcap = 0;
+  if (!c.lookahead) {
+    *sP = *sT = main;
+    return PLA_SUCCESS;
+  }
+  if (!g_ev_join[0])
+    for (auto &e : g_ev_join) PLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int k = c.panel_cus;
+  if (k > 0 && c.cu_count % 32 == 0 && k < c.cu_count) {
+    // enough CUs that the first (tallest) panel's leaf rows fit in LDS
+    while (k + 32 < c.cu_count && (c.leaf_kind ? leaf_width_for(m, k) : rleaf_width_for(m, 4 * k)) == 0) k += 32;
+    int rc = make_masked_streams(c, k);
+    if (rc) return rc;
+    *sP = c.stream_panel;
+    *sT = c.stream_trail;
+    *gcap = k;
+  } else {
+    *sP = c.stream_la;
+    *sT = main;
+  }
+  PLA_HIP(hipEventRecord(g_ev_join[0], main));
+  PLA_HIP(hipStreamWaitEvent(*sP, g_ev_join[0], 0));
+  if (*sT != main) PLA_HIP(hipStreamWaitEvent(*sT, g_ev_join[0], 0));
+  return PLA_SUCCESS;
+}
+
+int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT) {
+  if (sP != main) {
+    PLA_HIP(hipEventRecord(g_ev_join[1], sP));
+    PLA_HIP(hipStreamWaitEvent(main, g_ev_join[1], 0));
+  }
+  if (sT != main) {
+    PLA_HIP(hipEventRecord(g_ev_join[2], sT));
+    PLA_HIP(hipStreamWaitEvent(main, g_ev_join[2], 0));
+  }
+  return PLA_SUCCESS;
+}
+
 static leaf_ws g_leaf;
 
 leaf_ws &leaf_workspace(size_t slot_bytes) {
@@ -112,6 +183,21 @@ leaf_ws &leaf_workspace(size_t slot_bytes) {
   g_leaf.slots = (unsigned long long *)p;
   g_leaf.err = (int *)((char *)p + slot_bytes);
   return g_leaf;
+}
+
+// Exchange tags of the panel leaves: one counter for every leaf kernel (they
+// share the slot words), so a stale word never carries a current tag.
+static unsigned g_leaf_tag = 1;
+
+int leaf_tags(hipStream_t s, unsigned count, unsigned *tag0) {
+  if (!g_leaf.base) return fail(PLA_ERR_INVALID_PARAM, "panel exchange workspace not allocated");
+  if (g_leaf_tag > 0xF0000000u) {  // tags only grow; restart far below wrap-around
+    PLA_HIP(hipMemsetAsync(g_leaf.slots, 0, g_leaf.slot_bytes, s));
+    g_leaf_tag = 1;
+  }
+  *tag0 = g_leaf_tag;
+  g_leaf_tag += count;
+  return PLA_SUCCESS;
 }
 
 int *leaf_error_word() {
@@ -212,6 +298,8 @@ int PLA_Finalize(void) {
   pla::context &c = pla::ctx();
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   if (c.stream_la) (void)hipStreamSynchronize(c.stream_la);
+  if (c.stream_panel) (void)hipStreamSynchronize(c.stream_panel);
+  if (c.stream_trail) (void)hipStreamSynchronize(c.stream_trail);
   pla::release_workspaces();
   g_initialized = 0;
   c.initialized = false;
@@ -335,6 +423,20 @@ int PLA_Set_device_param(int param, int value) {
     case PLA_PARAM_LEAF_ROWS:
       if (value < 32 || value > 1024) return fail(PLA_ERR_INVALID_PARAM, "leaf rows must be in [32, 1024]");
       c.leaf_rows = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_PANEL_CUS:
+      if (value < 0 || value % 32 != 0 || (value > 0 && value >= c.cu_count))
+        return fail(PLA_ERR_INVALID_PARAM, "panel CUs must be a multiple of 32 below %d", c.cu_count);
+      c.panel_cus = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_LEAF_KIND:
+      if (value != 0 && value != 1) return fail(PLA_ERR_INVALID_PARAM, "leaf kind must be 0 or 1");
+      c.leaf_kind = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_LEAF_WIDTH:
+      if (value != 0 && value != 8 && value != 16 && value != 32)
+        return fail(PLA_ERR_INVALID_PARAM, "leaf width must be 0, 8, 16 or 32");
+      c.leaf_width = value;
       return PLA_SUCCESS;
     default:
       return fail(PLA_ERR_INVALID_PARAM, "unknown device parameter %d", param);

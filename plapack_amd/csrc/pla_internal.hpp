@@ -70,9 +70,29 @@ struct context {
   // tuning parameters (PLA_Set_device_param)
   int lookahead = 1;        // factor panel k+1 beside step k's trailing update
   int leaf_rows = 256;      // target rows per workgroup of the panel leaf
+  int panel_cus = 0;        // CUs reserved for the LU panel (0: panel shares every CU)
+  int leaf_width = 0;       // panel-LU leaf width (0: the widest that fits)
+  int leaf_kind = 0;        // panel-LU leaf: 0 rows in registers, 1 rows in LDS
+  unsigned leaf_spin_limit = 400000u;  // bounded wait of a panel-leaf exchange (polls)
+  // CU-masked stream pair of the LU look-ahead: the panel stream runs on
+  // masked_cus CUs (masked_cus / 8 per XCD), the trailing stream on the rest
+  hipStream_t stream_panel = nullptr, stream_trail = nullptr;
+  int masked_cus = 0;
 };
 context &ctx();
 int ensure_device();                 // lazily initialise device + streams
+
+// Streams of a blocked LU with look-ahead.  *sP receives the panel
+// factorizations, *sT the interchanges and trailing updates.  With panel CUs
+// reserved (PLA_PARAM_PANEL_CUS) they are a CU-masked pair that never share a
+// CU, so the panel leaf's workgroups (which must all be resident at once)
+// never wait for trailing-update workgroups to drain; *gcap is then the most
+// workgroups a panel-leaf launch may use (0: no cap beyond the device's).
+// Without look-ahead both are `main`.  lu_streams_begin makes *sT and *sP
+// wait for work already queued on `main`; lu_streams_end makes `main` wait
+// for both.
+int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, int *gcap);
+int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
@@ -114,6 +134,8 @@ struct leaf_ws {
   int *err = nullptr;                   // timeout flag
 };
 leaf_ws &leaf_workspace(size_t slot_bytes);
+// First of `count` fresh exchange tags for a leaf launch on stream s.
+int leaf_tags(hipStream_t s, unsigned count, unsigned *tag0);
 // Returns PLA_ERR_DEVICE if any panel exchange timed out since the last call.
 int leaf_check_error(hipStream_t s);
 // The device error word shared by the bounded waits of every kernel.
@@ -145,8 +167,21 @@ struct lu_leaf_call {
   int pw, pc0;     // panel width, leaf's first column inside the panel
   int *piv;        // w pivot offsets (relative to each row)
   int *pairs;      // leaf_pairs_stride() ints: count, (dst, src) row pairs from the panel top
+  int gcap;        // most workgroups the launch may use (0: the device's limit)
+  int W;           // register leaf: compile-time width (8, 16 or 32), c.w <= W
 };
+// Panel-LU leaf of the kind ctx().leaf_kind selects.
 int k_lu_leaf(hipStream_t s, const lu_leaf_call &c);
+// The register leaf (pla_leaf.hip): one wave per workgroup, no LDS.
+int k_lu_rleaf(hipStream_t s, const lu_leaf_call &c, int W);
+// Widest register leaf (32, 16 or 8) for a panel of mp rows within gcap waves
+// (0: no cap beyond the kernel's); 0 if none fits.
+int rleaf_width_for(int mp, int gcap);
+int rleaf_pairs_stride();
+// Widest leaf (32, 16 or 8 columns) whose rows fit in LDS when a panel of mp
+// rows is spread over at most gcap workgroups; 0 if none fits.
+int leaf_width_for(int mp, int gcap);
+constexpr int LU_MAX_LEAVES = 64;  // leaves of one panel (nb <= 512, width >= 8)
 // Compose the leaves' (dst, src) row lists of one panel into one list (comp,
 // panel_comp_stride() ints) and apply it to the rows of a panel (A = its first
 // row) in columns [0, nleft) and [c1, c2).
@@ -196,7 +231,7 @@ int trsm_rltn(hipStream_t s, int nrows, int b, const double *L, long ldl, double
 // The leaves apply their interchanges to all w columns of the panel and
 // append their (dst, src) row pairs to `pairs`; returns the number of leaves
 // in *nleaves.
-int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *pairs, int *nleaves);
+int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *pairs, int *nleaves, int gcap);
 // Recursive lower Cholesky of a b x b block; failures go to *info (1-based,
 // offset added).
 int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info);

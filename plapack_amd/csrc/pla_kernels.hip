@@ -308,7 +308,11 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
   const int r0 = g * R;
   const int rows = max(0, min(a.mp - r0, R));
   // a timed-out exchange earlier on this stream: leave the panel untouched
-  if (ld_dev(a.err)) return;
+  // and record no interchanges
+  if (ld_dev(a.err)) {
+    if (g == 0 && tid == 0) a.pairs[0] = 0;
+    return;
+  }
 
   for (int e = tid; e < rows * w; e += LEAF_THREADS) {
     const int c = e / rows, r = e - c * rows;
@@ -494,7 +498,12 @@ constexpr int HASH_SZ = 4096;
 __device__ inline int hslot(int key) { return (int)(((unsigned)key * 2654435761u) >> 20); }  // 12 bits
 
 __global__ __launch_bounds__(256) void panel_compose_kernel(const int *__restrict__ pairs, int nleaves,
-                                                            int *__restrict__ out) {
+                                                            int *__restrict__ out, const int *__restrict__ err) {
+  // after a timed-out exchange the leaves' lists are not trustworthy: no interchanges
+  if (ld_dev(err)) {
+    if (threadIdx.x == 0) out[0] = 0;
+    return;
+  }
   __shared__ int hk[HASH_SZ], hv[HASH_SZ];
   __shared__ int val[PAIRS_MAX];
   __shared__ int ncomp;
@@ -545,9 +554,10 @@ __global__ __launch_bounds__(256) void panel_compose_kernel(const int *__restric
 // column per wave: every lane loads all its sources before any lane stores.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long lda, int nleft, int c1, int c2,
-                                                         const int *__restrict__ comp) {
+                                                         const int *__restrict__ comp, const int *__restrict__ err) {
   __shared__ int sd[COMP_MAX], ss[COMP_MAX];
-  const int cnt = comp[0];
+  if (ld_dev(err)) return;  // a timed-out exchange: the list is not trustworthy
+  const int cnt = min(comp[0], COMP_MAX);
   for (int e = threadIdx.x; e < cnt; e += 256) {
     sd[e] = comp[1 + 2 * e];
     ss[e] = comp[2 + 2 * e];
@@ -991,27 +1001,46 @@ int k_fill_drand48(hipStream_t s, int m, int n, double *D, long ldd, uint64_t st
 
 // Panel leaf: chooses the grid (about 256 rows per workgroup, at most
 // LEAF_MAXG workgroups, all co-resident) and draws fresh exchange tags.
-static unsigned g_leaf_tag = 1;
 
-int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
-  if (c.mp <= 0 || c.w <= 0) return PLA_SUCCESS;
-  if (c.w > LEAF_W || c.mp < c.w) return fail(PLA_ERR_INVALID_PARAM, "lu leaf: %d x %d", c.mp, c.w);
+// Workgroups of a leaf over mp rows: about leaf_rows rows each, at most
+// LEAF_MAXG, the CU count and gcap.  Every workgroup must be resident at once
+// (they exchange pivot candidates every column): with more than 80 KiB of LDS
+// each, one fits per CU, so G must not exceed the CUs the stream may use.
+static int leaf_grid(int mp, int gcap) {
   int maxG = ctx().cu_count > 0 ? ctx().cu_count : 256;
   if (maxG > LEAF_MAXG) maxG = LEAF_MAXG;
+  if (gcap > 0 && maxG > gcap) maxG = gcap;
   const int target = ctx().leaf_rows > 0 ? ctx().leaf_rows : 256;
-  int G = (c.mp + target - 1) / target;
+  int G = (mp + target - 1) / target;
   if (G > maxG) G = maxG;
   if (G < 1) G = 1;
+  return G;
+}
+constexpr size_t LEAF_LDS_MAX = 150 * 1024;
+static size_t leaf_lds_bytes(int R, int w) { return (size_t)w * R * sizeof(double) + (size_t)R * sizeof(int); }
+
+int leaf_width_for(int mp, int gcap) {
+  const int G = leaf_grid(mp, gcap);
+  const int R = (mp + G - 1) / G;
+  for (int w = LEAF_W; w >= 8; w /= 2)
+    if (leaf_lds_bytes(R, w) <= LEAF_LDS_MAX) return w;
+  return 0;
+}
+
+int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
+  if (ctx().leaf_kind == 0) return k_lu_rleaf(s, c, c.W);
+  if (c.mp <= 0 || c.w <= 0) return PLA_SUCCESS;
+  if (c.w > LEAF_W || c.mp < c.w) return fail(PLA_ERR_INVALID_PARAM, "lu leaf: %d x %d", c.mp, c.w);
+  const int G = leaf_grid(c.mp, c.gcap);
   const int R = (c.mp + G - 1) / G;
-  const size_t lds_bytes = (size_t)c.w * R * sizeof(double) + (size_t)R * sizeof(int);
-  if (lds_bytes > 150 * 1024)
+  const size_t lds_bytes = leaf_lds_bytes(R, c.w);
+  if (lds_bytes > LEAF_LDS_MAX)
     return fail(PLA_ERR_DIMENSIONS, "lu leaf: panel of %d rows exceeds the LDS budget", c.mp);
   leaf_ws &ws = leaf_workspace((size_t)2 * LEAF_MAXG * SLOT_WORDS * sizeof(unsigned long long));
   if (!ws.base) return PLA_ERR_NO_MEMORY;
-  if (g_leaf_tag > 0xF0000000u) {  // tags only grow; restart far below wrap-around
-    PLA_HIP(hipMemsetAsync(ws.slots, 0, (size_t)2 * LEAF_MAXG * SLOT_WORDS * sizeof(unsigned long long), s));
-    g_leaf_tag = 1;
-  }
+  unsigned tag0;
+  int rc = leaf_tags(s, (unsigned)c.w, &tag0);
+  if (rc) return rc;
   leaf_args a;
   a.P = c.P;
   a.Ppanel = c.Ppanel;
@@ -1025,15 +1054,14 @@ int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
   a.pairs = c.pairs;
   a.slots = ws.slots;
   a.err = ws.err;
-  a.tag0 = g_leaf_tag;
-  a.spin_limit = 400000u;
+  a.tag0 = tag0;
+  a.spin_limit = ctx().leaf_spin_limit;
   static bool lds_attr = false;  // allow more than 64 KiB of dynamic LDS (gfx950 has 160 KiB)
   if (!lds_attr) {
     PLA_HIP(hipFuncSetAttribute((const void *)lu_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 150 * 1024));
     lds_attr = true;
   }
-  g_leaf_tag += (unsigned)c.w;
   hipLaunchKernelGGL(lu_leaf_kernel, dim3(G), dim3(LEAF_THREADS), lds_bytes, s, a);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
@@ -1041,8 +1069,10 @@ int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
 
 int k_panel_compose(hipStream_t s, const int *pairs, int nleaves, int *comp) {
   if (nleaves <= 0) return PLA_SUCCESS;
-  if (nleaves > 16) return fail(PLA_ERR_INVALID_PARAM, "panel compose: %d leaves", nleaves);
-  hipLaunchKernelGGL(panel_compose_kernel, dim3(1), dim3(256), 0, s, pairs, nleaves, comp);
+  if (nleaves > LU_MAX_LEAVES) return fail(PLA_ERR_INVALID_PARAM, "panel compose: %d leaves", nleaves);
+  int *err = leaf_error_word();
+  if (!err) return PLA_ERR_NO_MEMORY;
+  hipLaunchKernelGGL(panel_compose_kernel, dim3(1), dim3(256), 0, s, pairs, nleaves, comp, (const int *)err);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
@@ -1054,8 +1084,10 @@ static int panel_swap_launch(hipStream_t s, T *A, long lda, int nleft, int c1, i
   int blocks = (total + 3) / 4;
   const int cap = 16 * (ctx().cu_count > 0 ? ctx().cu_count : 256);
   if (blocks > cap) blocks = cap;
+  int *err = leaf_error_word();
+  if (!err) return PLA_ERR_NO_MEMORY;
   hipLaunchKernelGGL(panel_swap_kernel<T>, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1, c2 > c1 ? c2 : c1,
-                     comp);
+                     comp, (const int *)err);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
@@ -1070,7 +1102,7 @@ int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c
 
 int panel_comp_stride() { return 1 + 2 * COMP_MAX; }
 
-int leaf_pairs_stride() { return PAIRS_STRIDE; }
+int leaf_pairs_stride() { return PAIRS_STRIDE > rleaf_pairs_stride() ? PAIRS_STRIDE : rleaf_pairs_stride(); }
 int leaf_max_width() { return LEAF_W; }
 int trsm_leaf_size() { return TRSM_LEAF; }
 

@@ -1,0 +1,408 @@
+// pla_leaf.hip -- the unblocked panel-LU leaf with its rows in registers.
+//
+// Replaces PLA_LU_right_mv (SOLVERS/LU/PLA_LU_mv.c:52-82) for a leaf of up to
+// W columns of the panel: per column PLA_Iamax (BLAS1/PLA_Iamax.c:37-124)
+// becomes a wave reduction plus an exchange of (value, position, row)
+// records between workgroups, Pivot_rows_mv (PLA_Apply_pivots_new.c:208-305)
+// a relabelling of logical row positions, PLA_Local_inv_scal
+// (Local_BLAS/PLA_Local_BLAS1.c:258) one reciprocal and a multiply, and
+// PLA_Local_ger (Local_BLAS/PLA_Local_BLAS2.c:578-656) a rank-1 update.
+//
+// Layout.  A workgroup is ONE wave of 64 lanes and uses no LDS; lane l keeps
+// rows r0 + l + 64 i (i < RPL) of the leaf, all W columns, in registers
+// (RPL * W = 64 doubles per lane).  Without LDS and with about 200 VGPRs a
+// leaf wave fits on a CU beside two workgroups of the trailing-update GEMM
+// (FP64: 2 x 68.6 KB of LDS, 2 x 4 waves of 128 VGPRs; FP16/BF16: 2 x 80 KB,
+// 88 VGPRs), so the leaf never waits for trailing-update workgroups to drain,
+// and the waves of one leaf (all of which must be resident at once: they
+// exchange pivot candidates every column) are placed as soon as it launches.
+//
+// Per column j (rows are never moved; each carries its logical position,
+// initially its row number; positions < j hold the pivot rows chosen so far):
+//   1. candidate: the largest |L(r, j)| over rows at positions >= j, ties to
+//      the smallest position (idamax's first maximum, and the tie rule of
+//      PLA_Iamax.c:108-115), reduced over the wave;
+//   2. publish one record: (position, value) and the candidate row;
+//   3. read every workgroup's (position, value), pick the same winner, then
+//      read the winner's row from its record (already visible: it was stored
+//      with the header);
+//   4. relabel (pivot row -> j, the row at j -> p), scale by the reciprocal
+//      of the pivot, rank-1 update; then the candidate of column j+1 is
+//      published (its row must carry this step's update in every column).
+// Exchange words carry the column's 32-bit tag beside 32 bits of payload;
+// readers re-read until the tags match (no fences, no flags).  Records are
+// double buffered by column parity.  A bounded wait sets *err; every wave
+// still runs to the end, and kernels that consume the leaf's interchanges
+// skip them once *err is set.
+//
+// At the end rows are written to their final positions, the interchanges are
+// replayed into at most 2w (destination, source) row pairs, and the pairs are
+// applied to the panel's other columns, so the recursive panel needs no
+// separate row-interchange launches.
+#include "pla_internal.hpp"
+
+#include <climits>
+#include <cstdint>
+#include <utility>
+
+namespace {
+
+constexpr int RL_MAXG = 512;     // workgroups (waves) of one leaf
+constexpr int RL_REC = 72;       // words per record: 4 header words, 2 W row words, padding
+constexpr int RL_MAXW = 32;
+constexpr int RL_PAIRS = 2 * RL_MAXW;
+constexpr int RL_PAIRS_STRIDE = 1 + 2 * RL_PAIRS;  // count, then (dst, src) pairs
+
+struct rleaf_args {
+  double *P;                // leaf top-left (its row 0 is the leaf's first diagonal row)
+  double *Ppanel;           // panel column 0, at the leaf's first row
+  long lda;
+  int mp, w;                // rows from the leaf's first row to the panel bottom, leaf width (<= W)
+  int pw, pc0;              // panel width, leaf's first column inside the panel
+  int *piv;                 // w entries: pivot offset relative to row j
+  int *pairs;               // RL_PAIRS_STRIDE ints, positions relative to the leaf's first row + pc0
+  unsigned long long *slots;  // [2][RL_MAXG][RL_REC]
+  int *err;
+  unsigned tag0, spin_limit;
+};
+
+__device__ inline void st_dev(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long ld_dev(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_dev(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline double ld_dev(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned lo32(double v) { return (unsigned)(unsigned long long)__double_as_longlong(v); }
+__device__ inline unsigned hi32(double v) { return (unsigned)((unsigned long long)__double_as_longlong(v) >> 32); }
+__device__ inline double dbl_from(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>): loop bodies
+// with compile-time indices, so register arrays are never indexed at run time.
+template <typename F, int... Js>
+__device__ inline void static_for_impl(F &&f, std::integer_sequence<int, Js...>) {
+  (f(std::integral_constant<int, Js>{}), ...);
+}
+template <int N, typename F>
+__device__ inline void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// (|v|, position) order of the pivot search: larger value, then smaller position.
+__device__ inline bool better(double v, int p, double bv, int bp) { return v > bv || (v == bv && p < bp); }
+
+// Wave-wide argmax of (value, position) carrying a third int; valid in every lane.
+__device__ inline void wave_argmax(double &bv, int &bp, int &bx) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(bv, off, 64);
+    const int op = __shfl_xor(bp, off, 64);
+    const int ox = __shfl_xor(bx, off, 64);
+    if (better(ov, op, bv, bp)) {
+      bv = ov;
+      bp = op;
+      bx = ox;
+    }
+  }
+}
+
+// Bounded wait: the decision is wave-uniform.  Returns true when abandoned.
+__device__ inline bool spin_fail(unsigned &it, const rleaf_args &a) {
+  ++it;
+  const int bail = (it > a.spin_limit) || ((threadIdx.x & 63) == 0 && (it & 63) == 0 && ld_dev(a.err));
+  if (__ballot(bail)) {
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  __builtin_amdgcn_s_sleep(1);
+  return false;
+}
+
+template <int W>
+__global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
+  constexpr int RPL = 64 / W;  // rows per lane
+  constexpr int R = 64 * RPL;  // rows per workgroup
+  const int G = gridDim.x, g = blockIdx.x, lane = threadIdx.x;
+  const int w = a.w, mp = a.mp;
+  const long lda = a.lda;
+  if (ld_dev(a.err)) {  // a timed-out exchange earlier on this stream: no interchanges
+    if (g == 0 && lane == 0) a.pairs[0] = 0;
+    return;
+  }
+
+  double v[RPL][W];
+  int pos[RPL];
+#pragma unroll
+  for (int i = 0; i < RPL; i++) {
+    const int r = g * R + lane + 64 * i;
+    pos[i] = r < mp ? r : -1;
+  }
+#pragma unroll
+  for (int c = 0; c < W; c++)
+#pragma unroll
+    for (int i = 0; i < RPL; i++)
+      v[i][c] = (c < w && pos[i] >= 0) ? a.P[(long)c * lda + pos[i]] : 0.0;
+
+  unsigned long long *slots = a.slots;
+
+  // candidate of column j over this lane's rows, reduced over the wave,
+  // then published with the candidate row
+  auto publish = [&](int j) {
+    double bv = -1.0;
+    int bp = INT_MAX, bx = -1;
+#pragma unroll
+    for (int i = 0; i < RPL; i++) {
+      const double x = fabs(v[i][j]);
+      if (pos[i] >= j && better(x, pos[i], bv, bp)) {
+        bv = x;
+        bp = pos[i];
+        bx = i;
+      }
+    }
+    int who = bx >= 0 ? lane * RPL + bx : -1;  // lane and row slot of the candidate
+    wave_argmax(bv, bp, who);
+    const unsigned long long tag = (unsigned long long)(a.tag0 + (unsigned)j) << 32;
+    unsigned long long *rec = slots + ((size_t)(j & 1) * RL_MAXG + g) * RL_REC;
+    // the candidate row, gathered from its lane: word 4 + 2c (+1) holds lo (hi) of column c
+    unsigned wlo = 0u, whi = 0u;  // this lane's words `lane` and `lane + 64`
+    const int src_lane = who >= 0 ? who / RPL : 0, src_i = who >= 0 ? who % RPL : 0;
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+      double x = v[0][c];
+#pragma unroll
+      for (int i = 1; i < RPL; i++)
+        if (src_i == i) x = v[i][c];
+      x = __shfl(x, src_lane, 64);
+      if (who < 0) x = 0.0;
+      if (lane == 4 + 2 * c) wlo = lo32(x);
+      if (lane == 5 + 2 * c) wlo = hi32(x);
+      if (4 + 2 * c >= 64 && lane == 4 + 2 * c - 64) whi = lo32(x);
+      if (5 + 2 * c >= 64 && lane == 5 + 2 * c - 64) whi = hi32(x);
+    }
+    if (lane == 0) wlo = (unsigned)bp;
+    if (lane == 1) wlo = lo32(bv);
+    if (lane == 2) wlo = hi32(bv);
+    if (lane == 3) wlo = 0u;
+    if (lane < 4 + 2 * W) st_dev(rec + lane, tag | wlo);
+    if (4 + 2 * W > 64 && lane < 4 + 2 * W - 64) st_dev(rec + 64 + lane, tag | whi);
+  };
+
+  publish(0);
+  int ph = 0;  // lane j: the pivot position chosen for column j
+  static_for<W>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if (j < w) {
+      const unsigned tagv = a.tag0 + (unsigned)j;
+      const unsigned long long *recs = slots + (size_t)(j & 1) * RL_MAXG * RL_REC;
+      // every workgroup's (position, value)
+      double gv = -1.0;
+      int gp = INT_MAX, gw = -1;
+      {
+        unsigned it = 0;
+        for (;;) {
+          int ok = 1;
+          gv = -1.0;
+          gp = INT_MAX;
+          gw = -1;
+          for (int t = lane; t < G; t += 64) {
+            const unsigned long long *rt = recs + (size_t)t * RL_REC;
+            const unsigned long long w0 = ld_dev(rt), w1 = ld_dev(rt + 1), w2 = ld_dev(rt + 2);
+            if ((unsigned)(w0 >> 32) != tagv || (unsigned)(w1 >> 32) != tagv || (unsigned)(w2 >> 32) != tagv) ok = 0;
+            const int ci = (int)(unsigned)w0;
+            const double cv = dbl_from((unsigned)w1, (unsigned)w2);
+            if (ci >= j && ci < mp && better(cv, ci, gv, gp)) {
+              gv = cv;
+              gp = ci;
+              gw = t;
+            }
+          }
+          if (__all(ok)) break;
+          if (spin_fail(it, a)) break;
+        }
+        wave_argmax(gv, gp, gw);
+      }
+      // the winner's row (all-NaN column: no winner, no interchange); lane l < 2W
+      // holds record word 4 + l, so column c is in lanes 2c (low) and 2c + 1 (high)
+      unsigned rw = 0u;
+      {
+        const unsigned long long *rt = recs + (size_t)(gw >= 0 ? gw : 0) * RL_REC;
+        unsigned it = 0;
+        for (;;) {
+          int ok = 1;
+          if (gw >= 0 && lane < 2 * W) {
+            const unsigned long long x = ld_dev(rt + 4 + lane);
+            if ((unsigned)(x >> 32) != tagv) ok = 0;
+            rw = (unsigned)x;
+          }
+          if (__all(ok)) break;
+          if (spin_fail(it, a)) break;
+        }
+      }
+      // wave-uniform element c of the pivot row
+      auto prow = [&](int c) -> double {
+        if (gw < 0) return __longlong_as_double(0x7FF8000000000000LL);
+        return dbl_from((unsigned)__builtin_amdgcn_readlane((int)rw, 2 * c),
+                        (unsigned)__builtin_amdgcn_readlane((int)rw, 2 * c + 1));
+      };
+      const int p = gw >= 0 ? gp : j;
+      if (lane == j) ph = p;
+      if (g == 0 && lane == 0) a.piv[j] = p - j;
+      const double inv = 1.0 / prow(j);
+#pragma unroll
+      for (int i = 0; i < RPL; i++) {
+        int q = pos[i];
+        if (q == p) q = j;
+        else if (q == j) q = p;
+        pos[i] = q;
+        if (q > j) v[i][j] *= inv;
+      }
+      // rank-1 update, then the next column's candidate (its row must carry
+      // this step's update in every column)
+#pragma unroll
+      for (int c = 0; c < W; c++) {
+        if (c > j && c < w) {
+          const double pc = prow(c);
+#pragma unroll
+          for (int i = 0; i < RPL; i++)
+            if (pos[i] > j) v[i][c] = fma(v[i][j], -pc, v[i][c]);
+        }
+      }
+      if constexpr (j + 1 < W) {
+        if (j + 1 < w) publish(j + 1);
+      }
+    }
+  });
+
+  // rows to their final positions (a permutation: no two rows collide)
+#pragma unroll
+  for (int c = 0; c < W; c++)
+#pragma unroll
+    for (int i = 0; i < RPL; i++)
+      if (c < w && pos[i] >= 0) a.P[(long)c * lda + pos[i]] = v[i][c];
+
+  // replay the interchanges on (position -> source row) entries held in lanes
+  int key = -1, val = -1, cnt = 0;
+  for (int j = 0; j < w; j++) {
+    {
+      const int p = __shfl(ph, j, 64);
+      if (p != j) {
+        unsigned long long mj = __ballot(key == j);
+        if (!mj) {
+          if (lane == cnt) { key = j; val = j; }
+          mj = 1ULL << cnt;
+          cnt++;
+        }
+        unsigned long long mq = __ballot(key == p);
+        if (!mq) {
+          if (lane == cnt) { key = p; val = p; }
+          mq = 1ULL << cnt;
+          cnt++;
+        }
+        const int lj = __ffsll((long long)mj) - 1, lq = __ffsll((long long)mq) - 1;
+        const int vj = __shfl(val, lj, 64), vq = __shfl(val, lq, 64);
+        if (lane == lj) val = vq;
+        if (lane == lq) val = vj;
+      }
+    }
+  }
+  const bool moved = (key >= 0 && key != val);
+  const unsigned long long mm = __ballot(moved);
+  const int npairs = __popcll(mm);
+  // compact the moved entries: lane k ends up with the k-th (dst, src) pair
+  int dst = -1, src = -1;
+  {
+    unsigned long long rest = mm;
+    for (int k = 0; k < npairs; k++) {
+      const int from = __ffsll((long long)rest) - 1;
+      rest &= rest - 1;
+      const int kd = __shfl(key, from, 64), ks = __shfl(val, from, 64);
+      if (lane == k) {
+        dst = kd;
+        src = ks;
+      }
+    }
+  }
+  if (g == 0) {
+    if (lane < npairs) {
+      a.pairs[1 + 2 * lane] = dst + a.pc0;
+      a.pairs[2 + 2 * lane] = src + a.pc0;
+    }
+    if (lane == 0) a.pairs[0] = npairs;
+  }
+
+  // the same interchanges on the panel's other columns, one column per workgroup
+  const int others = a.pw - w;
+  if (npairs > 0) {
+    for (int q = g; q < others; q += G) {
+      const int cc = q < a.pc0 ? q : q + w;
+      double *col = a.Ppanel + (long)cc * lda;
+      if (lane < npairs) {
+        const double x = ld_dev(col + src);
+        col[dst] = x;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+namespace pla {
+
+int rleaf_pairs_stride() { return RL_PAIRS_STRIDE; }
+
+// Rows per workgroup of the register leaf of width W (64 doubles per lane).
+static int rleaf_rows(int W) { return 64 * (64 / W); }
+
+int rleaf_width_for(int mp, int gcap) {
+  int maxg = RL_MAXG;
+  if (gcap > 0 && gcap < maxg) maxg = gcap;
+  // widest leaf within 256 waves, else the widest within the cap
+  for (int lim : {256, maxg})
+    for (int W = RL_MAXW; W >= 8; W /= 2)
+      if ((mp + rleaf_rows(W) - 1) / rleaf_rows(W) <= (lim < maxg ? lim : maxg)) return W;
+  return 0;
+}
+
+int k_lu_rleaf(hipStream_t s, const lu_leaf_call &c, int W) {
+  if (c.mp <= 0 || c.w <= 0) return PLA_SUCCESS;
+  if ((W != 8 && W != 16 && W != 32) || c.w > W || c.mp < c.w)
+    return fail(PLA_ERR_INVALID_PARAM, "lu leaf: %d x %d in a width-%d leaf", c.mp, c.w, W);
+  const int R = rleaf_rows(W);
+  const int G = (c.mp + R - 1) / R;
+  if (G > RL_MAXG || (c.gcap > 0 && G > c.gcap))
+    return fail(PLA_ERR_DIMENSIONS, "lu leaf: %d rows need %d waves of width %d", c.mp, G, W);
+  const size_t slot_bytes = (size_t)2 * RL_MAXG * RL_REC * sizeof(unsigned long long);
+  leaf_ws &ws = leaf_workspace(slot_bytes);
+  if (!ws.base) return PLA_ERR_NO_MEMORY;
+  unsigned tag0;
+  int rc = leaf_tags(s, (unsigned)c.w, &tag0);
+  if (rc) return rc;
+  rleaf_args a;
+  a.P = c.P;
+  a.Ppanel = c.Ppanel;
+  a.lda = c.lda;
+  a.mp = c.mp;
+  a.w = c.w;
+  a.pw = c.pw;
+  a.pc0 = c.pc0;
+  a.piv = c.piv;
+  a.pairs = c.pairs;
+  a.slots = ws.slots;
+  a.err = ws.err;
+  a.tag0 = tag0;
+  a.spin_limit = ctx().leaf_spin_limit;
+  switch (W) {
+    case 8: hipLaunchKernelGGL(lu_rleaf_kernel<8>, dim3(G), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL(lu_rleaf_kernel<16>, dim3(G), dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL(lu_rleaf_kernel<32>, dim3(G), dim3(64), 0, s, a); break;
+  }
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+}  // namespace pla

@@ -33,6 +33,10 @@ static int check_lu_args(PLA_Obj A, PLA_Obj pivots) {
 
 namespace pla {
 
+static int lu_factor_steps(hipStream_t sP, hipStream_t sM, int gcap, int m, int n, double *a, long lda, int *piv,
+                           int nb, int *pairs, size_t pstride, int *comp, int cstride, hipEvent_t ev_ready,
+                           hipEvent_t ev_panel);
+
 // Blocked right-looking LU of the m x n column-major device matrix a, with
 // one step of look-ahead: panel k+1 is factored on the look-ahead stream as
 // soon as its own columns have received step k's update, while the main
@@ -42,14 +46,13 @@ namespace pla {
 //                       -> A12 <- inv(L11) A12 -> A22[:, next panel] -= A21 A12
 //                       -> signal -> A22[:, rest] -= A21 A12
 // look-ahead stream:    wait(signal) -> factor panel k+1 -> signal(panel k+1)
-int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int nb) {
-  hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
-  const int mn = m < n ? m : n;
+int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, int nb) {
   if (nb > 512) nb = 512;
   if (nb < 1) nb = 1;
   const int stride = leaf_pairs_stride();
   const int cstride = panel_comp_stride();
-  int *pairs = (int *)workspace(sizeof(int) * 2 * 16 * stride, 3);
+  const size_t pstride = (size_t)LU_MAX_LEAVES * stride;  // one parity of the leaves' pair lists
+  int *pairs = (int *)workspace(sizeof(int) * 2 * pstride, 3, true);
   int *comp = (int *)workspace(sizeof(int) * 2 * cstride, 4);
   if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
   static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
@@ -58,13 +61,28 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
   }
   int rc;
+  // sP: panels; sM: interchanges, A12 solves and trailing updates
+  hipStream_t sP, sM;
+  int gcap;
+  if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
+  rc = lu_factor_steps(sP, sM, gcap, m, n, a, lda, piv, nb, pairs, pstride, comp, cstride, ev_ready, ev_panel);
+  // the main stream waits for both streams, also after an error
+  const int rc_end = lu_streams_end(sMain, sP, sM);
+  if (rc) return rc;
+  if (rc_end) return rc_end;
+  return leaf_check_error(sMain);
+}
+
+static int lu_factor_steps(hipStream_t sP, hipStream_t sM, int gcap, int m, int n, double *a, long lda, int *piv,
+                           int nb, int *pairs, size_t pstride, int *comp, int cstride, hipEvent_t ev_ready,
+                           hipEvent_t ev_panel) {
+  const int mn = m < n ? m : n;
+  int rc;
   int nleaves[2] = {0, 0};
   // panel 0, after everything already queued on the main stream
   {
     const int b = mn < nb ? mn : nb;
-    PLA_HIP(hipEventRecord(ev_ready, sM));
-    PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
-    if ((rc = lu_panel(sP, a, m, b, lda, piv, pairs, &nleaves[0]))) return rc;
+    if ((rc = lu_panel(sP, a, m, b, lda, piv, pairs, &nleaves[0], gcap))) return rc;
     if ((rc = k_panel_compose(sP, pairs, nleaves[0], comp))) return rc;
     PLA_HIP(hipEventRecord(ev_panel, sP));
   }
@@ -91,10 +109,10 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
       PLA_HIP(hipEventRecord(ev_ready, sM));
       PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
       double *An = a + (long)kn * lda + kn;
-      if ((rc = lu_panel(sP, An, m - kn, bn, lda, piv + kn, pairs + (size_t)(par ^ 1) * 16 * stride,
-                         &nleaves[par ^ 1])))
+      if ((rc = lu_panel(sP, An, m - kn, bn, lda, piv + kn, pairs + (par ^ 1) * pstride, &nleaves[par ^ 1],
+                         gcap)))
         return rc;
-      if ((rc = k_panel_compose(sP, pairs + (size_t)(par ^ 1) * 16 * stride, nleaves[par ^ 1],
+      if ((rc = k_panel_compose(sP, pairs + (par ^ 1) * pstride, nleaves[par ^ 1],
                                 comp + (size_t)(par ^ 1) * cstride)))
         return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
@@ -107,7 +125,7 @@ int lu_factor(hipStream_t sM, int m, int n, double *a, long lda, int *piv, int n
       if (rc) return rc;
     }
   }
-  return leaf_check_error(sM);
+  return PLA_SUCCESS;
 }
 
 }  // namespace pla

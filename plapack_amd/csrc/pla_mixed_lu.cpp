@@ -20,8 +20,7 @@ enum { WS_W = 10, WS_P64, WS_U64, WS_LH, WS_LL, WS_UH, WS_UL, WS_PERM, WS_DINVL,
 // Blocked right-looking LU of the n x n FP32 matrix W (ldw) with one step
 // of look-ahead, as lu_factor (pla_lu.cpp) but with FP64 panels and
 // triangular solves and a low-precision trailing update.
-int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *piv, int nb) {
-  hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
+int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int *piv, int nb) {
   nb = (nb / 32) * 32;
   if (nb < 32) nb = 32;
   if (nb > 512) nb = 512;
@@ -35,7 +34,8 @@ int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *pi
   char *Ll = split ? (char *)workspace((size_t)2 * n * nbp * esz, WS_LL) : nullptr;
   char *Uh = (char *)workspace((size_t)n * nbp * esz, WS_UH);
   char *Ul = split ? (char *)workspace((size_t)n * nbp * esz, WS_UL) : nullptr;
-  int *pairs = (int *)workspace(sizeof(int) * 2 * 16 * stride, WS_PAIRS);
+  const size_t pstride = (size_t)LU_MAX_LEAVES * stride;  // one parity of the leaves' pair lists
+  int *pairs = (int *)workspace(sizeof(int) * 2 * pstride, WS_PAIRS, true);
   const int cstride = panel_comp_stride();
   int *comp = (int *)workspace(sizeof(int) * 2 * cstride, WS_COMP);
   if (!P64 || !U64 || !Lh || !Uh || !pairs || !comp || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
@@ -47,63 +47,73 @@ int lu_factor_mixed(hipStream_t sM, int prec, int n, float *W, long ldw, int *pi
   }
   int nleaves[2] = {0, 0};
   int rc;
-  // panel at column k, on the look-ahead stream
-  auto panel = [&](int k, int b, int par) -> int {
-    const int mp = n - k;
-    float *Wk = W + (long)k * ldw + k;
-    int r;
-    if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
-    if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + (size_t)par * 16 * stride, &nleaves[par]))) return r;
-    if ((r = k_panel_compose(sP, pairs + (size_t)par * 16 * stride, nleaves[par], comp + (size_t)par * cstride)))
-      return r;
-    if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
-    return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
-  };
-  {
-    const int b = n < nb ? n : nb;
-    PLA_HIP(hipEventRecord(ev_ready, sM));
-    PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
-    if ((rc = panel(0, b, 0))) return rc;
-    PLA_HIP(hipEventRecord(ev_panel, sP));
-  }
-  for (int k = 0; k < n; k += nb) {
-    const int b = n - k < nb ? n - k : nb;
-    const int par = (k / nb) & 1;
-    PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
-    if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, comp + (size_t)par * cstride))) return rc;
-    const int nr = n - k - b;
-    if (nr <= 0) continue;
-    float *W12 = W + (long)(k + b) * ldw + k;
-    // U12 = inv(L11) W12 in FP64 (L11: top of the FP64 panel, still in P64)
-    if ((rc = k_cvt_f32_f64(sM, b, nr, W12, ldw, U64, b))) return rc;
-    if ((rc = trsm_left(sM, true, false, true, b, nr, P64, n - k, U64, b, nullptr))) return rc;
-    if ((rc = k_upanel_out(sM, prec, b, nr, U64, b, W12, ldw, Uh, split ? Ul : nullptr, nbp))) return rc;
-    const int mr = nr;
-    const void *L_h = Lh + par * lbuf;
-    const void *L_l = split ? Ll + par * lbuf : nullptr;
-    const int kn = k + b;
-    const int bn = kn < n ? (n - kn < nb ? n - kn : nb) : 0;
-    int done = 0;
-    if (bn > 0) {
-      timing_begin(sM, 3);
-      rc = k_hgemm(sM, prec, mr, bn, nbp, L_h, L_l, nbp, Uh, split ? Ul : nullptr, nbp, W12 + b, ldw);
-      timing_end(sM, 3, 2.0 * mr * (double)bn * b);
-      if (rc) return rc;
-      done = bn;
-      PLA_HIP(hipEventRecord(ev_ready, sM));
-      PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
-      if ((rc = panel(kn, bn, par ^ 1))) return rc;
+  // sP: panels; sM: interchanges, U12 solves and trailing updates
+  hipStream_t sP, sM;
+  int gcap;
+  if ((rc = lu_streams_begin(sMain, n, &sP, &sM, &gcap))) return rc;
+  auto steps = [&]() -> int {
+    // panel at column k, on the look-ahead stream
+    auto panel = [&](int k, int b, int par) -> int {
+      const int mp = n - k;
+      float *Wk = W + (long)k * ldw + k;
+      int r;
+      if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
+      if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + par * pstride, &nleaves[par], gcap))) return r;
+      if ((r = k_panel_compose(sP, pairs + par * pstride, nleaves[par], comp + (size_t)par * cstride)))
+        return r;
+      if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
+      return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
+    };
+    {
+      const int b = n < nb ? n : nb;
+      if ((rc = panel(0, b, 0))) return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
-    if (nr - done > 0) {
-      timing_begin(sM, 3);
-      rc = k_hgemm(sM, prec, mr, nr - done, nbp, L_h, L_l, nbp, Uh + (size_t)done * nbp * esz,
-                   split ? Ul + (size_t)done * nbp * esz : nullptr, nbp, W12 + (long)done * ldw + b, ldw);
-      timing_end(sM, 3, 2.0 * mr * (double)(nr - done) * b);
-      if (rc) return rc;
+    for (int k = 0; k < n; k += nb) {
+      const int b = n - k < nb ? n - k : nb;
+      const int par = (k / nb) & 1;
+      PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
+      if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, comp + (size_t)par * cstride))) return rc;
+      const int nr = n - k - b;
+      if (nr <= 0) continue;
+      float *W12 = W + (long)(k + b) * ldw + k;
+      // U12 = inv(L11) W12 in FP64 (L11: top of the FP64 panel, still in P64)
+      if ((rc = k_cvt_f32_f64(sM, b, nr, W12, ldw, U64, b))) return rc;
+      if ((rc = trsm_left(sM, true, false, true, b, nr, P64, n - k, U64, b, nullptr))) return rc;
+      if ((rc = k_upanel_out(sM, prec, b, nr, U64, b, W12, ldw, Uh, split ? Ul : nullptr, nbp))) return rc;
+      const int mr = nr;
+      const void *L_h = Lh + par * lbuf;
+      const void *L_l = split ? Ll + par * lbuf : nullptr;
+      const int kn = k + b;
+      const int bn = kn < n ? (n - kn < nb ? n - kn : nb) : 0;
+      int done = 0;
+      if (bn > 0) {
+        timing_begin(sM, 3);
+        rc = k_hgemm(sM, prec, mr, bn, nbp, L_h, L_l, nbp, Uh, split ? Ul : nullptr, nbp, W12 + b, ldw);
+        timing_end(sM, 3, 2.0 * mr * (double)bn * b);
+        if (rc) return rc;
+        done = bn;
+        PLA_HIP(hipEventRecord(ev_ready, sM));
+        PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+        if ((rc = panel(kn, bn, par ^ 1))) return rc;
+        PLA_HIP(hipEventRecord(ev_panel, sP));
+      }
+      if (nr - done > 0) {
+        timing_begin(sM, 3);
+        rc = k_hgemm(sM, prec, mr, nr - done, nbp, L_h, L_l, nbp, Uh + (size_t)done * nbp * esz,
+                     split ? Ul + (size_t)done * nbp * esz : nullptr, nbp, W12 + (long)done * ldw + b, ldw);
+        timing_end(sM, 3, 2.0 * mr * (double)(nr - done) * b);
+        if (rc) return rc;
+      }
     }
-  }
-  return leaf_check_error(sM);
+    return PLA_SUCCESS;
+  };
+  rc = steps();
+  // the main stream waits for both streams, also after an error
+  const int rc_end = lu_streams_end(sMain, sP, sM);
+  if (rc) return rc;
+  if (rc_end) return rc_end;
+  return leaf_check_error(sMain);
 }
 
 }  // namespace pla

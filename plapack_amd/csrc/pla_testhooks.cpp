@@ -31,4 +31,11 @@ int pla_th_sym_inf_norm(int n, const double *A, long lda, double *out) {
   return sym_inf_norm(ctx().stream, n, A, lda, out);
 }
 
+// Poll limit of the panel-leaf exchange waits (0 restores the default): a
+// tiny limit makes the waits give up, to test the timed-out-exchange path.
+int pla_th_set_leaf_spin_limit(unsigned limit) {
+  ctx().leaf_spin_limit = limit ? limit : 400000u;
+  return PLA_SUCCESS;
+}
+
 }  // extern "C"

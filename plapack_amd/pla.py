@@ -50,6 +50,9 @@ PLA_MIXED_BF16X3 = 4
 PLA_MIXED_FP8 = 5
 PLA_PARAM_LOOKAHEAD = 1
 PLA_PARAM_LEAF_ROWS = 2
+PLA_PARAM_PANEL_CUS = 3
+PLA_PARAM_LEAF_WIDTH = 4
+PLA_PARAM_LEAF_KIND = 5
 PLA_BATCHED_FP64 = 1
 PLA_BATCHED_FP32_IR = 2
 PLA_BATCHED_AUTOTUNE = 0x100

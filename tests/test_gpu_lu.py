@@ -198,3 +198,61 @@ def test_attach_device_buffer(pla_session, ora):
     assert pla.lib().PLA_Obj_attach_buffer(Z, host.ctypes.data_as(ctypes.c_void_p), 10, 1) < 0
     assert pla.lib().PLA_Obj_attach_buffer(Z, ptr, 5, 1) < 0     # ldim < rows
     pla.free(X, Z)
+
+
+@pytest.mark.parametrize("kind,cus,width", [(0, 0, 0), (0, 0, 8), (0, 0, 16), (0, 0, 32), (0, 32, 0),
+                                            (1, 0, 0), (1, 32, 0), (1, 32, 8), (1, 32, 16), (1, 64, 32)])
+@pytest.mark.parametrize("m,n,nb", [(3000, 3000, 256), (6000, 700, 512), (20000, 100, 64)])
+def test_lu_leaf_kinds_widths_and_panel_cus(pla_session, ora, kind, cus, width, m, n, nb):
+    """Both panel leaves (rows in registers, rows in LDS), every leaf width and
+    reserved panel CUs (CU-masked panel / trailing streams) give the oracle's
+    pivots and factors."""
+    a = ora.uniform(m, n, seed=7)
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, kind)
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_PANEL_CUS, cus)
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_WIDTH, width)
+    try:
+        lu, piv = _lu_gpu(pla_session, a=a, nb=nb)
+    finally:
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, 0)
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_PANEL_CUS, 0)
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_WIDTH, 0)
+    lu_o, piv_o = ora.lu_blocked(a, nb=nb)
+    assert np.array_equal(piv, piv_o)
+    assert np.abs(lu - lu_o).max() <= 1e-7 * np.abs(a).max()
+
+
+def test_panel_param_errors(pla_session):
+    for cus in (-32, 8, 48, 100000):
+        with pytest.raises(pla.PLAError):
+            pla.call("PLA_Set_device_param", pla.PLA_PARAM_PANEL_CUS, cus)
+    for w in (-1, 4, 12, 64):
+        with pytest.raises(pla.PLAError):
+            pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_WIDTH, w)
+    for k in (-1, 2):
+        with pytest.raises(pla.PLAError):
+            pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, k)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_lu_exchange_timeout_reports_error(pla_session, ora, kind):
+    """A panel exchange that gives up (forced from the host with a one-poll
+    wait limit) makes PLA_LU return PLA_ERR_DEVICE; every later kernel of the
+    factorization skips the interchanges, and the next call is correct."""
+    hooks = ctypes.CDLL(os.path.join(os.path.dirname(pla.LIB_PATH), "libpla_testhooks.so"))
+    hooks.pla_th_set_leaf_spin_limit.argtypes = [ctypes.c_uint]
+    s = pla_session
+    n = 4096
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, kind)
+    hooks.pla_th_set_leaf_spin_limit(1)
+    try:
+        with pytest.raises(pla.PLAError) as e:
+            _lu_gpu(s, n=n, seed=0, nb=256)
+        assert e.value.code == -1006 and "timed out" in str(e.value)
+    finally:
+        hooks.pla_th_set_leaf_spin_limit(0)
+        pla.call("PLA_Set_device_nb", 0)
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, 0)
+    lu, piv = _lu_gpu(s, n=1000, seed=3, nb=128)
+    lu_o, piv_o = ora.lu_blocked(ora.uniform(1000, 1000, 3), nb=128)
+    assert np.array_equal(piv, piv_o)
