@@ -33,9 +33,9 @@ static int check_lu_args(PLA_Obj A, PLA_Obj pivots) {
 
 namespace pla {
 
-static int lu_factor_steps(hipStream_t sP, hipStream_t sM, int gcap, int m, int n, double *a, long lda, int *piv,
-                           int nb, int *pairs, size_t pstride, int *comp, int cstride, hipEvent_t ev_ready,
-                           hipEvent_t ev_panel);
+static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
+                           long lda, int *piv, int nb, int *pairs, size_t pstride, int *comp, int cstride,
+                           hipEvent_t ev_ready, hipEvent_t ev_panel, hipEvent_t ev_left);
 
 // Blocked right-looking LU of the m x n column-major device matrix a, with
 // one step of look-ahead: panel k+1 is factored on the look-ahead stream as
@@ -55,17 +55,20 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   int *pairs = (int *)workspace(sizeof(int) * 2 * pstride, 3, true);
   int *comp = (int *)workspace(sizeof(int) * 2 * cstride, 4);
   if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr, ev_left = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
   }
   int rc;
-  // sP: panels; sM: interchanges, A12 solves and trailing updates
+  // sP: panels; sM: interchanges right of the panel, A12 solves and trailing
+  // updates; the main stream: interchanges left of the panel
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
-  rc = lu_factor_steps(sP, sM, gcap, m, n, a, lda, piv, nb, pairs, pstride, comp, cstride, ev_ready, ev_panel);
+  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, pairs, pstride, comp, cstride, ev_ready, ev_panel,
+                       ev_left);
   // the main stream waits for both streams, also after an error
   const int rc_end = lu_streams_end(sMain, sP, sM);
   if (rc) return rc;
@@ -73,9 +76,14 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   return leaf_check_error(sMain);
 }
 
-static int lu_factor_steps(hipStream_t sP, hipStream_t sM, int gcap, int m, int n, double *a, long lda, int *piv,
-                           int nb, int *pairs, size_t pstride, int *comp, int cstride, hipEvent_t ev_ready,
-                           hipEvent_t ev_panel) {
+// The interchanges of panel k on the columns left of it (L, which nothing
+// reads until the factorization ends) go to sL when it differs from sM, off
+// the trailing update's stream; the compose of panel k+2 into the same
+// double-buffered list waits for them (ev_left).
+static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
+                           long lda, int *piv, int nb, int *pairs, size_t pstride, int *comp, int cstride,
+                           hipEvent_t ev_ready, hipEvent_t ev_panel, hipEvent_t ev_left) {
+  const bool split_left = (sL != sM);
   const int mn = m < n ? m : n;
   int rc;
   int nleaves[2] = {0, 0};
@@ -92,7 +100,15 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, int gcap, int m, int 
     double *Akk = a + (long)k * lda + k;
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
     const int nr = n - k - b, mr = m - k - b;
-    if ((rc = k_panel_swap(sM, a + k, lda, k, k + b, n, comp + (size_t)par * cstride))) return rc;
+    const int *cp = comp + (size_t)par * cstride;
+    if (split_left) {
+      PLA_HIP(hipStreamWaitEvent(sL, ev_panel, 0));
+      if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
+      PLA_HIP(hipEventRecord(ev_left, sL));
+      if ((rc = k_panel_swap(sM, a + k, lda, 0, k + b, n, cp))) return rc;
+    } else if ((rc = k_panel_swap(sM, a + k, lda, k, k + b, n, cp))) {
+      return rc;
+    }
     if (nr <= 0) continue;
     double *A12 = a + (long)(k + b) * lda + k;
     if ((rc = trsm_left(sM, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
@@ -112,6 +128,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, int gcap, int m, int 
       if ((rc = lu_panel(sP, An, m - kn, bn, lda, piv + kn, pairs + (par ^ 1) * pstride, &nleaves[par ^ 1],
                          gcap)))
         return rc;
+      if (split_left) PLA_HIP(hipStreamWaitEvent(sP, ev_left, 0));  // step k-1 read this list
       if ((rc = k_panel_compose(sP, pairs + (par ^ 1) * pstride, nleaves[par ^ 1],
                                 comp + (size_t)(par ^ 1) * cstride)))
         return rc;

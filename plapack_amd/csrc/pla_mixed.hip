@@ -33,6 +33,7 @@
 #include "pla_internal.hpp"
 
 #include <climits>
+#include <utility>
 #include <cstdint>
 #include <vector>
 
@@ -65,11 +66,15 @@ __device__ inline f4v mfma16(bhalf8 a, bhalf8 b, f4v c) {
 // with k contiguous: row i of A at Ah + i*lda, column j of B at Bh + j*ldb.
 // k is a multiple of 32; lda, ldb multiples of 8.  128 x 128 block tile, 4
 // waves of 64 x 64 as 4 x 4 MFMA 16x16x32 tiles, K staged 32 at a time
-// through double-buffered LDS (rows padded to 40 elements: 16-byte lane
-// reads are bank-conflict free).  MFMA 16x16x32 layout (lane l): A row l%16,
+// through double-buffered LDS.  An LDS row is 32 elements (64 bytes, four
+// 16-byte chunks); chunk c of row r sits at chunk c ^ ((r >> 1) & 3), which
+// makes the fragment reads (16 lanes: 16 rows, one chunk) and the staging
+// writes (4 lanes per row) free of bank conflicts -- measured on MI355X with
+// SQ_LDS_BANK_CONFLICT, where rows padded to 40 elements cost 4 conflict
+// cycles per read.  MFMA 16x16x32 layout (lane l): A row l%16,
 // k = 8*(l/16)..+7; B column l%16, same k; result lane l holds
 // D[4*(l/16) + r][l%16].
-constexpr int HBM = 128, HBN = 128, HKS = 32, HPAD = 8;
+constexpr int HBM = 128, HBN = 128, HKS = 32;
 
 template <typename H, bool SPLIT>
 __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, const H *__restrict__ Ah,
@@ -79,8 +84,8 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
                                                        int tiles_n, int vec_ok) {
   typedef typename hvec<H>::type V;
   constexpr int NS = SPLIT ? 2 : 1;
-  __shared__ H As[2][NS][HBM][HKS + HPAD];
-  __shared__ H Bs[2][NS][HBN][HKS + HPAD];
+  __shared__ V As[2][NS][HBM * 4];
+  __shared__ V Bs[2][NS][HBN * 4];
 
   const int ntiles = tiles_m * tiles_n;
   const int nblk = gridDim.x;  // a multiple of 8
@@ -103,6 +108,7 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
 #pragma unroll
     for (int bj = 0; bj < 4; bj++) acc[bi][bj] = f4v{0.f, 0.f, 0.f, 0.f};
 
+  // Global loads run one K step ahead of the MFMAs (staged through registers).
   V ra[NS][2], rb[NS][2];
   const V zero = V{};
   auto load_tile = [&](int kt) {
@@ -120,14 +126,17 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
       }
     }
   };
+  // the swizzle depends on the lane only: staging rows (tid >> 2) + 64 e,
+  // fragment rows 16 bi + l16 (+ 64 wm / wn)
+  const int wchunk = (tid >> 2) * 4 + ((tid & 3) ^ ((tid >> 3) & 3));
+  const int rchunk = l16 * 4 + (lq ^ ((l16 >> 1) & 3));
   auto store_tile = [&](int buf) {
 #pragma unroll
     for (int e = 0; e < 2; e++) {
-      const int c = tid + 256 * e, row = c >> 2, kc = c & 3;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        *(V *)&As[buf][s][row][kc * 8] = ra[s][e];
-        *(V *)&Bs[buf][s][row][kc * 8] = rb[s][e];
+        As[buf][s][wchunk + 256 * e] = ra[s][e];
+        Bs[buf][s][wchunk + 256 * e] = rb[s][e];
       }
     }
   };
@@ -145,9 +154,9 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
 #pragma unroll
     for (int s = 0; s < NS; s++) {
 #pragma unroll
-      for (int bi = 0; bi < 4; bi++) af[s][bi] = *(const V *)&As[cur][s][wm * 64 + bi * 16 + l16][lq * 8];
+      for (int bi = 0; bi < 4; bi++) af[s][bi] = As[cur][s][(wm * 64 + bi * 16) * 4 + rchunk];
 #pragma unroll
-      for (int bj = 0; bj < 4; bj++) bf[s][bj] = *(const V *)&Bs[cur][s][wn * 64 + bj * 16 + l16][lq * 8];
+      for (int bj = 0; bj < 4; bj++) bf[s][bj] = Bs[cur][s][(wn * 64 + bj * 16) * 4 + rchunk];
     }
 #pragma unroll
     for (int bi = 0; bi < 4; bi++)

@@ -40,17 +40,21 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   int *comp = (int *)workspace(sizeof(int) * 2 * cstride, WS_COMP);
   if (!P64 || !U64 || !Lh || !Uh || !pairs || !comp || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
   const size_t lbuf = (size_t)n * nbp * esz;  // one parity of the L copies
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr, ev_left = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
   }
   int nleaves[2] = {0, 0};
   int rc;
-  // sP: panels; sM: interchanges, U12 solves and trailing updates
+  // sP: panels; sM: interchanges right of the panel, U12 solves and trailing
+  // updates; sMain: interchanges left of the panel (nothing reads L until the
+  // factorization ends), ordered before the compose that reuses their list
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, n, &sP, &sM, &gcap))) return rc;
+  const bool split_left = (sMain != sM);
   auto steps = [&]() -> int {
     // panel at column k, on the look-ahead stream
     auto panel = [&](int k, int b, int par) -> int {
@@ -59,6 +63,7 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       int r;
       if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
       if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + par * pstride, &nleaves[par], gcap))) return r;
+      if (split_left && k > 0) PLA_HIP(hipStreamWaitEvent(sP, ev_left, 0));  // step k-2 read this list
       if ((r = k_panel_compose(sP, pairs + par * pstride, nleaves[par], comp + (size_t)par * cstride)))
         return r;
       if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
@@ -73,7 +78,15 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       const int b = n - k < nb ? n - k : nb;
       const int par = (k / nb) & 1;
       PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
-      if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, comp + (size_t)par * cstride))) return rc;
+      const int *cp = comp + (size_t)par * cstride;
+      if (split_left) {
+        PLA_HIP(hipStreamWaitEvent(sMain, ev_panel, 0));
+        if ((rc = k_panel_swap_f32(sMain, W + k, ldw, k, 0, 0, cp))) return rc;
+        PLA_HIP(hipEventRecord(ev_left, sMain));
+        if ((rc = k_panel_swap_f32(sM, W + k, ldw, 0, k + b, n, cp))) return rc;
+      } else if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, cp))) {
+        return rc;
+      }
       const int nr = n - k - b;
       if (nr <= 0) continue;
       float *W12 = W + (long)(k + b) * ldw + k;
