@@ -21,11 +21,8 @@
 // initially its row number; positions < j hold the pivot rows chosen so far):
 //   1. candidate: the largest |L(r, j)| over rows at positions >= j, ties to
 //      the smallest position (idamax's first maximum, and the tie rule of
-//      PLA_Iamax.c:108-115), reduced over the wave with DPP row exchanges and
-//      readlane (no LDS traffic: the trailing GEMM beside it keeps the LDS
-//      busy);
-//   2. publish one record: (position, value) and the candidate row, stored
-//      by the candidate's own lane (agent-scope stores, visible to every XCD);
+//      PLA_Iamax.c:108-115), reduced over the wave;
+//   2. publish one record: (position, value) and the candidate row;
 //   3. read every workgroup's (position, value), pick the same winner, then
 //      read the winner's row from its record (already visible: it was stored
 //      with the header);
@@ -99,49 +96,19 @@ __device__ inline void static_for(F &&f) {
 // (|v|, position) order of the pivot search: larger value, then smaller position.
 __device__ inline bool better(double v, int p, double bv, int bp) { return v > bv || (v == bv && p < bp); }
 
-// One DPP exchange step of the (value, position) argmax within rows of 16
-// lanes (VALU data movement: no LDS traffic).
-template <int CTRL>
-__device__ inline void dpp_argmax_step(double &bv, int &bp) {
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)lo32(bv), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)hi32(bv), CTRL, 0xF, 0xF, false);
-  const int op = __builtin_amdgcn_update_dpp(0, bp, CTRL, 0xF, 0xF, false);
-  const double ov = dbl_from((unsigned)lo, (unsigned)hi);
-  if (better(ov, op, bv, bp)) {
-    bv = ov;
-    bp = op;
-  }
-}
-
-// Wave-wide argmax of (value, position), result wave-uniform: rotations by 8
-// and 4 and two quad permutations reduce each row of 16 lanes, then the four
-// row results are read with readlane and compared.
-__device__ inline void wave_argmax(double &bv, int &bp) {
-  dpp_argmax_step<0x128>(bv, bp);  // row_ror:8
-  dpp_argmax_step<0x124>(bv, bp);  // row_ror:4
-  dpp_argmax_step<0x4E>(bv, bp);   // quad_perm [2,3,0,1]
-  dpp_argmax_step<0xB1>(bv, bp);   // quad_perm [1,0,3,2]
-  const unsigned lo = lo32(bv), hi = hi32(bv);
-  double rv = dbl_from((unsigned)__builtin_amdgcn_readlane((int)lo, 0), (unsigned)__builtin_amdgcn_readlane((int)hi, 0));
-  int rp = __builtin_amdgcn_readlane(bp, 0);
+// Wave-wide argmax of (value, position) carrying a third int; valid in every lane.
+__device__ inline void wave_argmax(double &bv, int &bp, int &bx) {
 #pragma unroll
-  for (int r = 16; r < 64; r += 16) {
-    const double v = dbl_from((unsigned)__builtin_amdgcn_readlane((int)lo, r),
-                              (unsigned)__builtin_amdgcn_readlane((int)hi, r));
-    const int p = __builtin_amdgcn_readlane(bp, r);
-    if (better(v, p, rv, rp)) {
-      rv = v;
-      rp = p;
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(bv, off, 64);
+    const int op = __shfl_xor(bp, off, 64);
+    const int ox = __shfl_xor(bx, off, 64);
+    if (better(ov, op, bv, bp)) {
+      bv = ov;
+      bp = op;
+      bx = ox;
     }
   }
-  bv = rv;
-  bp = rp;
-}
-
-// The lane whose `mine` equals the wave-uniform `key` (first such lane), or 0.
-__device__ inline int lane_of(int mine, int key) {
-  const unsigned long long m = __ballot(mine == key);
-  return m ? __ffsll((long long)m) - 1 : 0;
 }
 
 // Bounded wait: the decision is wave-uniform.  Returns true when abandoned.
@@ -197,32 +164,32 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
         bx = i;
       }
     }
-    const int mypos = bp;
-    wave_argmax(bv, bp);  // now wave-uniform; positions are unique, so bp names the lane
-    const int who = lane_of(mypos, bp);
-    const int slot = __builtin_amdgcn_readlane(bx, who);  // wave-uniform row slot of the candidate
+    int who = bx >= 0 ? lane * RPL + bx : -1;  // lane and row slot of the candidate
+    wave_argmax(bv, bp, who);
     const unsigned long long tag = (unsigned long long)(a.tag0 + (unsigned)j) << 32;
     unsigned long long *rec = slots + ((size_t)(j & 1) * RL_MAXG + g) * RL_REC;
-    // the candidate row from its own lane: word 4 + 2c (+1) holds lo (hi) of
-    // column c (the row slot is picked with bit masks, which keeps v[][] in
-    // registers -- a select chain is turned into a run-time index)
-    if (lane == who) {
-      if (slot >= 0) {
+    // the candidate row, gathered from its lane: word 4 + 2c (+1) holds lo (hi) of column c
+    unsigned wlo = 0u, whi = 0u;  // this lane's words `lane` and `lane + 64`
+    const int src_lane = who >= 0 ? who / RPL : 0, src_i = who >= 0 ? who % RPL : 0;
 #pragma unroll
-        for (int c = 0; c < W; c++) {
-          unsigned long long b = 0;
+    for (int c = 0; c < W; c++) {
+      double x = v[0][c];
 #pragma unroll
-          for (int i = 0; i < RPL; i++)
-            b |= (unsigned long long)__double_as_longlong(v[i][c]) & (0ULL - (unsigned long long)(slot == i));
-          st_dev(rec + 4 + 2 * c, tag | (unsigned)b);
-          st_dev(rec + 5 + 2 * c, tag | (unsigned)(b >> 32));
-        }
-      }
-      // header words after the row (readers check every word's tag anyway)
-      st_dev(rec, tag | (unsigned)bp);
-      st_dev(rec + 1, tag | lo32(bv));
-      st_dev(rec + 2, tag | hi32(bv));
+      for (int i = 1; i < RPL; i++)
+        if (src_i == i) x = v[i][c];
+      x = __shfl(x, src_lane, 64);
+      if (who < 0) x = 0.0;
+      if (lane == 4 + 2 * c) wlo = lo32(x);
+      if (lane == 5 + 2 * c) wlo = hi32(x);
+      if (4 + 2 * c >= 64 && lane == 4 + 2 * c - 64) whi = lo32(x);
+      if (5 + 2 * c >= 64 && lane == 5 + 2 * c - 64) whi = hi32(x);
     }
+    if (lane == 0) wlo = (unsigned)bp;
+    if (lane == 1) wlo = lo32(bv);
+    if (lane == 2) wlo = hi32(bv);
+    if (lane == 3) wlo = 0u;
+    if (lane < 4 + 2 * W) st_dev(rec + lane, tag | wlo);
+    if (4 + 2 * W > 64 && lane < 4 + 2 * W - 64) st_dev(rec + 64 + lane, tag | whi);
   };
 
   publish(0);
@@ -257,9 +224,7 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
           if (__all(ok)) break;
           if (spin_fail(it, a)) break;
         }
-        const int mypos = gp;
-        wave_argmax(gv, gp);  // wave-uniform; the winning record's lane holds its index
-        gw = gp == INT_MAX ? -1 : __builtin_amdgcn_readlane(gw, lane_of(mypos, gp));
+        wave_argmax(gv, gp, gw);
       }
       // the winner's row (all-NaN column: no winner, no interchange); lane l < 2W
       // holds record word 4 + l, so column c is in lanes 2c (low) and 2c + 1 (high)
