@@ -48,6 +48,84 @@ def cpu_baseline(n):
                       f"drand48 seed 0; {dt:.1f} s"}
 
 
+def run_batched(pla, s, count, nrhs, reps, seed=2):
+    """Batched uneven SPD solves: `count` systems, n_i uniform in [16, 512],
+    A_i = B_i B_i^T / n_i + 1e-2 I (B_i uniform[-1,1]), nrhs uniform[-1,1]
+    right-hand sides each, in one persistent kernel replayed from a hipGraph
+    (PLA_Batched_plan_*).  FP64 factor-and-solve, then FP32 factors with FP64
+    refinement to a backward error <= 1e-14.  Each timed replay starts from
+    pristine inputs already in HBM (restored by device copies outside the
+    timed region)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(16, 513, count).astype(np.int32)
+    a_len = ns.astype(np.int64) ** 2
+    b_len = ns.astype(np.int64) * nrhs
+    a_off = np.concatenate([[0], np.cumsum(a_len)[:-1]]).astype(np.int64)
+    b_off = np.concatenate([[0], np.cumsum(b_len)[:-1]]).astype(np.int64)
+    ha = np.empty(int(a_len.sum()))
+    hb = rng.uniform(-1.0, 1.0, int(b_len.sum()))
+    for i, n in enumerate(ns):
+        bm = rng.uniform(-1.0, 1.0, (n, n))
+        a = bm @ bm.T / n
+        a[np.diag_indices(n)] += 1e-2
+        ha[a_off[i]:a_off[i] + a_len[i]] = a.reshape(-1, order="F")
+    flops = float((ns.astype(np.float64) ** 3).sum() / 3.0)
+
+    def dev(host, dt):
+        o = s.mvector(host.size, 1, dt)
+        pla.set_contents(o, host)
+        ptr = ctypes.c_void_p()
+        pla.call("PLA_Obj_device_buffer", o, ctypes.byref(ptr))
+        return o, ptr
+
+    A0, _ = dev(ha, pla.MPI_DOUBLE)
+    B0, _ = dev(hb, pla.MPI_DOUBLE)
+    A, pa = dev(ha, pla.MPI_DOUBLE)
+    B, pb = dev(hb, pla.MPI_DOUBLE)
+    del ha
+    info, pinfo = dev(np.zeros(count, np.int32), pla.MPI_INT)
+    iters, piters = dev(np.zeros(count, np.int32), pla.MPI_INT)
+    berr, pberr = dev(np.zeros(count), pla.MPI_DOUBLE)
+    nptr = ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    aoff = a_off.ctypes.data_as(ctypes.c_void_p)
+    boff = b_off.ctypes.data_as(ctypes.c_void_p)
+    out = {"systems": count, "n_range": [16, 512], "rhs": nrhs, "flops_chol": flops}
+    for name, mode in (("fp64", pla.PLA_BATCHED_FP64), ("fp32_ir", pla.PLA_BATCHED_FP32_IR)):
+        plan = ctypes.c_void_p()
+        pla.call("PLA_Batched_plan_create", count, nptr, aoff, boff, nrhs, mode | pla.PLA_BATCHED_AUTOTUNE,
+                 1e-14, 10, pa, pb, pinfo, piters, pberr, ctypes.byref(plan))
+        nbv, wpc = ctypes.c_int(), ctypes.c_int()
+        pla.call("PLA_Batched_plan_config", plan, ctypes.byref(nbv), ctypes.byref(wpc))
+        times = []
+        for r in range(reps + 1):  # the first replay is warmup
+            pla.call("PLA_Copy", A0, A)
+            pla.call("PLA_Copy", B0, B)
+            pla.call("PLA_Device_synchronize")
+            t = time.perf_counter()
+            pla.call("PLA_Batched_plan_execute", plan)
+            pla.call("PLA_Device_synchronize")
+            if r:
+                times.append(time.perf_counter() - t)
+        pla.call("PLA_Batched_plan_free", ctypes.byref(plan))
+        inf = pla.get_contents(info)[:, 0]
+        if (inf != 0).any():
+            raise RuntimeError(f"batched {name}: {int((inf != 0).sum())} systems reported a failure")
+        t = float(np.median(times))
+        res = {"solves_per_s": round(count / t, 1), "ms": round(t * 1e3, 3),
+               "tflops_chol": round(flops / t / 1e12, 3), "nb": nbv.value, "workgroups_per_cu": wpc.value}
+        if mode == pla.PLA_BATCHED_FP32_IR:
+            be = pla.get_contents(berr)[:, 0]
+            it = pla.get_contents(iters)[:, 0]
+            res["max_backward_error"] = float(be.max())
+            res["refinement_steps_max"] = int(it.max())
+            if not be.max() <= 1e-14:
+                raise RuntimeError(f"batched fp32_ir: backward error {be.max()}")
+        out[name] = res
+        print(f"batched {name}: {res}", file=sys.stderr, flush=True)
+    pla.free(A0, B0, A, B, info, iters, berr)
+    return out
+
+
 def timing_query(pla, role):
     launches, ms, fl = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
     pla.call("PLA_Kernel_timing_query", role, ctypes.byref(launches), ctypes.byref(ms), ctypes.byref(fl))
@@ -78,6 +156,8 @@ def main():
     ap.add_argument("--nb", type=int, default=0, help="block size (0: library's choice)")
     ap.add_argument("--plain-steps", type=int, default=1, help="plain FP64 steps timed beside mixed (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=4096, help="oracle sample size (0: skip)")
+    ap.add_argument("--batched", type=int, default=10000, help="batched SPD systems (0: skip)")
+    ap.add_argument("--batched-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -92,8 +172,10 @@ def main():
             "b = A*1, generated on the GPU")
     with pla.Session() as s:
         pla.call("PLA_Set_device_nb", args.nb)
-        nbv = ctypes.c_int()
-        pla.call("PLA_Get_device_nb", n, ctypes.byref(nbv))
+        nbv, nb_plain = ctypes.c_int(), ctypes.c_int()
+        pla.call("PLA_Get_device_nb_for", n, pla.PLA_NB_LU_MIXED if args.mode == "mixed" else pla.PLA_NB_LU,
+                 ctypes.byref(nbv))
+        pla.call("PLA_Get_device_nb_for", n, pla.PLA_NB_LU, ctypes.byref(nb_plain))
         A0 = s.matrix(n, n)
         pla.call("PLA_Obj_fill_drand48", A0, 0, 0)
         ones, b, x, r = s.mvector(n), s.mvector(n), s.mvector(n), s.mvector(n)
@@ -139,7 +221,7 @@ def main():
             eta, xerr = backward_error()
             ms = el / steps * 1e3
             return {"value": round(work / (ms * 1e-3) / 1e12, 3), "ms_per_step": round(ms, 2),
-                    "steps": steps, "warmup": warmup, "roofline": rl,
+                    "steps": steps, "warmup": warmup, "nb": nb_plain.value, "roofline": rl,
                     "backward_error_inf": eta, "hpl_scaled_residual": eta / (n * 2.0 ** -52),
                     "x_minus_1_inf": xerr}
 
@@ -206,6 +288,8 @@ def main():
                 out["plain_fp64"] = plain
                 out["speedup_vs_plain"] = round(value / plain["value"], 3)
         pla.free(A0, ones, b, x, r, P, one, zero, mone, nrm)
+        if args.batched > 0:
+            out["batched"] = run_batched(pla, s, args.batched, 4, args.batched_reps)
 
     if args.cpu_n > 0:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

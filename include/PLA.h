@@ -368,6 +368,13 @@ int PLA_Set_device_nb(int nb);
 #define PLA_PARAM_LEAF_KIND 5
 int PLA_Set_device_param(int param, int value);
 int PLA_Get_device_nb(int n, int *nb);
+/* Block size the device uses for an n x n factorization of the given kind:
+   PLA_NB_LU (PLA_LU; what PLA_Get_device_nb returns), PLA_NB_CHOL
+   (PLA_Chol), PLA_NB_LU_MIXED (PLA_LU_mixed). */
+#define PLA_NB_LU 0
+#define PLA_NB_CHOL 1
+#define PLA_NB_LU_MIXED 2
+int PLA_Get_device_nb_for(int n, int factorization, int *nb);
 /* Per-launch hipEvent timing of the trailing-update kernels on the library
    stream (the reference keeps per-operation timers in PLA_TIMINGS,
    OBJ/PLA_Timings.c:17-125).  role 1: PLA_LU's A22 -= A21 A12 GEMM;

@@ -124,7 +124,7 @@ int PLA_Chol(int uplo, PLA_Obj A) {
   int rc = dev_ptr(A, (void **)&a);
   if (rc) return rc;
   int info = 0;
-  if ((rc = chol_factor(ctx().stream, n, a, ld(A), choose_nb(n, PLA_OP_PAN_PAN), &info))) return rc;
+  if ((rc = chol_factor(ctx().stream, n, a, ld(A), choose_nb(n, NB_CHOL), &info))) return rc;
   if ((rc = written(A))) return rc;
   return info;
 }

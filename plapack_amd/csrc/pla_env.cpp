@@ -228,12 +228,15 @@ void release_workspaces() {
   g_leaf = leaf_ws();
 }
 
-int choose_nb(int n, int op) {
+int choose_nb(int n, int kind) {
   const context &c = g_ctx;
   if (c.nb_device > 0) return c.nb_device;
   if (c.nb_user > 0) return c.nb_user;
-  (void)op;
-  // measured on MI355X, plain FP64 LU n=32768: nb 256 / 384 / 512 -> 25.8 / 27.1 / 27.9 TF/s
+  // Measured on MI355X.  Plain FP64 LU: n=32768 nb 256 / 384 / 512 -> 25.8 /
+  // 27.1 / 27.9 TF/s; n=65536 nb 512 / 1024 -> 42.6 / 45.1 TF/s (the
+  // trailing GEMM runs faster at K = 1024).  Mixed LU n=65536: nb 512 / 1024
+  // -> 101.3 / 92.1 TF/s (the wider FP64 panel lengthens the critical path).
+  if (kind == NB_LU && n >= 49152) return 1024;
   if (n >= 16384) return 512;
   if (n >= 4096) return 256;
   return 128;
@@ -443,9 +446,17 @@ int PLA_Set_device_param(int param, int value) {
   }
 }
 
+int PLA_Get_device_nb_for(int n, int factorization, int *nb) {
+  if (!nb) return fail(PLA_ERR_INVALID_PARAM, "PLA_Get_device_nb_for: NULL");
+  if (factorization < PLA_NB_LU || factorization > PLA_NB_LU_MIXED)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Get_device_nb_for: factorization %d", factorization);
+  *nb = pla::choose_nb(n, factorization);
+  return PLA_SUCCESS;
+}
+
 int PLA_Get_device_nb(int n, int *nb) {
   if (!nb) return fail(PLA_ERR_INVALID_PARAM, "PLA_Get_device_nb: NULL");
-  *nb = pla::choose_nb(n, PLA_OP_PAN_PAN);
+  *nb = pla::choose_nb(n, pla::NB_LU);
   return PLA_SUCCESS;
 }
 

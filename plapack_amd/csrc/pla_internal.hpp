@@ -120,8 +120,9 @@ int dev_ptr(PLA_Obj o, void **p);
 int written(PLA_Obj o);
 int ld(PLA_Obj o);
 
-// Block size the solvers use for an n x n problem.
-int choose_nb(int n, int op);
+// Block size a factorization of kind NB_* uses for an n x n problem.
+enum { NB_LU = PLA_NB_LU, NB_CHOL = PLA_NB_CHOL, NB_LU_MIXED = PLA_NB_LU_MIXED };
+int choose_nb(int n, int kind);
 
 // Device scalar (1x1 object) read back to the host.
 int scalar_value(PLA_Obj o, double *v);
@@ -181,7 +182,7 @@ int rleaf_pairs_stride();
 // Widest leaf (32, 16 or 8 columns) whose rows fit in LDS when a panel of mp
 // rows is spread over at most gcap workgroups; 0 if none fits.
 int leaf_width_for(int mp, int gcap);
-constexpr int LU_MAX_LEAVES = 64;  // leaves of one panel (nb <= 512, width >= 8)
+constexpr int LU_MAX_LEAVES = 128;  // leaves of one panel (nb <= 1024, width >= 8)
 // Compose the leaves' (dst, src) row lists of one panel into one list (comp,
 // panel_comp_stride() ints) and apply it to the rows of a panel (A = its first
 // row) in columns [0, nleft) and [c1, c2).

@@ -492,7 +492,7 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
 // applied one after the other, are replayed on a position -> source-row map
 // kept in an LDS hash table (one workgroup), and the positions whose source
 // changed become one list of at most 2*nb (dst, src) pairs: count, then pairs.
-constexpr int COMP_MAX = 1024;  // 2 * the largest panel width
+constexpr int COMP_MAX = 2048;  // 2 * the largest panel width (1024)
 constexpr int HASH_SZ = 4096;
 
 __device__ inline int hslot(int key) { return (int)(((unsigned)key * 2654435761u) >> 20); }  // 12 bits

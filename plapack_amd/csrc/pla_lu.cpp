@@ -47,7 +47,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
 //                       -> signal -> A22[:, rest] -= A21 A12
 // look-ahead stream:    wait(signal) -> factor panel k+1 -> signal(panel k+1)
 int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, int nb) {
-  if (nb > 512) nb = 512;
+  if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
   const int stride = leaf_pairs_stride();
   const int cstride = panel_comp_stride();
@@ -158,7 +158,7 @@ int PLA_LU(PLA_Obj A, PLA_Obj pivots) {
   double *a;
   int *piv;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(pivots, (void **)&piv))) return rc;
-  const int nb = choose_nb(mn, PLA_OP_PAN_PAN);
+  const int nb = choose_nb(mn, NB_LU);
   if ((rc = lu_factor(ctx().stream, m, n, a, ld(A), piv, nb))) return rc;
   if ((rc = written(A))) return rc;
   return written(pivots);

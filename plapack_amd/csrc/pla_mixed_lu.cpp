@@ -23,7 +23,7 @@ enum { WS_W = 10, WS_P64, WS_U64, WS_LH, WS_LL, WS_UH, WS_UL, WS_PERM, WS_DINVL,
 int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int *piv, int nb) {
   nb = (nb / 32) * 32;
   if (nb < 32) nb = 32;
-  if (nb > 512) nb = 512;
+  if (nb > 1024) nb = 1024;
   const int nbp = nb;  // low-precision copies are padded to nb columns/rows (a multiple of 32)
   const int esz = 2;   // bytes of one FP16 / BF16 element
   const bool split = prec_split(prec);
@@ -166,7 +166,7 @@ int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision,
   float *W = (float *)workspace(sizeof(float) * (size_t)n * n, WS_W);
   if (!W) return PLA_ERR_NO_MEMORY;
   if ((rc = k_cvt_f64_f32(s, n, n, a, lda, W, ldw))) return rc;
-  if ((rc = lu_factor_mixed(s, precision, n, W, ldw, piv, choose_nb(n, PLA_OP_PAN_PAN)))) return rc;
+  if ((rc = lu_factor_mixed(s, precision, n, W, ldw, piv, choose_nb(n, NB_LU_MIXED)))) return rc;
 
   // row permutation of the interchanges (position i holds original row perm[i])
   std::vector<int> hp(n), perm(n);

@@ -53,6 +53,9 @@ PLA_PARAM_LEAF_ROWS = 2
 PLA_PARAM_PANEL_CUS = 3
 PLA_PARAM_LEAF_WIDTH = 4
 PLA_PARAM_LEAF_KIND = 5
+PLA_NB_LU = 0
+PLA_NB_CHOL = 1
+PLA_NB_LU_MIXED = 2
 PLA_BATCHED_FP64 = 1
 PLA_BATCHED_FP32_IR = 2
 PLA_BATCHED_AUTOTUNE = 0x100
@@ -139,6 +142,7 @@ _SIGS = {
     "PLA_Set_device_nb": [_i],
     "PLA_Set_device_param": [_i, _i],
     "PLA_Get_device_nb": [_i, _pi],
+    "PLA_Get_device_nb_for": [_i, _i, _pi],
     "PLA_Kernel_timing": [_i],
     "PLA_Kernel_timing_query": [_i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_d),
                                 ctypes.POINTER(_d)],

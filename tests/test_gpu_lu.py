@@ -55,7 +55,8 @@ def test_lu_golden_n2048_seed0(pla_session, ora):
 
 @pytest.mark.parametrize("m,n,nb", [(1, 1, 128), (5, 5, 128), (100, 37, 32), (37, 100, 32),
                                     (300, 300, 64), (1000, 1000, 128), (1000, 1000, 256),
-                                    (2049, 2049, 128), (1500, 700, 256), (700, 1500, 512)])
+                                    (2049, 2049, 128), (1500, 700, 256), (700, 1500, 512),
+                                    (3000, 3000, 1024), (2500, 1100, 1024)])
 def test_lu_shapes_vs_oracle(pla_session, ora, m, n, nb):
     a = ora.uniform(m, n, seed=m + 3 * n)
     lu, piv = _lu_gpu(pla_session, a=a, nb=nb)

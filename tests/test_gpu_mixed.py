@@ -43,7 +43,7 @@ def _eta(a, x, b):
 
 
 @pytest.mark.parametrize("prec", [pla.PLA_MIXED_FP16X3, pla.PLA_MIXED_BF16X3])
-@pytest.mark.parametrize("n,nrhs,nb", [(300, 1, 64), (1000, 16, 128), (2049, 4, 256), (4096, 1, 512)])
+@pytest.mark.parametrize("n,nrhs,nb", [(300, 1, 64), (1000, 16, 128), (2049, 4, 256), (4096, 1, 512), (5000, 2, 1024)])
 def test_split_precisions_reach_fp64_backward_error(pla_session, ora, prec, n, nrhs, nb):
     a = ora.uniform(n, n, seed=n)
     b = np.random.default_rng(n).uniform(-1, 1, (n, nrhs))
