@@ -132,12 +132,18 @@ def timing_query(pla, role):
     return launches.value, ms.value, fl.value
 
 
-def roofline(kernel, launches, ms, flops, peak, mfma_per_flop=1):
+# HBM bytes per launch of the trailing-update kernels at n=65536 (rocprofv3
+# --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors measured on a
+# copy of known size): profiles/trailing_gemm_traffic_n65536.md
+TRAFFIC_N65536 = {"hgemm": 9.84e9, "dgemm": 2.87e10}
+
+
+def roofline(kernel, launches, ms, flops, peak, mfma_per_flop=1, traffic=None):
     achieved = flops / (ms * 1e-3) / 1e12 if ms > 0 else None
     out = {"kernel": kernel, "bound": "mfma", "achieved": round(achieved, 3) if achieved else None,
            "peak": peak, "unit": "TFLOP/s",
            "frac": round(mfma_per_flop * achieved / peak, 4) if achieved else None,
-           "traffic": None, "launches": launches,
+           "traffic": traffic, "launches": launches,
            "avg_launch_ms": round(ms / max(1, launches), 4)}
     if mfma_per_flop != 1:
         out["mfma_products_per_flop"] = mfma_per_flop
@@ -158,6 +164,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=4096, help="oracle sample size (0: skip)")
     ap.add_argument("--batched", type=int, default=10000, help="batched SPD systems (0: skip)")
     ap.add_argument("--batched-reps", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 16384 x 16384 FP64 matrix 3 times (2 GiB read + 2 GiB written each): "
+                         "the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -165,6 +174,17 @@ def main():
         sys.exit(2)
 
     from plapack_amd import pla
+
+    if args.copy_calibration:
+        with pla.Session() as s:
+            c_a, c_b = s.matrix(16384, 16384), s.matrix(16384, 16384)
+            pla.call("PLA_Obj_fill_drand48", c_a, 0, 0)
+            for _ in range(3):
+                pla.call("PLA_Copy", c_a, c_b)
+            pla.call("PLA_Device_synchronize")
+            pla.free(c_a, c_b)
+        print(json.dumps({"copy_calibration_bytes": 8 * 16384 * 16384}), flush=True)
+        return
 
     n = args.n
     work = 2.0 / 3.0 * n ** 3
@@ -215,7 +235,7 @@ def main():
             pla.call("PLA_Device_synchronize")
             el = time.perf_counter() - t0
             rl = roofline("dgemm_kernel<NN, ROLE 1> (A22 -= A21*A12, FP64)", *timing_query(pla, 1),
-                          FP64_MFMA_PEAK)
+                          FP64_MFMA_PEAK, traffic=TRAFFIC_N65536["dgemm"] if n == 65536 and args.nb == 0 else None)
             pla.call("PLA_Kernel_timing", 0)
             pla.free(A)
             eta, xerr = backward_error()
@@ -263,7 +283,8 @@ def main():
             el = time.perf_counter() - t0
             split = 3 if args.prec in (3, 4) else 1
             rl = roofline(f"hgemm_kernel<{PREC_NAMES[args.prec]}> (W22 -= L21*U12, FP32 accumulate)",
-                          *timing_query(pla, 3), FP16_MFMA_PEAK, mfma_per_flop=split)
+                          *timing_query(pla, 3), FP16_MFMA_PEAK, mfma_per_flop=split,
+                          traffic=TRAFFIC_N65536["hgemm"] if n == 65536 and args.nb == 0 and args.prec == 3 else None)
             pla.call("PLA_Kernel_timing", 0)
             pla.call("PLA_Copy", X, x)
             eta, xerr = backward_error()   # recomputed independently (GEMV on A0)
