@@ -360,12 +360,18 @@ int PLA_Set_device_nb(int nb);
    that fits).
    PLA_PARAM_LEAF_KIND: 0 (default) keeps the leaf's rows in registers, one
    wave per workgroup and no LDS, so it runs beside the trailing update; 1
-   keeps them in LDS, four waves per workgroup. */
+   keeps them in LDS, four waves per workgroup.
+   PLA_PARAM_NB_TAIL / PLA_PARAM_NB_SWITCH: the LU factorizations use block
+   size NB_TAIL (at most the main block size) for the steps where at most
+   NB_SWITCH rows remain; NB_TAIL 0 (default) keeps one block size
+   throughout. */
 #define PLA_PARAM_LOOKAHEAD 1
 #define PLA_PARAM_LEAF_ROWS 2
 #define PLA_PARAM_PANEL_CUS 3
 #define PLA_PARAM_LEAF_WIDTH 4
 #define PLA_PARAM_LEAF_KIND 5
+#define PLA_PARAM_NB_TAIL 6
+#define PLA_PARAM_NB_SWITCH 7
 int PLA_Set_device_param(int param, int value);
 int PLA_Get_device_nb(int n, int *nb);
 /* Block size the device uses for an n x n factorization of the given kind:

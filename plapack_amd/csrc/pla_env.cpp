@@ -242,6 +242,16 @@ int choose_nb(int n, int kind) {
   return 128;
 }
 
+void nb_tail(int nb, int *tail, int *rows) {
+  const context &c = g_ctx;
+  *tail = nb;
+  *rows = 0;
+  if (c.nb_tail > 0) {  // PLA_PARAM_NB_TAIL / PLA_PARAM_NB_SWITCH
+    *tail = c.nb_tail < nb ? c.nb_tail : nb;
+    *rows = c.nb_switch;
+  }
+}
+
 }  // namespace pla
 
 using pla::fail;
@@ -435,6 +445,14 @@ int PLA_Set_device_param(int param, int value) {
     case PLA_PARAM_LEAF_KIND:
       if (value != 0 && value != 1) return fail(PLA_ERR_INVALID_PARAM, "leaf kind must be 0 or 1");
       c.leaf_kind = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_NB_TAIL:
+      if (value < 0) return fail(PLA_ERR_INVALID_PARAM, "tail block size %d", value);
+      c.nb_tail = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_NB_SWITCH:
+      if (value < 0) return fail(PLA_ERR_INVALID_PARAM, "switch rows %d", value);
+      c.nb_switch = value;
       return PLA_SUCCESS;
     case PLA_PARAM_LEAF_WIDTH:
       if (value != 0 && value != 8 && value != 16 && value != 32)

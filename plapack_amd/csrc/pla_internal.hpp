@@ -73,6 +73,8 @@ struct context {
   int panel_cus = 0;        // CUs reserved for the LU panel (0: panel shares every CU)
   int leaf_width = 0;       // panel-LU leaf width (0: the widest that fits)
   int leaf_kind = 0;        // panel-LU leaf: 0 rows in registers, 1 rows in LDS
+  int nb_tail = 0;          // LU block size once at most nb_switch rows remain (0: none)
+  int nb_switch = 0;
   unsigned leaf_spin_limit = 400000u;  // bounded wait of a panel-leaf exchange (polls)
   // CU-masked stream pair of the LU look-ahead: the panel stream runs on
   // masked_cus CUs (masked_cus / 8 per XCD), the trailing stream on the rest
@@ -123,6 +125,9 @@ int ld(PLA_Obj o);
 // Block size a factorization of kind NB_* uses for an n x n problem.
 enum { NB_LU = PLA_NB_LU, NB_CHOL = PLA_NB_CHOL, NB_LU_MIXED = PLA_NB_LU_MIXED };
 int choose_nb(int n, int kind);
+// Block size of the last steps of an LU with block size nb: *tail once at
+// most *rows rows remain (*tail == nb: no change).
+void nb_tail(int nb, int *tail, int *rows);
 
 // Device scalar (1x1 object) read back to the host.
 int scalar_value(PLA_Obj o, double *v);

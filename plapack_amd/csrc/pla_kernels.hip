@@ -6,7 +6,8 @@
 //                     dgemm/dsyrk calls of PLA_Syrk_perform_local_part
 //                     (BLAS3/PLA_Syrk_perform_local_part.c:54,133,136):
 //                     FP64 MFMA (v_mfma_f64_16x16x4_f64), 128x128 block
-//                     tile, LDS-staged 16-deep K panels, double buffered.
+//                     tile, LDS-staged 16-deep K panels, double buffered,
+//                     next-step loads interleaved with the MFMAs.
 //   lu_leaf_kernel    PLA_LU_right_mv (SOLVERS/LU/PLA_LU_mv.c:52-82):
 //                     unblocked panel LU of up to 32 columns; PLA_Iamax
 //                     (BLAS1/PLA_Iamax.c:37-124) becomes a workgroup
@@ -36,7 +37,8 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // DGEMM: C = beta*C + alpha*op(A)*op(B), m x n result, inner dimension k.
 // Block tile 128 x 128, 4 waves in a 2 x 2 arrangement, 64 x 64 per wave as a
-// 4 x 4 grid of 16 x 16 MFMA tiles.  K is staged through LDS 16 at a time.
+// 4 x 4 grid of 16 x 16 MFMA tiles.  K is staged through LDS 16 at a time,
+// double buffered, one barrier per 16-deep step.
 // MFMA f64 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16];
 // result lane l, register r holds D[l/16 + 4r][l%16].  The kernel feeds the
 // B fragment as the A operand and vice versa, so D is the transposed C tile
@@ -46,40 +48,52 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // tiles entirely above the diagonal exit at once (SYRK).
 // ROLE only separates kernel symbols (the trailing update is ROLE 1) so that
 // profiles can attribute time to the LU/Cholesky trailing update.
+//
+// Two tile paths share the LDS layout and the MFMA step:
+//  * interior tiles (all 128 x 128 in range) with alpha = +-1 load operand
+//    pairs with 16-byte loads from a uniform (SGPR) base plus one 32-bit lane
+//    offset, and interleave the next step's loads and LDS stores with the
+//    current step's 64 MFMAs (sched_group_barrier), so no wave idles in a
+//    separate load/store phase.  alpha = -1 is applied exactly through the
+//    sign of the accumulator: acc = -(beta C), acc += op(A) op(B), C = -acc
+//    (round-to-nearest is symmetric, so this equals beta C - op(A) op(B)
+//    accumulated in the same order);
+//  * edge tiles and general alpha use element loads with bounds checks and
+//    scale A by alpha when it is staged.
 constexpr int GBM = 128, GBN = 128, GBK = 16;
 constexpr int APAD = 4;   // As row stride GBM + APAD doubles
 constexpr int BPAD = 1;   // Bs row stride GBK + BPAD doubles
+typedef double dbl2u __attribute__((ext_vector_type(2), aligned(8)));
 
-template <bool TA, bool TB, bool LOWER, int ROLE>
-__global__ __launch_bounds__(256, 2) void dgemm_kernel(
-    int m, int n, int k, double alpha, const double *__restrict__ A, long lda,
-    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
-    int tiles_m, int tiles_n, const int *__restrict__ abort_flag) {
-  if (abort_flag && *abort_flag) return;
-  const int ntiles = tiles_m * tiles_n;
-  // Consecutive workgroup ids are dispatched round-robin over the 8 XCDs;
-  // give each XCD a contiguous range of tiles so its L2 sees neighbours.
-  const int nblk = gridDim.x;  // a multiple of 8
-  int bid = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-  if (bid >= ntiles) return;
-  constexpr int GROUP = 8;  // tile rows per group: A rows are reused inside a group
-  const int group = bid / (GROUP * tiles_n);
-  const int first_m = group * GROUP;
-  const int gsz = min(tiles_m - first_m, GROUP);
-  const int within = bid - group * GROUP * tiles_n;
-  const int tm = first_m + within % gsz;
-  const int tn = within / gsz;
-  const int i0 = tm * GBM, j0 = tn * GBN;
-  if (LOWER && i0 + GBM <= j0) return;
+struct gemm_smem {
+  double As[2][GBK][GBM + APAD];  // [depth][row]
+  double Bs[2][GBN][GBK + BPAD];  // [column][depth]
+};
 
-  __shared__ double As[2][GBK][GBM + APAD];
-  __shared__ double Bs[2][GBN][GBK + BPAD];
+// One 16-deep MFMA step of a wave's 64 x 64 block from LDS buffer cur.
+__device__ __forceinline__ void gemm_mfma_step(const gemm_smem &sm, int cur, int wm, int wn, int l16, int lq,
+                                               dbl4 (&acc)[4][4]) {
+#pragma unroll
+  for (int kq = 0; kq < 4; kq++) {
+    double af[4], bf[4];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++) af[bi] = sm.As[cur][kq * 4 + lq][wm * 64 + bi * 16 + l16];
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++) bf[bj] = sm.Bs[cur][wn * 64 + bj * 16 + l16][kq * 4 + lq];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+        acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[bj], af[bi], acc[bi][bj], 0, 0, 0);
+  }
+}
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
-  const int l16 = lane & 15, lq = lane >> 4;
-
-  dbl4 acc[4][4];
+// acc = sgn * beta * C over the tile (0 outside C or, for LOWER, above the diagonal)
+template <bool LOWER>
+__device__ __forceinline__ void gemm_load_c(int m, int n, double beta, double sgn, const double *__restrict__ C,
+                                            long ldc, int i0, int j0, dbl4 (&acc)[4][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
 #pragma unroll
   for (int bi = 0; bi < 4; bi++)
 #pragma unroll
@@ -89,11 +103,160 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
         const int gi = i0 + wm * 64 + bi * 16 + l16;
         const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
         double v = 0.0;
-        if (beta != 0.0 && gi < m && gj < n && (!LOWER || gi >= gj))
-          v = beta * C[(long)gj * ldc + gi];
-        acc[bi][bj][r] = v;
+        if (beta != 0.0 && gi < m && gj < n && (!LOWER || gi >= gj)) v = beta * C[(long)gj * ldc + gi];
+        acc[bi][bj][r] = sgn * v;
       }
+}
 
+template <bool LOWER>
+__device__ __forceinline__ void gemm_store_c(int m, int n, double sgn, double *__restrict__ C, long ldc, int i0,
+                                             int j0, const dbl4 (&acc)[4][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = i0 + wm * 64 + bi * 16 + l16;
+        const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
+        if (gi < m && gj < n && (!LOWER || gi >= gj)) C[(long)gj * ldc + gi] = sgn * acc[bi][bj][r];
+      }
+}
+
+// Interior tile, alpha = sgn = +-1.  Operand tiles are 128 (index) x 16
+// (depth).  An index-contiguous operand (A, or B transposed) is read as index
+// pairs (2p, 2p+1), p = tid % 64, at depths tid / 64 + 4e; a depth-contiguous
+// one (A transposed, or B) as depth pairs (2q, 2q+1), q = tid % 8, of indices
+// tid / 8 + 32e; e = 0..3.
+template <bool TA, bool TB, bool LOWER>
+__device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int k, double sgn,
+                                               const double *__restrict__ A, long lda, const double *__restrict__ B,
+                                               long ldb, double beta, double *__restrict__ C, long ldc, int i0,
+                                               int j0) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
+  const int ax = TA ? (tid >> 3) : 2 * (tid & 63), ak = TA ? 2 * (tid & 7) : (tid >> 6);
+  const int bx = TB ? 2 * (tid & 63) : (tid >> 3), bk = TB ? (tid >> 6) : 2 * (tid & 7);
+  const unsigned voa = (unsigned)((TA ? (long)ax * lda + ak : (long)ak * lda + ax) * 8);
+  const unsigned vob = (unsigned)((TB ? (long)bk * ldb + bx : (long)bx * ldb + bk) * 8);
+  auto pa = [&](int kb, int e) {
+    const double *base = TA ? A + (long)(i0 + 32 * e) * lda + kb : A + (long)(kb + 4 * e) * lda + i0;
+    return (const double *)((const char *)base + voa);
+  };
+  auto pb = [&](int kb, int e) {
+    const double *base = TB ? B + (long)(kb + 4 * e) * ldb + j0 : B + (long)(j0 + 32 * e) * ldb + kb;
+    return (const double *)((const char *)base + vob);
+  };
+  dbl2u ra[4], rb[4];
+  auto load = [&](int kb) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) ra[e] = *(const dbl2u *)pa(kb, e);
+#pragma unroll
+    for (int e = 0; e < 4; e++) rb[e] = *(const dbl2u *)pb(kb, e);
+  };
+  // last, partial 16-deep step: depths >= k read as zero
+  auto load_tail = [&](int kb) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const double *p = pa(kb, e);
+      const int d0 = TA ? kb + ak : kb + ak + 4 * e, d1 = TA ? d0 + 1 : d0;
+      ra[e].x = d0 < k ? p[0] : 0.0;
+      ra[e].y = d1 < k ? p[1] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const double *p = pb(kb, e);
+      const int d0 = TB ? kb + bk + 4 * e : kb + bk, d1 = TB ? d0 : d0 + 1;
+      rb[e].x = d0 < k ? p[0] : 0.0;
+      rb[e].y = d1 < k ? p[1] : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      if (!TA) {
+        sm.As[buf][ak + 4 * e][ax] = ra[e].x;
+        sm.As[buf][ak + 4 * e][ax + 1] = ra[e].y;
+      } else {
+        sm.As[buf][ak][ax + 32 * e] = ra[e].x;
+        sm.As[buf][ak + 1][ax + 32 * e] = ra[e].y;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      if (!TB) {
+        sm.Bs[buf][bx + 32 * e][bk] = rb[e].x;
+        sm.Bs[buf][bx + 32 * e][bk + 1] = rb[e].y;
+      } else {
+        sm.Bs[buf][bx][bk + 4 * e] = rb[e].x;
+        sm.Bs[buf][bx + 1][bk + 4 * e] = rb[e].y;
+      }
+    }
+  };
+  const int nkf = k / GBK, nk = (k + GBK - 1) / GBK;
+  dbl4 acc[4][4];
+  if (nk > 0) {
+    if (nkf > 0) load(0);
+    else load_tail(0);
+  }
+  gemm_load_c<LOWER>(m, n, beta, sgn, C, ldc, i0, j0, acc);
+  if (nk > 0) store(0);
+  __syncthreads();
+  int kt = 0;
+  for (; kt + 1 < nkf; kt++) {
+    const int cur = kt & 1;
+    load((kt + 1) * GBK);
+    gemm_mfma_step(sm, cur, wm, wn, l16, lq, acc);
+    store(cur ^ 1);
+    // 64 MFMAs in four groups of 16 (one per 4-deep slice).  The 8 loads of
+    // the next step are spread over the first group; each slice's fragment
+    // reads are issued two MFMAs before the end of the previous group; the 8
+    // LDS stores of the next step (whose loads have landed by then) are spread
+    // over the last group.
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      if (q < 2) __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+    __syncthreads();
+  }
+  if (nkf > 0 && nk > nkf) {
+    load_tail(nkf * GBK);
+    gemm_mfma_step(sm, kt & 1, wm, wn, l16, lq, acc);
+    store((kt & 1) ^ 1);
+    __syncthreads();
+    kt++;
+  }
+  if (nk > 0) gemm_mfma_step(sm, kt & 1, wm, wn, l16, lq, acc);
+  gemm_store_c<LOWER>(m, n, sgn, C, ldc, i0, j0, acc);
+}
+
+// Any tile, any alpha: element loads with bounds checks, A scaled by alpha.
+template <bool TA, bool TB, bool LOWER>
+__device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, int k, double alpha,
+                                                  const double *__restrict__ A, long lda,
+                                                  const double *__restrict__ B, long ldb, double beta,
+                                                  double *__restrict__ C, long ldc, int i0, int j0) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
+  dbl4 acc[4][4];
+  gemm_load_c<LOWER>(m, n, beta, 1.0, C, ldc, i0, j0, acc);
   double ra[8], rb[8];
   auto load_tile = [&](int kt) {
     const int kb = kt * GBK;
@@ -127,7 +290,7 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
       int i, kk;
       if (!TA) { i = idx & 127; kk = idx >> 7; }
       else { kk = idx & 15; i = idx >> 4; }
-      As[buf][kk][i] = alpha * ra[e];
+      sm.As[buf][kk][i] = alpha * ra[e];
     }
 #pragma unroll
     for (int e = 0; e < 8; e++) {
@@ -135,10 +298,9 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
       int j, kk;
       if (!TB) { kk = idx & 15; j = idx >> 4; }
       else { j = idx & 127; kk = idx >> 7; }
-      Bs[buf][j][kk] = rb[e];
+      sm.Bs[buf][j][kk] = rb[e];
     }
   };
-
   const int nk = (k + GBK - 1) / GBK;
   if (nk > 0) {
     load_tile(0);
@@ -148,33 +310,40 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load_tile(kt + 1);
-#pragma unroll
-    for (int kq = 0; kq < 4; kq++) {
-      double af[4], bf[4];
-#pragma unroll
-      for (int bi = 0; bi < 4; bi++) af[bi] = As[cur][kq * 4 + lq][wm * 64 + bi * 16 + l16];
-#pragma unroll
-      for (int bj = 0; bj < 4; bj++) bf[bj] = Bs[cur][wn * 64 + bj * 16 + l16][kq * 4 + lq];
-#pragma unroll
-      for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-        for (int bj = 0; bj < 4; bj++)
-          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[bj], af[bi], acc[bi][bj], 0, 0, 0);
-    }
+    gemm_mfma_step(sm, cur, wm, wn, l16, lq, acc);
     if (kt + 1 < nk) store_tile(cur ^ 1);
     __syncthreads();
   }
+  gemm_store_c<LOWER>(m, n, 1.0, C, ldc, i0, j0, acc);
+}
 
-#pragma unroll
-  for (int bi = 0; bi < 4; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 4; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int gi = i0 + wm * 64 + bi * 16 + l16;
-        const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
-        if (gi < m && gj < n && (!LOWER || gi >= gj)) C[(long)gj * ldc + gi] = acc[bi][bj][r];
-      }
+// fast: alpha is +-1 and every operand lane offset fits 32 bits (host check)
+template <bool TA, bool TB, bool LOWER, int ROLE>
+__global__ __launch_bounds__(256, 2) void dgemm_kernel(
+    int m, int n, int k, double alpha, const double *__restrict__ A, long lda,
+    const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n, const int *__restrict__ abort_flag, int fast) {
+  if (abort_flag && *abort_flag) return;
+  const int ntiles = tiles_m * tiles_n;
+  // Consecutive workgroup ids are dispatched round-robin over the 8 XCDs;
+  // give each XCD a contiguous range of tiles so its L2 sees neighbours.
+  const int nblk = gridDim.x;  // a multiple of 8
+  int bid = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+  if (bid >= ntiles) return;
+  constexpr int GROUP = 8;  // tile rows per group: A rows are reused inside a group
+  const int group = bid / (GROUP * tiles_n);
+  const int first_m = group * GROUP;
+  const int gsz = min(tiles_m - first_m, GROUP);
+  const int within = bid - group * GROUP * tiles_n;
+  const int tm = first_m + within % gsz;
+  const int tn = within / gsz;
+  const int i0 = tm * GBM, j0 = tn * GBN;
+  if (LOWER && i0 + GBM <= j0) return;
+  __shared__ gemm_smem sm;
+  if (fast && i0 + GBM <= m && j0 + GBN <= n)
+    gemm_tile_fast<TA, TB, LOWER>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
+  else
+    gemm_tile_checked<TA, TB, LOWER>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
 }
 
 // ---------------------------------------------------------------------------
@@ -861,8 +1030,12 @@ static void launch_gemm(hipStream_t s, int m, int n, int k, double alpha, const 
                         const int *abort_flag) {
   const int tm = (m + GBM - 1) / GBM, tn = (n + GBN - 1) / GBN;
   const int nblk = ((tm * tn + 7) / 8) * 8;
+  // the interior path addresses a lane's operand pairs as a 32-bit byte offset
+  // of at most (31 ld + 128) doubles from a uniform base
+  const long lim = (1L << 32) / 8 - 128;
+  const int fast = (alpha == 1.0 || alpha == -1.0) && 31 * lda < lim && 31 * ldb < lim;
   hipLaunchKernelGGL((dgemm_kernel<TA, TB, LOWER, ROLE>), dim3(nblk), dim3(256), 0, s, m, n, k, alpha,
-                     A, lda, B, ldb, beta, C, ldc, tm, tn, abort_flag);
+                     A, lda, B, ldb, beta, C, ldc, tm, tn, abort_flag, fast);
 }
 
 int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, double alpha,

@@ -77,9 +77,9 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
 }
 
 // The interchanges of panel k on the columns left of it (L, which nothing
-// reads until the factorization ends) go to sL when it differs from sM, off
-// the trailing update's stream; the compose of panel k+2 into the same
-// double-buffered list waits for them (ev_left).
+// reads until the factorization ends) go to sL when it differs from sM, else
+// to the end of step k on sM; the compose of panel k+2 into the same
+// double-buffered list waits for them (ev_left, or stream order).
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
                            long lda, int *piv, int nb, int *pairs, size_t pstride, int *comp, int cstride,
                            hipEvent_t ev_ready, hipEvent_t ev_panel, hipEvent_t ev_left) {
@@ -87,16 +87,26 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
   const int mn = m < n ? m : n;
   int rc;
   int nleaves[2] = {0, 0};
+  // block size of the step at column k: nb, then the tail block size once at
+  // most nb_switch rows remain (a narrower panel shortens the critical path
+  // when the trailing update no longer hides it)
+  int tail, sw;
+  nb_tail(nb, &tail, &sw);
+  auto width_at = [&](int k) {
+    const int w = (m - k <= sw) ? tail : nb;
+    return mn - k < w ? mn - k : w;
+  };
   // panel 0, after everything already queued on the main stream
   {
-    const int b = mn < nb ? mn : nb;
+    const int b = width_at(0);
     if ((rc = lu_panel(sP, a, m, b, lda, piv, pairs, &nleaves[0], gcap))) return rc;
     if ((rc = k_panel_compose(sP, pairs, nleaves[0], comp))) return rc;
     PLA_HIP(hipEventRecord(ev_panel, sP));
   }
-  for (int k = 0; k < mn; k += nb) {
-    const int b = mn - k < nb ? mn - k : nb;
-    const int par = (k / nb) & 1;
+  int step = 0;
+  for (int k = 0, b = 0; k < mn; k += b, step++) {
+    b = width_at(k);
+    const int par = step & 1;
     double *Akk = a + (long)k * lda + k;
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
     const int nr = n - k - b, mr = m - k - b;
@@ -105,23 +115,28 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
       PLA_HIP(hipStreamWaitEvent(sL, ev_panel, 0));
       if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
       PLA_HIP(hipEventRecord(ev_left, sL));
-      if ((rc = k_panel_swap(sM, a + k, lda, 0, k + b, n, cp))) return rc;
-    } else if ((rc = k_panel_swap(sM, a + k, lda, k, k + b, n, cp))) {
-      return rc;
     }
-    if (nr <= 0) continue;
-    double *A12 = a + (long)(k + b) * lda + k;
-    if ((rc = trsm_left(sM, true, false, true, b, nr, Akk, lda, A12, lda, nullptr))) return rc;
-    if (mr <= 0) continue;
+    // Right of the panel, the next panel's columns go first (interchanges,
+    // A12 solve, update): they gate panel k+1.  The other columns follow
+    // once panel k+1 has been released.
     const int kn = k + b;
-    const int bn = kn < mn ? (mn - kn < nb ? mn - kn : nb) : 0;
-    int done = 0;  // trailing columns already updated
-    if (bn > 0) {
+    const int bn = (kn < mn && mr > 0) ? width_at(kn) : 0;  // next panel's width
+    const int n1 = bn > 0 ? bn : nr;
+    double *A12 = a + (long)kn * lda + k;
+    auto right_part = [&](int c0, int nc) -> int {  // columns kn + c0 .. kn + c0 + nc
+      if (nc <= 0) return PLA_SUCCESS;
+      int r;
+      if ((r = k_panel_swap(sM, a + k, lda, 0, kn + c0, kn + c0 + nc, cp))) return r;
+      double *A12c = A12 + (long)c0 * lda;
+      if ((r = trsm_left(sM, true, false, true, b, nc, Akk, lda, A12c, lda, nullptr))) return r;
+      if (mr <= 0) return PLA_SUCCESS;
       timing_begin(sM, 1);
-      rc = k_gemm(sM, 1, false, false, mr, bn, b, -1.0, Akk + b, lda, A12, lda, 1.0, A12 + b, lda, nullptr);
-      timing_end(sM, 1, 2.0 * mr * (double)bn * b);
-      if (rc) return rc;
-      done = bn;
+      r = k_gemm(sM, 1, false, false, mr, nc, b, -1.0, Akk + b, lda, A12c, lda, 1.0, A12c + b, lda, nullptr);
+      timing_end(sM, 1, 2.0 * mr * (double)nc * b);
+      return r;
+    };
+    if ((rc = right_part(0, n1))) return rc;
+    if (bn > 0) {
       PLA_HIP(hipEventRecord(ev_ready, sM));
       PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
       double *An = a + (long)kn * lda + kn;
@@ -134,13 +149,9 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
         return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
-    if (nr - done > 0) {
-      timing_begin(sM, 1);
-      rc = k_gemm(sM, 1, false, false, mr, nr - done, b, -1.0, Akk + b, lda, A12 + (long)done * lda, lda,
-                  1.0, A12 + (long)done * lda + b, lda, nullptr);
-      timing_end(sM, 1, 2.0 * mr * (double)(nr - done) * b);
-      if (rc) return rc;
-    }
+    if ((rc = right_part(n1, nr - n1))) return rc;
+    // left of the panel last: nothing reads L before the factorization ends
+    if (!split_left && (rc = k_panel_swap(sM, a + k, lda, k, 0, 0, cp))) return rc;
   }
   return PLA_SUCCESS;
 }

@@ -95,7 +95,7 @@ int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, i
   // completes (E4M3 unit roundoff 2^-4)
   double shift = 0.0;
   int info = 0;
-  for (int attempt = 0;; attempt++) {
+  for (;;) {
     if ((rc = k_cvt_lower_shift(s, n, a, lda, shift, W, ldw))) return rc;
     if ((rc = chol_factor_fp8(s, n, W, ldw, &info))) return rc;
     if (info == 0) break;

@@ -28,7 +28,9 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   const int esz = 2;   // bytes of one FP16 / BF16 element
   const bool split = prec_split(prec);
   const int stride = leaf_pairs_stride();
-  double *P64 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_P64);
+  // FP64 panel copies, double buffered by panel parity: step k's solves read
+  // L11 from panel k's copy while panel k+1 is factored into the other
+  double *P64 = (double *)workspace(sizeof(double) * 2 * (size_t)n * nb, WS_P64);
   double *U64 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_U64);
   char *Lh = (char *)workspace((size_t)2 * n * nbp * esz, WS_LH);
   char *Ll = split ? (char *)workspace((size_t)2 * n * nbp * esz, WS_LL) : nullptr;
@@ -55,11 +57,20 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   int gcap;
   if ((rc = lu_streams_begin(sMain, n, &sP, &sM, &gcap))) return rc;
   const bool split_left = (sMain != sM);
+  int tail, sw;
+  nb_tail(nb, &tail, &sw);
+  tail = (tail / 32) * 32 > 0 ? (tail / 32) * 32 : nb;
+  auto width_at = [&](int k) {
+    const int w = (n - k <= sw) ? tail : nb;
+    return n - k < w ? n - k : w;
+  };
   auto steps = [&]() -> int {
     // panel at column k, on the look-ahead stream
+    auto P64_of = [&](int par) { return P64 + (size_t)par * n * nb; };
     auto panel = [&](int k, int b, int par) -> int {
       const int mp = n - k;
       float *Wk = W + (long)k * ldw + k;
+      double *P64 = P64_of(par);
       int r;
       if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
       if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + par * pstride, &nleaves[par], gcap))) return r;
@@ -70,54 +81,58 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
     };
     {
-      const int b = n < nb ? n : nb;
+      const int b = width_at(0);
       if ((rc = panel(0, b, 0))) return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
-    for (int k = 0; k < n; k += nb) {
-      const int b = n - k < nb ? n - k : nb;
-      const int par = (k / nb) & 1;
+    int step = 0;
+    for (int k = 0, b = 0; k < n; k += b, step++) {
+      b = width_at(k);
+      const int par = step & 1;
+      const int kpad = (b + 31) / 32 * 32;  // the update's inner dimension (copies are zero beyond b)
       PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
       const int *cp = comp + (size_t)par * cstride;
       if (split_left) {
         PLA_HIP(hipStreamWaitEvent(sMain, ev_panel, 0));
         if ((rc = k_panel_swap_f32(sMain, W + k, ldw, k, 0, 0, cp))) return rc;
         PLA_HIP(hipEventRecord(ev_left, sMain));
-        if ((rc = k_panel_swap_f32(sM, W + k, ldw, 0, k + b, n, cp))) return rc;
-      } else if ((rc = k_panel_swap_f32(sM, W + k, ldw, k, k + b, n, cp))) {
-        return rc;
       }
-      const int nr = n - k - b;
-      if (nr <= 0) continue;
-      float *W12 = W + (long)(k + b) * ldw + k;
-      // U12 = inv(L11) W12 in FP64 (L11: top of the FP64 panel, still in P64)
-      if ((rc = k_cvt_f32_f64(sM, b, nr, W12, ldw, U64, b))) return rc;
-      if ((rc = trsm_left(sM, true, false, true, b, nr, P64, n - k, U64, b, nullptr))) return rc;
-      if ((rc = k_upanel_out(sM, prec, b, nr, U64, b, W12, ldw, Uh, split ? Ul : nullptr, nbp))) return rc;
-      const int mr = nr;
+      const int nr = n - k - b, mr = nr;
+      const int kn = k + b;
+      const int bn = kn < n ? width_at(kn) : 0;  // next panel's width
+      const int n1 = bn > 0 ? bn : nr;
+      float *W12 = W + (long)kn * ldw + k;
       const void *L_h = Lh + par * lbuf;
       const void *L_l = split ? Ll + par * lbuf : nullptr;
-      const int kn = k + b;
-      const int bn = kn < n ? (n - kn < nb ? n - kn : nb) : 0;
-      int done = 0;
-      if (bn > 0) {
+      // columns kn + c0 .. kn + c0 + nc: interchanges, U12 = inv(L11) W12 in
+      // FP64 (L11: top of panel k's FP64 copy), the low-precision U12 copy,
+      // then the update.  The next panel's columns go first: they gate panel
+      // k+1; the rest follow once it has been released.
+      auto right_part = [&](int c0, int nc) -> int {
+        if (nc <= 0) return PLA_SUCCESS;
+        int r;
+        if ((r = k_panel_swap_f32(sM, W + k, ldw, 0, kn + c0, kn + c0 + nc, cp))) return r;
+        float *W12c = W12 + (long)c0 * ldw;
+        double *U64c = U64 + (size_t)c0 * b;
+        char *Uhc = Uh + (size_t)c0 * nbp * esz, *Ulc = split ? Ul + (size_t)c0 * nbp * esz : nullptr;
+        if ((r = k_cvt_f32_f64(sM, b, nc, W12c, ldw, U64c, b))) return r;
+        if ((r = trsm_left(sM, true, false, true, b, nc, P64_of(par), n - k, U64c, b, nullptr))) return r;
+        if ((r = k_upanel_out(sM, prec, b, nc, U64c, b, W12c, ldw, Uhc, Ulc, nbp))) return r;
         timing_begin(sM, 3);
-        rc = k_hgemm(sM, prec, mr, bn, nbp, L_h, L_l, nbp, Uh, split ? Ul : nullptr, nbp, W12 + b, ldw);
-        timing_end(sM, 3, 2.0 * mr * (double)bn * b);
-        if (rc) return rc;
-        done = bn;
+        r = k_hgemm(sM, prec, mr, nc, kpad, L_h, L_l, nbp, Uhc, Ulc, nbp, W12c + b, ldw);
+        timing_end(sM, 3, 2.0 * mr * (double)nc * b);
+        return r;
+      };
+      if ((rc = right_part(0, n1))) return rc;
+      if (bn > 0) {
         PLA_HIP(hipEventRecord(ev_ready, sM));
         PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
         if ((rc = panel(kn, bn, par ^ 1))) return rc;
         PLA_HIP(hipEventRecord(ev_panel, sP));
       }
-      if (nr - done > 0) {
-        timing_begin(sM, 3);
-        rc = k_hgemm(sM, prec, mr, nr - done, nbp, L_h, L_l, nbp, Uh + (size_t)done * nbp * esz,
-                     split ? Ul + (size_t)done * nbp * esz : nullptr, nbp, W12 + (long)done * ldw + b, ldw);
-        timing_end(sM, 3, 2.0 * mr * (double)(nr - done) * b);
-        if (rc) return rc;
-      }
+      if ((rc = right_part(n1, nr - n1))) return rc;
+      // left of the panel last: nothing reads L before the factorization ends
+      if (!split_left && (rc = k_panel_swap_f32(sM, W + k, ldw, k, 0, 0, cp))) return rc;
     }
     return PLA_SUCCESS;
   };

@@ -233,6 +233,10 @@ def test_panel_param_errors(pla_session):
     for k in (-1, 2):
         with pytest.raises(pla.PLAError):
             pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, k)
+    with pytest.raises(pla.PLAError):
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_TAIL, -1)
+    with pytest.raises(pla.PLAError):
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_SWITCH, -1)
 
 
 @pytest.mark.parametrize("kind", [0, 1])
@@ -257,3 +261,22 @@ def test_lu_exchange_timeout_reports_error(pla_session, ora, kind):
     lu, piv = _lu_gpu(s, n=1000, seed=3, nb=128)
     lu_o, piv_o = ora.lu_blocked(ora.uniform(1000, 1000, 3), nb=128)
     assert np.array_equal(piv, piv_o)
+
+
+@pytest.mark.parametrize("m,n,nb,tail,switch", [(3000, 3000, 512, 128, 1500), (2500, 1100, 1024, 256, 2000),
+                                                (2000, 2000, 256, 64, 100000)])
+def test_lu_block_size_schedule(pla_session, ora, m, n, nb, tail, switch):
+    """A smaller block size for the last steps (PLA_PARAM_NB_TAIL once at most
+    PLA_PARAM_NB_SWITCH rows remain) gives the oracle's pivots and factors."""
+    a = ora.uniform(m, n, seed=11)
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_TAIL, tail)
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_SWITCH, switch)
+    try:
+        lu, piv = _lu_gpu(pla_session, a=a, nb=nb)
+    finally:
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_TAIL, 0)
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_SWITCH, 0)
+    lu_o, piv_o = ora.lu_blocked(a, nb=nb)
+    assert np.array_equal(piv, piv_o)
+    assert np.abs(lu - lu_o).max() <= 1e-7 * np.abs(a).max()
+    assert ora.lu_scaled_residual(a, lu, piv) < 16

@@ -97,3 +97,22 @@ def test_argument_errors(pla_session):
     B1, X1 = s.matrix(8, 1), s.matrix(8, 1)
     assert pla.lib().PLA_LU_mixed(A, P, B1, X1, 9, 1e-14, 5, None, None) < 0  # unknown precision
     pla.free(A, B, X, P, B1, X1)
+
+
+def test_block_size_schedule_reaches_fp64_backward_error(pla_session, ora):
+    """A smaller block size for the last steps (PLA_PARAM_NB_TAIL once at most
+    PLA_PARAM_NB_SWITCH rows remain; 200 is no multiple of the 32-deep
+    low-precision product, so its K is padded) still refines to FP64."""
+    n = 3000
+    a = ora.uniform(n, n, seed=5)
+    b = np.random.default_rng(5).uniform(-1, 1, (n, 2))
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_TAIL, 200)
+    pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_SWITCH, 1800)
+    try:
+        x, piv, it, be = _mixed(pla_session, a, b, pla.PLA_MIXED_FP16X3, nb=512)
+    finally:
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_TAIL, 0)
+        pla.call("PLA_Set_device_param", pla.PLA_PARAM_NB_SWITCH, 0)
+    assert be <= 1e-14 and it <= 8, (it, be)
+    assert _eta(a, x, b) <= 2e-14
+    assert piv.min() >= 0 and (piv + np.arange(n)).max() < n
