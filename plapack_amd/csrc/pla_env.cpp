@@ -52,6 +52,7 @@ int ensure_device() {
   c.cu_count = cus > 0 ? cus : 256;
   PLA_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   PLA_HIP(hipStreamCreateWithFlags(&c.stream_la, hipStreamNonBlocking));
+  PLA_HIP(hipStreamCreateWithFlags(&c.stream_tr, hipStreamNonBlocking));
   return PLA_SUCCESS;
 }
 
@@ -138,7 +139,7 @@ int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, 
     *gcap = k;
   } else {
     *sP = c.stream_la;
-    *sT = main;
+    *sT = c.stream_tr;
   }
   PLA_HIP(hipEventRecord(g_ev_join[0], main));
   PLA_HIP(hipStreamWaitEvent(*sP, g_ev_join[0], 0));
@@ -242,14 +243,17 @@ int choose_nb(int n, int kind) {
   return 128;
 }
 
-void nb_tail(int nb, int *tail, int *rows) {
+int lu_step_width(int m, int mn, int nb, int k) {
   const context &c = g_ctx;
-  *tail = nb;
-  *rows = 0;
-  if (c.nb_tail > 0) {  // PLA_PARAM_NB_TAIL / PLA_PARAM_NB_SWITCH
-    *tail = c.nb_tail < nb ? c.nb_tail : nb;
-    *rows = c.nb_switch;
-  }
+  int w = nb;
+  if (c.nb_tail > 0 && m - k <= c.nb_switch) w = c.nb_tail < nb ? c.nb_tail : nb;  // PLA_PARAM_NB_TAIL / _SWITCH
+  return mn - k < w ? mn - k : w;
+}
+
+int lu_step_count(int m, int mn, int nb) {
+  int steps = 0;
+  for (int k = 0; k < mn; k += lu_step_width(m, mn, nb, k)) steps++;
+  return steps > 0 ? steps : 1;
 }
 
 }  // namespace pla
@@ -311,6 +315,7 @@ int PLA_Finalize(void) {
   pla::context &c = pla::ctx();
   if (c.stream) (void)hipStreamSynchronize(c.stream);
   if (c.stream_la) (void)hipStreamSynchronize(c.stream_la);
+  if (c.stream_tr) (void)hipStreamSynchronize(c.stream_tr);
   if (c.stream_panel) (void)hipStreamSynchronize(c.stream_panel);
   if (c.stream_trail) (void)hipStreamSynchronize(c.stream_trail);
   pla::release_workspaces();

@@ -64,6 +64,7 @@ struct context {
   int device = 0;
   hipStream_t stream = nullptr;      // main stream (default priority)
   hipStream_t stream_la = nullptr;   // look-ahead stream (default priority)
+  hipStream_t stream_tr = nullptr;   // trailing-update stream of the factorizations (default priority)
   int nb_user = 0;                   // set by pla_Environ_set_nb_alg (0 = not set)
   int nb_device = 0;                 // PLA_Set_device_nb (0 = built-in choice)
   int cu_count = 256;
@@ -85,7 +86,8 @@ context &ctx();
 int ensure_device();                 // lazily initialise device + streams
 
 // Streams of a blocked LU with look-ahead.  *sP receives the panel
-// factorizations, *sT the interchanges and trailing updates.  With panel CUs
+// factorizations, *sT the interchanges right of the panel and the trailing
+// updates; `main` keeps the interchanges left of the panel.  With panel CUs
 // reserved (PLA_PARAM_PANEL_CUS) they are a CU-masked pair that never share a
 // CU, so the panel leaf's workgroups (which must all be resident at once)
 // never wait for trailing-update workgroups to drain; *gcap is then the most
@@ -125,9 +127,11 @@ int ld(PLA_Obj o);
 // Block size a factorization of kind NB_* uses for an n x n problem.
 enum { NB_LU = PLA_NB_LU, NB_CHOL = PLA_NB_CHOL, NB_LU_MIXED = PLA_NB_LU_MIXED };
 int choose_nb(int n, int kind);
-// Block size of the last steps of an LU with block size nb: *tail once at
-// most *rows rows remain (*tail == nb: no change).
-void nb_tail(int nb, int *tail, int *rows);
+// Width of the LU step at column k of an m-row factorization (mn = min(m, n))
+// with block size nb: nb, or PLA_PARAM_NB_TAIL once at most
+// PLA_PARAM_NB_SWITCH rows remain; and the number of steps.
+int lu_step_width(int m, int mn, int nb, int k);
+int lu_step_count(int m, int mn, int nb);
 
 // Device scalar (1x1 object) read back to the host.
 int scalar_value(PLA_Obj o, double *v);

@@ -33,9 +33,16 @@ static int check_lu_args(PLA_Obj A, PLA_Obj pivots) {
 
 namespace pla {
 
+struct lu_bufs {
+  int *pairs;      // leaves' pair lists, two parities of pstride
+  size_t pstride;
+  int *comp;       // composite interchange list of each step, cstride apart
+  int cstride;
+};
+
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
-                           long lda, int *piv, int nb, int *pairs, size_t pstride, int *comp, int cstride,
-                           hipEvent_t ev_ready, hipEvent_t ev_panel, hipEvent_t ev_left);
+                           long lda, int *piv, int nb, const lu_bufs &bf, hipEvent_t ev_ready,
+                           hipEvent_t ev_panel);
 
 // Blocked right-looking LU of the m x n column-major device matrix a, with
 // one step of look-ahead: panel k+1 is factored on the look-ahead stream as
@@ -53,13 +60,15 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   const int cstride = panel_comp_stride();
   const size_t pstride = (size_t)LU_MAX_LEAVES * stride;  // one parity of the leaves' pair lists
   int *pairs = (int *)workspace(sizeof(int) * 2 * pstride, 3, true);
-  int *comp = (int *)workspace(sizeof(int) * 2 * cstride, 4);
+  // one composite interchange list per step: the left-of-panel interchanges
+  // of a step may still be reading theirs while later panels are composed
+  int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(m, m < n ? m : n, nb) * cstride, 4);
   if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr, ev_left = nullptr;
+  const lu_bufs bf{pairs, pstride, comp, cstride};
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
   }
   int rc;
   // sP: panels; sM: interchanges right of the panel, A12 solves and trailing
@@ -67,8 +76,7 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
-  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, pairs, pstride, comp, cstride, ev_ready, ev_panel,
-                       ev_left);
+  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, bf, ev_ready, ev_panel);
   // the main stream waits for both streams, also after an error
   const int rc_end = lu_streams_end(sMain, sP, sM);
   if (rc) return rc;
@@ -77,12 +85,23 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
 }
 
 // The interchanges of panel k on the columns left of it (L, which nothing
-// reads until the factorization ends) go to sL when it differs from sM, else
-// to the end of step k on sM; the compose of panel k+2 into the same
-// double-buffered list waits for them (ev_left, or stream order).
+// reads until the factorization ends) go to sL when it differs from sM, off
+// the trailing-update stream, else to the end of step k on sM.  Step k's
+// composite list is comp + k * cstride.
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
-                           long lda, int *piv, int nb, int *pairs, size_t pstride, int *comp, int cstride,
-                           hipEvent_t ev_ready, hipEvent_t ev_panel, hipEvent_t ev_left) {
+                           long lda, int *piv, int nb, const lu_bufs &bf, hipEvent_t ev_ready,
+                           hipEvent_t ev_panel) {
+  int *const pairs = bf.pairs, *const comp = bf.comp;
+  const size_t pstride = bf.pstride;
+  const int cstride = bf.cstride;
+  // panel at column k, b columns, leaves' lists of parity par: factor, then
+  // compose its interchanges into list `step`
+  auto panel = [&](int k, int b, int par, int step, int *nl) -> int {
+    int r;
+    if ((r = lu_panel(sP, a + (long)k * lda + k, m - k, b, lda, piv + k, pairs + par * pstride, nl, gcap)))
+      return r;
+    return k_panel_compose(sP, pairs + par * pstride, *nl, comp + (size_t)step * cstride);
+  };
   const bool split_left = (sL != sM);
   const int mn = m < n ? m : n;
   int rc;
@@ -90,17 +109,11 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
   // block size of the step at column k: nb, then the tail block size once at
   // most nb_switch rows remain (a narrower panel shortens the critical path
   // when the trailing update no longer hides it)
-  int tail, sw;
-  nb_tail(nb, &tail, &sw);
-  auto width_at = [&](int k) {
-    const int w = (m - k <= sw) ? tail : nb;
-    return mn - k < w ? mn - k : w;
-  };
+  auto width_at = [&](int k) { return lu_step_width(m, mn, nb, k); };
   // panel 0, after everything already queued on the main stream
   {
     const int b = width_at(0);
-    if ((rc = lu_panel(sP, a, m, b, lda, piv, pairs, &nleaves[0], gcap))) return rc;
-    if ((rc = k_panel_compose(sP, pairs, nleaves[0], comp))) return rc;
+    if ((rc = panel(0, b, 0, 0, &nleaves[0]))) return rc;
     PLA_HIP(hipEventRecord(ev_panel, sP));
   }
   int step = 0;
@@ -110,11 +123,10 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     double *Akk = a + (long)k * lda + k;
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
     const int nr = n - k - b, mr = m - k - b;
-    const int *cp = comp + (size_t)par * cstride;
+    const int *cp = comp + (size_t)step * cstride;
     if (split_left) {
       PLA_HIP(hipStreamWaitEvent(sL, ev_panel, 0));
       if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
-      PLA_HIP(hipEventRecord(ev_left, sL));
     }
     // Right of the panel, the next panel's columns go first (interchanges,
     // A12 solve, update): they gate panel k+1.  The other columns follow
@@ -139,14 +151,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     if (bn > 0) {
       PLA_HIP(hipEventRecord(ev_ready, sM));
       PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
-      double *An = a + (long)kn * lda + kn;
-      if ((rc = lu_panel(sP, An, m - kn, bn, lda, piv + kn, pairs + (par ^ 1) * pstride, &nleaves[par ^ 1],
-                         gcap)))
-        return rc;
-      if (split_left) PLA_HIP(hipStreamWaitEvent(sP, ev_left, 0));  // step k-1 read this list
-      if ((rc = k_panel_compose(sP, pairs + (par ^ 1) * pstride, nleaves[par ^ 1],
-                                comp + (size_t)(par ^ 1) * cstride)))
-        return rc;
+      if ((rc = panel(kn, bn, par ^ 1, step + 1, &nleaves[par ^ 1]))) return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
     if ((rc = right_part(n1, nr - n1))) return rc;

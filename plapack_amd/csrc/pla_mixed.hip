@@ -193,23 +193,28 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
 }
 
 // ---------------------------------------------------------------------------
-// Precision conversions.
-__global__ __launch_bounds__(256) void cvt_f64_f32_kernel(int m, int n, const double *__restrict__ A, long lda,
-                                                          float *__restrict__ W, long ldw) {
-  const long total = (long)m * n;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const int j = (int)(e / m), i = (int)(e - (long)j * m);
-    W[(long)j * ldw + i] = (float)A[(long)j * lda + i];
+// Precision conversions of m x n column-major blocks: grid y walks the
+// columns, grid x the rows, four consecutive rows per thread (coalesced, no
+// index division).
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void cvt2d_kernel(int m, int n, const S *__restrict__ A, long lda,
+                                                    D *__restrict__ B, long ldb) {
+  for (int j = blockIdx.y; j < n; j += gridDim.y) {
+    const S *a = A + (long)j * lda;
+    D *b = B + (long)j * ldb;
+    for (int i0 = (blockIdx.x * 256 + threadIdx.x) * 4; i0 < m; i0 += gridDim.x * 1024) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (i0 + r < m) b[i0 + r] = (D)a[i0 + r];
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void cvt_f32_f64_kernel(int m, int n, const float *__restrict__ W, long ldw,
-                                                          double *__restrict__ P, long ldp) {
-  const long total = (long)m * n;
-  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const int j = (int)(e / m), i = (int)(e - (long)j * m);
-    P[(long)j * ldp + i] = (double)W[(long)j * ldw + i];
-  }
+template <typename S, typename D>
+static void launch_cvt2d(hipStream_t s, int m, int n, const S *A, long lda, D *B, long ldb) {
+  const int gx = (m + 1023) / 1024 < 64 ? (m + 1023) / 1024 : 64;
+  const int gy = n < 65535 ? n : 65535;
+  hipLaunchKernelGGL((cvt2d_kernel<S, D>), dim3(gx, gy), dim3(256), 0, s, m, n, A, lda, B, ldb);
 }
 
 template <typename H, bool SPLIT>
@@ -569,14 +574,14 @@ int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const 
 
 int k_cvt_f64_f32(hipStream_t s, int m, int n, const double *A, long lda, float *W, long ldw) {
   if (m <= 0 || n <= 0) return PLA_SUCCESS;
-  hipLaunchKernelGGL(cvt_f64_f32_kernel, dim3(grid1((long)m * n)), dim3(256), 0, s, m, n, A, lda, W, ldw);
+  launch_cvt2d(s, m, n, A, lda, W, ldw);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
 
 int k_cvt_f32_f64(hipStream_t s, int m, int n, const float *W, long ldw, double *P, long ldp) {
   if (m <= 0 || n <= 0) return PLA_SUCCESS;
-  hipLaunchKernelGGL(cvt_f32_f64_kernel, dim3(grid1((long)m * n)), dim3(256), 0, s, m, n, W, ldw, P, ldp);
+  launch_cvt2d(s, m, n, W, ldw, P, ldp);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }

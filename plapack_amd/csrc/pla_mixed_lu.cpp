@@ -39,50 +39,44 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   const size_t pstride = (size_t)LU_MAX_LEAVES * stride;  // one parity of the leaves' pair lists
   int *pairs = (int *)workspace(sizeof(int) * 2 * pstride, WS_PAIRS, true);
   const int cstride = panel_comp_stride();
-  int *comp = (int *)workspace(sizeof(int) * 2 * cstride, WS_COMP);
+  // one composite interchange list per step (the left-of-panel interchanges
+  // of a step may still be reading theirs while later panels are composed)
+  int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(n, n, nb) * cstride, WS_COMP);
   if (!P64 || !U64 || !Lh || !Uh || !pairs || !comp || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
   const size_t lbuf = (size_t)n * nbp * esz;  // one parity of the L copies
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr, ev_left = nullptr;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
   }
   int nleaves[2] = {0, 0};
   int rc;
   // sP: panels; sM: interchanges right of the panel, U12 solves and trailing
   // updates; sMain: interchanges left of the panel (nothing reads L until the
-  // factorization ends), ordered before the compose that reuses their list
+  // factorization ends)
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, n, &sP, &sM, &gcap))) return rc;
   const bool split_left = (sMain != sM);
-  int tail, sw;
-  nb_tail(nb, &tail, &sw);
-  tail = (tail / 32) * 32 > 0 ? (tail / 32) * 32 : nb;
-  auto width_at = [&](int k) {
-    const int w = (n - k <= sw) ? tail : nb;
-    return n - k < w ? n - k : w;
-  };
+  auto width_at = [&](int k) { return lu_step_width(n, n, nb, k); };
   auto steps = [&]() -> int {
     // panel at column k, on the look-ahead stream
     auto P64_of = [&](int par) { return P64 + (size_t)par * n * nb; };
-    auto panel = [&](int k, int b, int par) -> int {
+    auto panel = [&](int k, int b, int par, int step) -> int {
       const int mp = n - k;
       float *Wk = W + (long)k * ldw + k;
       double *P64 = P64_of(par);
       int r;
       if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
       if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + par * pstride, &nleaves[par], gcap))) return r;
-      if (split_left && k > 0) PLA_HIP(hipStreamWaitEvent(sP, ev_left, 0));  // step k-2 read this list
-      if ((r = k_panel_compose(sP, pairs + par * pstride, nleaves[par], comp + (size_t)par * cstride)))
+      if ((r = k_panel_compose(sP, pairs + par * pstride, nleaves[par], comp + (size_t)step * cstride)))
         return r;
       if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
       return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
     };
     {
       const int b = width_at(0);
-      if ((rc = panel(0, b, 0))) return rc;
+      if ((rc = panel(0, b, 0, 0))) return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
     int step = 0;
@@ -91,11 +85,10 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       const int par = step & 1;
       const int kpad = (b + 31) / 32 * 32;  // the update's inner dimension (copies are zero beyond b)
       PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
-      const int *cp = comp + (size_t)par * cstride;
+      const int *cp = comp + (size_t)step * cstride;
       if (split_left) {
         PLA_HIP(hipStreamWaitEvent(sMain, ev_panel, 0));
         if ((rc = k_panel_swap_f32(sMain, W + k, ldw, k, 0, 0, cp))) return rc;
-        PLA_HIP(hipEventRecord(ev_left, sMain));
       }
       const int nr = n - k - b, mr = nr;
       const int kn = k + b;
@@ -127,7 +120,7 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       if (bn > 0) {
         PLA_HIP(hipEventRecord(ev_ready, sM));
         PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
-        if ((rc = panel(kn, bn, par ^ 1))) return rc;
+        if ((rc = panel(kn, bn, par ^ 1, step + 1))) return rc;
         PLA_HIP(hipEventRecord(ev_panel, sP));
       }
       if ((rc = right_part(n1, nr - n1))) return rc;
