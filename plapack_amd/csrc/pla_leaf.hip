@@ -22,10 +22,12 @@
 //   1. candidate: the largest |L(r, j)| over rows at positions >= j, ties to
 //      the smallest position (idamax's first maximum, and the tie rule of
 //      PLA_Iamax.c:108-115), reduced over the wave;
-//   2. publish one record: (position, value) and the candidate row;
+//   2. publish (position, value) in three dense header arrays (one word per
+//      workgroup each, so a poll of all workgroups reads 3 G contiguous
+//      words) and the candidate row, columns j.. only, stored straight from
+//      the lane that holds it;
 //   3. read every workgroup's (position, value), pick the same winner, then
-//      read the winner's row from its record (already visible: it was stored
-//      with the header);
+//      read the winner's row;
 //   4. relabel (pivot row -> j, the row at j -> p), scale by the reciprocal
 //      of the pivot, rank-1 update; then the candidate of column j+1 is
 //      published (its row must carry this step's update in every column).
@@ -48,8 +50,9 @@
 namespace {
 
 constexpr int RL_MAXG = 512;     // workgroups (waves) of one leaf
-constexpr int RL_REC = 72;       // words per record: 4 header words, 2 W row words, padding
 constexpr int RL_MAXW = 32;
+constexpr int RL_ROWW = 2 * RL_MAXW;                    // words of one candidate row
+constexpr int RL_PAR = 3 * RL_MAXG + RL_MAXG * RL_ROWW;  // words of one column parity
 constexpr int RL_PAIRS = 2 * RL_MAXW;
 constexpr int RL_PAIRS_STRIDE = 1 + 2 * RL_PAIRS;  // count, then (dst, src) pairs
 
@@ -61,7 +64,7 @@ struct rleaf_args {
   int pw, pc0;              // panel width, leaf's first column inside the panel
   int *piv;                 // w entries: pivot offset relative to row j
   int *pairs;               // RL_PAIRS_STRIDE ints, positions relative to the leaf's first row + pc0
-  unsigned long long *slots;  // [2][RL_MAXG][RL_REC]
+  unsigned long long *slots;  // per column parity: headers [3][RL_MAXG], rows [RL_MAXG][RL_ROWW]
   int *err;
   unsigned tag0, spin_limit;
 };
@@ -151,8 +154,9 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
   unsigned long long *slots = a.slots;
 
   // candidate of column j over this lane's rows, reduced over the wave,
-  // then published with the candidate row
-  auto publish = [&](int j) {
+  // then published with the candidate row (j is a compile-time constant)
+  auto publish = [&](auto J) {
+    constexpr int j = decltype(J)::value;
     double bv = -1.0;
     int bp = INT_MAX, bx = -1;
 #pragma unroll
@@ -167,38 +171,34 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
     int who = bx >= 0 ? lane * RPL + bx : -1;  // lane and row slot of the candidate
     wave_argmax(bv, bp, who);
     const unsigned long long tag = (unsigned long long)(a.tag0 + (unsigned)j) << 32;
-    unsigned long long *rec = slots + ((size_t)(j & 1) * RL_MAXG + g) * RL_REC;
-    // the candidate row, gathered from its lane: word 4 + 2c (+1) holds lo (hi) of column c
-    unsigned wlo = 0u, whi = 0u;  // this lane's words `lane` and `lane + 64`
-    const int src_lane = who >= 0 ? who / RPL : 0, src_i = who >= 0 ? who % RPL : 0;
+    unsigned long long *par = slots + (size_t)(j & 1) * RL_PAR;
+    // the candidate row, columns j.. (words 2c, 2c + 1: lo, hi of column c),
+    // stored by the lane that holds it
+    // (the slot is found by its position, one test per slot, so that every
+    // register index stays a constant)
+    if (who >= 0 && lane == who / RPL) {
+      unsigned long long *row = par + 3 * RL_MAXG + (size_t)g * RL_ROWW;
 #pragma unroll
-    for (int c = 0; c < W; c++) {
-      double x = v[0][c];
+      for (int i = 0; i < RPL; i++) {
+        if (pos[i] == bp) {
 #pragma unroll
-      for (int i = 1; i < RPL; i++)
-        if (src_i == i) x = v[i][c];
-      x = __shfl(x, src_lane, 64);
-      if (who < 0) x = 0.0;
-      if (lane == 4 + 2 * c) wlo = lo32(x);
-      if (lane == 5 + 2 * c) wlo = hi32(x);
-      if (4 + 2 * c >= 64 && lane == 4 + 2 * c - 64) whi = lo32(x);
-      if (5 + 2 * c >= 64 && lane == 5 + 2 * c - 64) whi = hi32(x);
+          for (int c = j; c < W; c++) {
+            st_dev(row + 2 * c, tag | lo32(v[i][c]));
+            st_dev(row + 2 * c + 1, tag | hi32(v[i][c]));
+          }
+        }
+      }
     }
-    if (lane == 0) wlo = (unsigned)bp;
-    if (lane == 1) wlo = lo32(bv);
-    if (lane == 2) wlo = hi32(bv);
-    if (lane == 3) wlo = 0u;
-    if (lane < 4 + 2 * W) st_dev(rec + lane, tag | wlo);
-    if (4 + 2 * W > 64 && lane < 4 + 2 * W - 64) st_dev(rec + 64 + lane, tag | whi);
+    if (lane < 3) st_dev(par + lane * RL_MAXG + g, tag | (lane == 0 ? (unsigned)bp : lane == 1 ? lo32(bv) : hi32(bv)));
   };
 
-  publish(0);
+  publish(std::integral_constant<int, 0>{});
   int ph = 0;  // lane j: the pivot position chosen for column j
   static_for<W>([&](auto J) {
     constexpr int j = decltype(J)::value;
     if (j < w) {
       const unsigned tagv = a.tag0 + (unsigned)j;
-      const unsigned long long *recs = slots + (size_t)(j & 1) * RL_MAXG * RL_REC;
+      const unsigned long long *par = slots + (size_t)(j & 1) * RL_PAR;
       // every workgroup's (position, value)
       double gv = -1.0;
       int gp = INT_MAX, gw = -1;
@@ -210,8 +210,8 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
           gp = INT_MAX;
           gw = -1;
           for (int t = lane; t < G; t += 64) {
-            const unsigned long long *rt = recs + (size_t)t * RL_REC;
-            const unsigned long long w0 = ld_dev(rt), w1 = ld_dev(rt + 1), w2 = ld_dev(rt + 2);
+            const unsigned long long w0 = ld_dev(par + t), w1 = ld_dev(par + RL_MAXG + t),
+                                     w2 = ld_dev(par + 2 * RL_MAXG + t);
             if ((unsigned)(w0 >> 32) != tagv || (unsigned)(w1 >> 32) != tagv || (unsigned)(w2 >> 32) != tagv) ok = 0;
             const int ci = (int)(unsigned)w0;
             const double cv = dbl_from((unsigned)w1, (unsigned)w2);
@@ -226,16 +226,16 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
         }
         wave_argmax(gv, gp, gw);
       }
-      // the winner's row (all-NaN column: no winner, no interchange); lane l < 2W
-      // holds record word 4 + l, so column c is in lanes 2c (low) and 2c + 1 (high)
+      // the winner's row (all-NaN column: no winner, no interchange); lane l in
+      // [2j, 2W) holds row word l, so column c is in lanes 2c (low) and 2c + 1 (high)
       unsigned rw = 0u;
       {
-        const unsigned long long *rt = recs + (size_t)(gw >= 0 ? gw : 0) * RL_REC;
+        const unsigned long long *rt = par + 3 * RL_MAXG + (size_t)(gw >= 0 ? gw : 0) * RL_ROWW;
         unsigned it = 0;
         for (;;) {
           int ok = 1;
-          if (gw >= 0 && lane < 2 * W) {
-            const unsigned long long x = ld_dev(rt + 4 + lane);
+          if (gw >= 0 && lane >= 2 * j && lane < 2 * W) {
+            const unsigned long long x = ld_dev(rt + lane);
             if ((unsigned)(x >> 32) != tagv) ok = 0;
             rw = (unsigned)x;
           }
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
         }
       }
       if constexpr (j + 1 < W) {
-        if (j + 1 < w) publish(j + 1);
+        if (j + 1 < w) publish(std::integral_constant<int, j + 1>{});
       }
     }
   });
@@ -376,7 +376,7 @@ int k_lu_rleaf(hipStream_t s, const lu_leaf_call &c, int W) {
   const int G = (c.mp + R - 1) / R;
   if (G > RL_MAXG || (c.gcap > 0 && G > c.gcap))
     return fail(PLA_ERR_DIMENSIONS, "lu leaf: %d rows need %d waves of width %d", c.mp, G, W);
-  const size_t slot_bytes = (size_t)2 * RL_MAXG * RL_REC * sizeof(unsigned long long);
+  const size_t slot_bytes = (size_t)2 * RL_PAR * sizeof(unsigned long long);
   leaf_ws &ws = leaf_workspace(slot_bytes);
   if (!ws.base) return PLA_ERR_NO_MEMORY;
   unsigned tag0;

@@ -427,6 +427,134 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
   }
 }
 
+// Untransposed solves (M = A, column-major): the chain of dependent blocks is
+// the cost, so everything a block needs apart from its predecessors' solution
+// blocks is fetched when it starts: the inverted diagonal block (thread t
+// keeps row t % 64, columns 16 (t / 64) .. +15) and the right-hand side, both
+// pinned in registers.  Thread t owns row t % 64 and columns 16 (t / 64) ..
+// +15 of every 64 x 64 tile, read straight from A (coalesced: 64 rows of one
+// column per wave), one tile ahead of its use.  Per tile only the solution
+// block comes through LDS.
+template <typename T, bool LOWER, int NR>
+__global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv_nt_kernel(int n, int nrhs, const T *__restrict__ A, long lda,
+                                                      const double *__restrict__ Dinv, double *__restrict__ X,
+                                                      long ldx, unsigned long long *__restrict__ xs, unsigned tag32,
+                                                      unsigned spin_limit, int *__restrict__ err) {
+  constexpr int RPT = (TB * NR + 255) / 256;  // right-hand-side entries per thread
+  __shared__ double xb[2][TB][NR];
+  __shared__ double red[4][TB][NR];
+  const int nblk = (n + TB - 1) / TB;
+  const int b = blockIdx.x;
+  const int blk = LOWER ? b : nblk - 1 - b;
+  const int r0 = blk * TB, bs = min(TB, n - r0);
+  const int tid = threadIdx.x, row = tid & 63, cg = tid >> 6;
+  const unsigned long long tag = (unsigned long long)tag32 << 32;
+  double dv[16];
+  {
+    const double *D = Dinv + (size_t)blk * TB * TB;
+#pragma unroll
+    for (int c = 0; c < 16; c++) dv[c] = D[(size_t)(cg * 16 + c) * TB + row];
+  }
+  double rhs[RPT];
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    const int e = tid + 256 * j, i = e / NR, q = e % NR;
+    rhs[j] = (e < TB * NR && i < bs && q < nrhs) ? X[(long)q * ldx + r0 + i] : 0.0;
+  }
+  // wait for them here, while the block has nothing else to do
+#pragma unroll
+  for (int c = 0; c < 16; c++) asm volatile("" : "+v"(dv[c]));
+#pragma unroll
+  for (int j = 0; j < RPT; j++) asm volatile("" : "+v"(rhs[j]));
+  double part[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) part[q] = 0.0;
+  T tr[16];
+  auto load_tile = [&](int s) {
+    const int iblk = LOWER ? s : nblk - 1 - s;
+    const int c0 = iblk * TB, cs = min(TB, n - c0);
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      const int cc = cg * 16 + c;
+      tr[c] = (row < bs && cc < cs) ? A[(long)(c0 + cc) * lda + r0 + row] : (T)0;
+    }
+  };
+  bool failed = false;
+  if (b > 0) load_tile(0);
+  for (int s = 0; s < b; s++) {
+    const int iblk = LOWER ? s : nblk - 1 - s;
+    const int cs = min(TB, n - iblk * TB);
+    const unsigned long long *src = xs + (size_t)iblk * TB * MAXRHS * 2;
+    double (*x)[NR] = xb[s & 1];
+    unsigned it = 0;
+    for (;;) {
+      int ok = 1;
+      for (int e = tid; e < TB * NR; e += 256) {
+        const int c = e / NR, q = e % NR;
+        double v = 0.0;
+        if (c < cs && q < nrhs) {
+          const unsigned long long lo = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2);
+          const unsigned long long hi = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2 + 1);
+          if ((lo & 0xFFFFFFFF00000000ULL) != tag || (hi & 0xFFFFFFFF00000000ULL) != tag) ok = 0;
+          v = __longlong_as_double((long long)(((hi & 0xFFFFFFFFULL) << 32) | (lo & 0xFFFFFFFFULL)));
+        }
+        x[c][q] = v;
+      }
+      if (__syncthreads_and(ok)) break;
+      ++it;
+      const int bail = (it > spin_limit) || (tid == 0 && (it & 63) == 0 &&
+                                               __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (__syncthreads_or(bail)) {
+        failed = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (failed) break;
+    double a[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) a[c] = (double)tr[c];
+    if (s + 1 < b) load_tile(s + 1);
+#pragma unroll
+    for (int c = 0; c < 16; c++)
+#pragma unroll
+      for (int q = 0; q < NR; q++) part[q] = fma(-a[c], x[cg * 16 + c][q], part[q]);
+  }
+  if (failed && tid == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int q = 0; q < NR; q++) red[cg][row][q] = part[q];
+  __syncthreads();
+  // acc = rhs + partials (into xb[b & 1], not read since the last barrier),
+  // then x = inv(M_bb) acc
+  double (*acc)[NR] = xb[b & 1];
+#pragma unroll
+  for (int j = 0; j < RPT; j++) {
+    const int e = tid + 256 * j, i = e / NR, q = e % NR;
+    if (e < TB * NR) acc[i][q] = rhs[j] + ((red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]));
+  }
+  __syncthreads();
+  double y[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) y[q] = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; c++)
+#pragma unroll
+    for (int q = 0; q < NR; q++) y[q] = fma(dv[c], acc[cg * 16 + c][q], y[q]);
+  __syncthreads();  // red consumed
+#pragma unroll
+  for (int q = 0; q < NR; q++) red[cg][row][q] = y[q];
+  __syncthreads();
+  unsigned long long *dst = xs + (size_t)blk * TB * MAXRHS * 2;
+  for (int e = tid; e < bs * nrhs; e += 256) {
+    const int i = e % bs, q = e / bs;
+    const double t = (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
+    X[(long)q * ldx + r0 + i] = t;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
+    st_dev64(dst + ((size_t)i * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
+    st_dev64(dst + ((size_t)i * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Residual R = B - A X (FP64, nrhs <= NR <= 16): per 512-column chunk a plain
 // FMA sum, then the chunks combined in order with compensated (Neumaier)
@@ -632,11 +760,15 @@ static int trsv_launch(hipStream_t s, bool lower, bool trans, int n, int nrhs, c
 #define PLA_TSF(LO, TR)                                                                                      \
   hipLaunchKernelGGL((trsv_sf_kernel<T, LO, TR, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, \
                      ldx, xs, tag, 2000000u, err)
-  if (lower && !trans) PLA_TSF(true, false);
+#define PLA_TNT(LO)                                                                                          \
+  hipLaunchKernelGGL((trsv_nt_kernel<T, LO, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, ldx, \
+                     xs, tag, 2000000u, err)
+  if (lower && !trans) PLA_TNT(true);
   else if (lower && trans) PLA_TSF(true, true);
-  else if (!lower && !trans) PLA_TSF(false, false);
+  else if (!lower && !trans) PLA_TNT(false);
   else PLA_TSF(false, true);
 #undef PLA_TSF
+#undef PLA_TNT
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
