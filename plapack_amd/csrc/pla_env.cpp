@@ -79,7 +79,9 @@ void *workspace(size_t bytes, int slot, bool zero) {
     fail(PLA_ERR_NO_MEMORY, "workspace allocation of %zu bytes failed", want);
     return nullptr;
   }
-  if (zero && hipMemset(w.p, 0, want) != hipSuccess) {
+  // the library's streams do not synchronise with the null stream: finish the
+  // clear before any of them can touch the buffer
+  if (zero && (hipMemset(w.p, 0, want) != hipSuccess || hipDeviceSynchronize() != hipSuccess)) {
     (void)hipFree(w.p);
     w.p = nullptr;
     fail(PLA_ERR_DEVICE, "workspace initialisation failed");
@@ -174,7 +176,7 @@ leaf_ws &leaf_workspace(size_t slot_bytes) {
     fail(PLA_ERR_NO_MEMORY, "panel exchange workspace allocation failed");
     return g_leaf;
   }
-  if (hipMemset(p, 0, total) != hipSuccess) {
+  if (hipMemset(p, 0, total) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
     (void)hipFree(p);
     fail(PLA_ERR_DEVICE, "panel exchange workspace initialisation failed");
     return g_leaf;

@@ -42,7 +42,7 @@ struct lu_bufs {
 
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
                            long lda, int *piv, int nb, const lu_bufs &bf, hipEvent_t ev_ready,
-                           hipEvent_t ev_panel);
+                           hipEvent_t ev_panel, hipEvent_t ev_step);
 
 // Blocked right-looking LU of the m x n column-major device matrix a, with
 // one step of look-ahead: panel k+1 is factored on the look-ahead stream as
@@ -65,10 +65,11 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(m, m < n ? m : n, nb) * cstride, 4);
   if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
   const lu_bufs bf{pairs, pstride, comp, cstride};
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr, ev_step = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
   }
   int rc;
   // sP: panels; sM: interchanges right of the panel, A12 solves and trailing
@@ -76,7 +77,7 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
-  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, bf, ev_ready, ev_panel);
+  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, bf, ev_ready, ev_panel, ev_step);
   // the main stream waits for both streams, also after an error
   const int rc_end = lu_streams_end(sMain, sP, sM);
   if (rc) return rc;
@@ -84,13 +85,14 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   return leaf_check_error(sMain);
 }
 
-// The interchanges of panel k on the columns left of it (L, which nothing
-// reads until the factorization ends) go to sL when it differs from sM, off
-// the trailing-update stream, else to the end of step k on sM.  Step k's
-// composite list is comp + k * cstride.
+// The interchanges of panel k on the columns left of it go to sL when it
+// differs from sM, off the trailing-update stream, else to the end of step k
+// on sM.  Those columns include panel k-1's L21, which step k-1's trailing
+// update reads: sL waits for the end of step k-1 on sM (ev_step) before
+// permuting them.  Step k's composite list is comp + k * cstride.
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
                            long lda, int *piv, int nb, const lu_bufs &bf, hipEvent_t ev_ready,
-                           hipEvent_t ev_panel) {
+                           hipEvent_t ev_panel, hipEvent_t ev_step) {
   int *const pairs = bf.pairs, *const comp = bf.comp;
   const size_t pstride = bf.pstride;
   const int cstride = bf.cstride;
@@ -126,6 +128,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     const int *cp = comp + (size_t)step * cstride;
     if (split_left) {
       PLA_HIP(hipStreamWaitEvent(sL, ev_panel, 0));
+      if (k > 0) PLA_HIP(hipStreamWaitEvent(sL, ev_step, 0));  // step k-1 no longer reads panel k-1's L21
       if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
     }
     // Right of the panel, the next panel's columns go first (interchanges,
@@ -157,6 +160,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     if ((rc = right_part(n1, nr - n1))) return rc;
     // left of the panel last: nothing reads L before the factorization ends
     if (!split_left && (rc = k_panel_swap(sM, a + k, lda, k, 0, 0, cp))) return rc;
+    if (split_left) PLA_HIP(hipEventRecord(ev_step, sM));
   }
   return PLA_SUCCESS;
 }

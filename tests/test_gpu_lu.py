@@ -103,10 +103,13 @@ def test_general_solve_multi_rhs(pla_session, ora):
     pla.free(A, B, P)
 
 
-def test_lu_solve_n16384_backward_error(pla_session):
-    """Full-path property at a size the CPU oracle cannot finish quickly."""
+@pytest.mark.parametrize("n,nb,repeat", [(16384, 256, 1), (65536, 0, 2)])
+def test_lu_solve_large_backward_error(pla_session, n, nb, repeat):
+    """Full-path property at sizes the CPU oracle cannot finish quickly.  At
+    n=65536 (the library's block size, all streams busy at once) the
+    factorization is repeated: the pivots must not change between runs (a
+    stream-ordering race shows up as different pivots and a failed solve)."""
     s = pla_session
-    n = 16384
     A = s.matrix(n, n)
     A0 = s.matrix(n, n)
     pla.call("PLA_Obj_fill_drand48", A, 0, 0)
@@ -117,8 +120,15 @@ def test_lu_solve_n16384_backward_error(pla_session):
     pla.call("PLA_Gemv", pla.PLA_NO_TRANSPOSE, one, A0, ones, zero, b)   # b = A*1
     pla.call("PLA_Copy", b, x)
     P = s.mvector(n, dtype=pla.MPI_INT)
-    pla.call("PLA_Set_device_nb", 256)
-    pla.call("PLA_LU", A, P)
+    pla.call("PLA_Set_device_nb", nb)
+    first = None
+    for _ in range(repeat):
+        pla.call("PLA_Copy", A0, A)
+        pla.call("PLA_LU", A, P)
+        piv = pla.get_contents(P)[:, 0]
+        if first is None:
+            first = piv.copy()
+        assert np.array_equal(piv, first)
     pla.call("PLA_Set_device_nb", 0)
     pla.call("PLA_Apply_pivots_to_rows", x, P)
     pla.call("PLA_Trsv", pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_UNIT_DIAG, A, x)
