@@ -135,7 +135,7 @@ def timing_query(pla, role):
 # HBM bytes per launch of the trailing-update kernels at n=65536 (rocprofv3
 # --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors measured on a
 # copy of known size): profiles/trailing_gemm_traffic_n65536.md
-TRAFFIC_N65536 = {"hgemm": 9.84e9, "dgemm": 2.87e10}
+TRAFFIC_N65536 = {"hgemm": 9.95e9, "dgemm": 5.84e10}
 
 
 def roofline(kernel, launches, ms, flops, peak, mfma_per_flop=1, traffic=None):
@@ -243,6 +243,7 @@ def main():
             return {"value": round(work / (ms * 1e-3) / 1e12, 3), "ms_per_step": round(ms, 2),
                     "steps": steps, "warmup": warmup, "nb": nb_plain.value, "roofline": rl,
                     "backward_error_inf": eta, "hpl_scaled_residual": eta / (n * 2.0 ** -52),
+                    "hpl_pass": bool(eta / (n * 2.0 ** -52) < 16),
                     "x_minus_1_inf": xerr}
 
         if args.mode == "plain":
@@ -254,7 +255,7 @@ def main():
                    "config": {"workload": f"plain FP64 PLA_LU (partial pivoting) + 1-RHS solve, n={n}",
                               "n": n, "nb": nbv.value, "rhs": 1, "template": "1x1"},
                    "roofline": res["roofline"],
-                   "accuracy": {k: res[k] for k in ("backward_error_inf", "hpl_scaled_residual",
+                   "accuracy": {k: res[k] for k in ("backward_error_inf", "hpl_scaled_residual", "hpl_pass",
                                                     "x_minus_1_inf")}}
         else:
             X = s.matrix(n, 1)
