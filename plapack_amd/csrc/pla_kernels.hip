@@ -347,6 +347,68 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Tall-skinny DGEMM, n <= 64 and k <= 64: C = beta C + alpha A op(B) with A
+// untransposed (the panel recursion's updates, m up to the whole column).
+// One workgroup per 64 rows, one wave per 16 of them: alpha op(B) (k x n) is
+// staged once in LDS, each lane loads its A entries (16 consecutive rows of
+// one column per quarter-wave) for all of K up front, and the wave runs
+// ceil(n/16) x ceil(k/4) MFMAs.  The 128 x 128 tiles of dgemm_kernel would
+// leave most of their MFMA work and every operand load on the bounds-checked
+// edge path for these shapes.
+constexpr int TS_N = 64, TS_K = 64;
+
+template <bool TB>
+__global__ __launch_bounds__(256) void dgemm_tall_kernel(int m, int n, int k, double alpha,
+                                                         const double *__restrict__ A, long lda,
+                                                         const double *__restrict__ B, long ldb, double beta,
+                                                         double *__restrict__ C, long ldc,
+                                                         const int *__restrict__ abort_flag) {
+  if (abort_flag && *abort_flag) return;
+  __shared__ double Bs[TS_K][TS_N + 1];  // Bs[kk][j] = alpha op(B)(kk, j), zero outside k x n
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  for (int e = tid; e < TS_K * TS_N; e += 256) {
+    const int kk = e & 63, j = e >> 6;  // consecutive threads: consecutive depth (untransposed B is depth-contiguous)
+    double v = 0.0;
+    if (kk < k && j < n) v = alpha * (TB ? B[(long)kk * ldb + j] : B[(long)j * ldb + kk]);
+    Bs[kk][j] = v;
+  }
+  const int row = blockIdx.x * 64 + wave * 16 + l16;
+  const bool rok = row < m;
+  const int nkq = (k + 3) >> 2, njt = (n + 15) >> 4;
+  double a[TS_K / 4];
+#pragma unroll
+  for (int q = 0; q < TS_K / 4; q++) {
+    const int kk = 4 * q + lq;
+    a[q] = (q < nkq && rok && kk < k) ? A[(long)kk * lda + row] : 0.0;
+  }
+  dbl4 acc[TS_N / 16];
+#pragma unroll
+  for (int t = 0; t < TS_N / 16; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = 16 * t + lq + 4 * r;
+      acc[t][r] = (beta != 0.0 && rok && col < n) ? beta * C[(long)col * ldc + row] : 0.0;
+    }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < TS_K / 4; q++) {
+    if (q < nkq) {
+#pragma unroll
+      for (int t = 0; t < TS_N / 16; t++)
+        if (t < njt) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Bs[4 * q + lq][16 * t + l16], a[q], acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < TS_N / 16; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = 16 * t + lq + 4 * r;
+      if (rok && col < n) C[(long)col * ldc + row] = acc[t][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Unblocked panel LU of a tall leaf (mp rows, w <= 32 columns) in one
 // persistent grid of G workgroups; workgroup g keeps rows [g*R, g*R+R) in LDS.
 //
@@ -1042,6 +1104,17 @@ int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, doubl
            const double *A, long lda, const double *B, long ldb, double beta, double *C, long ldc,
            const int *abort_flag) {
   if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  if (!ta && n <= TS_N && k <= TS_K && k > 0 && m >= 256 && role == 0) {
+    const unsigned grid = (unsigned)((m + 63) / 64);
+    if (tb)
+      hipLaunchKernelGGL((dgemm_tall_kernel<true>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
+                         C, ldc, abort_flag);
+    else
+      hipLaunchKernelGGL((dgemm_tall_kernel<false>), dim3(grid), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb,
+                         beta, C, ldc, abort_flag);
+    PLA_HIP(hipGetLastError());
+    return PLA_SUCCESS;
+  }
   if ((long)((m + GBM - 1) / GBM) * ((n + GBN - 1) / GBN) > (1L << 31) - 8)
     return fail(PLA_ERR_DIMENSIONS, "gemm: too many tiles");
   if (role == 1 && !ta && !tb)
