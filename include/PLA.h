@@ -82,6 +82,12 @@ extern "C" {
 #define PLA_OP_TRIAN_MAT_PAN (PLA_OP_OFFSET + 6)
 #define PLA_OP_ALL_ALG       (PLA_OP_OFFSET + 7)
 
+#define PLA_API_OFFSET 70
+#define PLA_MODE_CLOSED      (PLA_API_OFFSET + 1)
+#define PLA_MODE_OPEN        (PLA_API_OFFSET + 2)
+#define PLA_STATE_API_ACTIVE (PLA_API_OFFSET + 3)
+#define PLA_STATE_NORMAL     (PLA_API_OFFSET + 4)
+
 #define PLA_PROJ_OFFSET 90
 #define PLA_PROJ_ONTO_COL (PLA_PROJ_OFFSET + 1)
 #define PLA_PROJ_ONTO_ROW (PLA_PROJ_OFFSET + 2)
@@ -217,6 +223,52 @@ int PLA_Obj_vert_split_2(PLA_Obj obj, int width, PLA_Obj *left, PLA_Obj *right);
 /* COPY/PLA_Copy.c:18 -- same-shape copy between any two objects */
 int PLA_Copy(PLA_Obj from, PLA_Obj to);
 
+/* ---- application interface (API/PLA_API.c; prototypes as in
+   INCLUDE/PLA_API_prototypes.h).  Host buffers are added into objects in HBM
+   (*_to_global) or pieces of objects added into host buffers (global_to_*).
+   The objects must be PLA_MATRIX or PLA_MVECTOR of MPI_DOUBLE, MPI_FLOAT or
+   MPI_INT, opened with PLA_Obj_API_open inside PLA_API_begin / PLA_API_end.
+   A *_to_global call may return before the add is done (the buffer is
+   already copied and may be reused); PLA_Obj_API_sync / _close / PLA_API_end
+   complete it.  A global_to_* call returns with the buffer updated.  alpha
+   (a pointer to one element of the object's type; NULL means 1) scales the
+   added values -- the reference ignores it and always adds with 1.  Vector
+   forms address column zero_or_one of the template, as the reference does
+   (API/PLA_API.c:247,285); local increments other than 1 are accepted. ---- */
+int PLA_API_begin(void);                                   /* API/PLA_API.c:24 */
+int PLA_API_end(void);                                     /* API/PLA_API.c:61 */
+int PLA_API_state(int *state);                             /* API/PLA_API.c:109 */
+int PLA_Obj_API_open(PLA_Obj obj);                         /* API/PLA_API.c:126 */
+int PLA_Obj_API_close(PLA_Obj obj);                        /* API/PLA_API.c:166 */
+int PLA_Obj_API_mode(PLA_Obj obj, int *mode);              /* API/PLA_API.c:200 */
+int PLA_Obj_API_sync(PLA_Obj obj);                         /* API/PLA_API.c:916 */
+/* API/PLA_API.c:220 */
+int PLA_API_axpy_vector_to_global(int size, void *alpha, void *local_vector, int local_inc, PLA_Obj laobj,
+                                  int disp);
+/* API/PLA_API.c:257 */
+int PLA_API_axpy_global_to_vector(int size, void *alpha, PLA_Obj laobj, int disp, void *local_vector,
+                                  int local_inc);
+/* API/PLA_API.c:295 */
+int PLA_API_multi_axpy_vector_to_global(int nsub, int *sizes, void *alpha, void *local_vector, int local_stride,
+                                        PLA_Obj laobj, int *displs);
+/* API/PLA_API.c:343 */
+int PLA_API_multi_axpy_global_to_vector(int nsub, int *sizes, void *alpha, PLA_Obj laobj, int *displs,
+                                        void *local_vector, int local_stride);
+/* API/PLA_API.c:390 */
+int PLA_API_axpy_matrix_to_global(int m, int n, void *alpha, void *local_buf, int local_ldim, PLA_Obj A,
+                                  int disp_row, int disp_col);
+/* API/PLA_API.c:593 */
+int PLA_API_axpy_global_to_matrix(int m, int n, void *alpha, PLA_Obj A, int disp_row, int disp_col,
+                                  void *local_buf, int local_ldim);
+/* API/PLA_API.c:797 */
+int PLA_API_multi_axpy_matrix_to_global(int nsub_row, int nsub_col, int *size_row, int *size_col, void *alpha,
+                                        void *local_matrix, int local_ldim, PLA_Obj obj, int *disp_row,
+                                        int *disp_col);
+/* API/PLA_API.c:856 */
+int PLA_API_multi_axpy_global_to_matrix(int nsub_row, int nsub_col, int *size_row, int *size_col, void *alpha,
+                                        PLA_Obj obj, int *disp_row, int *disp_col, void *local_matrix,
+                                        int local_ldim);
+
 /* ---- factor and solve path ---- */
 /* SOLVERS/LU/PLA_LU.c:13.  A: m x n MPI_DOUBLE matrix, overwritten with L
    (unit, below the diagonal) and U; pivots: MPI_INT mvector of length
@@ -233,13 +285,17 @@ int PLA_Chol(int uplo, PLA_Obj A);
 int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots);
 /* BLAS2/PLA_Trsv.c:15 */
 int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x);
-/* BLAS3/PLA_Trsm.c:13 (side = PLA_SIDE_LEFT) */
+/* BLAS3/PLA_Trsm.c:13: B <- alpha op(A)^-1 B (PLA_SIDE_LEFT) or
+   alpha B op(A)^-1 (PLA_SIDE_RIGHT), every uplo / trans / diag */
 int PLA_Trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, PLA_Obj B);
 /* BLAS2/PLA_Gemv.c:13 */
 int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y);
 /* BLAS3/PLA_Gemm.c:13 */
 int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta,
              PLA_Obj C);
+/* BLAS3/PLA_Syrk.c:13: C <- beta C on all of C (PLA_Syrk_panpan.c:22), then
+   the uplo triangle += alpha op(A) op(A)^T */
+int PLA_Syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Obj C);
 /* BLAS2/PLA_Matrix_infinity_norm.c:13, BLAS2/PLA_Matrix_one_norm.c:13 --
    alpha (1x1 MPI_DOUBLE) <- max absolute row / column sum of A */
 int PLA_Matrix_infinity_norm(PLA_Obj A, PLA_Obj alpha);

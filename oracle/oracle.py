@@ -53,6 +53,12 @@ def lib():
         L.ora_trsv.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i, _p, _i, _p]
         L.ora_trsm_left.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i, _i, _d, _p, _i,
                                     _p, _i]
+        L.ora_trsm_right.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_char, _i, _i, _d, _p, _i,
+                                     _p, _i]
+        L.ora_syrk.argtypes = [ctypes.c_char, ctypes.c_char, _i, _i, _d, _p, _i, _d, _p, _i]
+        L.ora_api_to_global.argtypes = [_i, _i, _d, _p, _i, _p, _i, _i, _i]
+        L.ora_api_from_global.argtypes = [_i, _i, _d, _p, _i, _i, _i, _p, _i]
+        L.ora_api_multi.argtypes = [_i, _i, _i, _p, _p, _d, _p, _i, _p, _i, _p, _p]
         L.ora_gemm.argtypes = [ctypes.c_char, ctypes.c_char, _i, _i, _i, _d, _p, _i, _p, _i, _d, _p, _i]
         L.ora_lu_solve.argtypes = [_i, _i, _p, _i, _p, _p, _i]
         L.ora_chol_solve.argtypes = [_i, _i, _p, _i, _p, _i]
@@ -132,6 +138,39 @@ def trsm_left(uplo, trans, diag, t, b, alpha=1.0):
     lib().ora_trsm_left(uplo.encode(), trans.encode(), diag.encode(), n, nrhs, alpha, _ptr(t),
                         max(t.shape[0], 1), _ptr(b), max(n, 1))
     return b
+
+
+def trsm_right(uplo, trans, diag, t, b, alpha=1.0):
+    """B <- alpha B op(T)^-1 (ora_trsm_right)."""
+    t = _f(t)
+    b = _f(b).copy(order="F")
+    m, n = b.shape
+    lib().ora_trsm_right(uplo.encode(), trans.encode(), diag.encode(), m, n, alpha, _ptr(t),
+                         max(t.shape[0], 1), _ptr(b), max(m, 1))
+    return b
+
+
+def syrk(uplo, trans, alpha, a, beta, c):
+    """All of C scaled by beta, then the uplo triangle += alpha op(A) op(A)^T (ora_syrk)."""
+    a = _f(a)
+    c = _f(c).copy(order="F")
+    n = c.shape[0]
+    k = a.shape[1] if trans == "N" else a.shape[0]
+    lib().ora_syrk(uplo.encode(), trans.encode(), n, k, alpha, _ptr(a), max(a.shape[0], 1), beta, _ptr(c),
+                   max(n, 1))
+    return c
+
+
+def api_multi(to_global, rsz, csz, alpha, buf, a, rd, cd):
+    """PLA_API_multi_axpy_{matrix_to_global,global_to_matrix} on one process
+    (ora_api_multi): returns the updated (buf, a)."""
+    buf = _f(buf).copy(order="F")
+    a = _f(a).copy(order="F")
+    i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+    rsz, csz, rd, cd = i32(rsz), i32(csz), i32(rd), i32(cd)
+    lib().ora_api_multi(1 if to_global else 0, len(rsz), len(csz), _ptr(rsz), _ptr(csz), alpha, _ptr(buf),
+                        max(buf.shape[0], 1), _ptr(a), max(a.shape[0], 1), _ptr(rd), _ptr(cd))
+    return buf, a
 
 
 def gemm(ta, tb, alpha, a, b, beta, c):

@@ -140,6 +140,36 @@ void ora_gemm(char ta, char tb, int m, int n, int k, double alpha, const double 
   }
 }
 
+/* PLA_Syrk (BLAS3/PLA_Syrk.c:13, PLA_Syrk_panpan.c:13-95): every entry of C
+   is first multiplied by beta (PLA_Local_scal, PLA_Syrk_panpan.c:22; a
+   multiply, so 0 * NaN stays NaN), then the uplo triangle receives
+   alpha op(A) op(A)^T in dsyrk's loop order.  op(A) = A (n x k) for
+   trans 'N', A^T (A k x n) for 'T'. */
+void ora_syrk(char uplo, char trans, int n, int k, double alpha, const double *a, int lda, double beta, double *c,
+              int ldc) {
+  int lower = (uplo == 'L' || uplo == 'l');
+  int tr = (trans == 'T' || trans == 't');
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) c[(size_t)j * ldc + i] *= beta;
+  for (int j = 0; j < n; j++) {
+    double *cj = c + (size_t)j * ldc;
+    int i0 = lower ? j : 0, i1 = lower ? n : j + 1;
+    if (!tr) {
+      for (int l = 0; l < k; l++) {
+        double temp = alpha * A_(j, l);
+        if (temp != 0.0)
+          for (int i = i0; i < i1; i++) cj[i] += temp * A_(i, l);
+      }
+    } else {
+      for (int i = i0; i < i1; i++) {
+        double s = 0.0;
+        for (int l = 0; l < k; l++) s += A_(l, i) * A_(l, j);
+        cj[i] += alpha * s;
+      }
+    }
+  }
+}
+
 /* PLA_LU_right, SOLVERS/LU/PLA_LU.c:37-212, on a 1x1 template. */
 void ora_lu_blocked(int m, int n, double *a, int lda, int nb, int *piv) {
   int mn = m < n ? m : n;
@@ -233,6 +263,40 @@ void ora_trsm_left(char uplo, char trans, char diag, int n, int nrhs, double alp
   }
 }
 
+/* B <- alpha B op(A)^{-1}, A n x n, B m x n: dtrsm's right-side column loops
+   (the side PLA_Trsm_right_lower / _upper take, BLAS3/PLA_Trsm_right_lower.c:13,
+   BLAS3/PLA_Trsm_right_upper.c:13).  op(A) upper-type (upper untransposed,
+   lower transposed) runs over the columns left to right, lower-type right to
+   left; a non-unit diagonal is applied as a multiply by its reciprocal. */
+void ora_trsm_right(char uplo, char trans, char diag, int m, int n, double alpha, const double *a, int lda,
+                    double *b, int ldb) {
+  int lower = (uplo == 'L' || uplo == 'l');
+  int tr = (trans == 'T' || trans == 't');
+  int unit = (diag == 'U' || diag == 'u');
+#define B_(i, j) b[(size_t)(j) * ldb + (i)]
+  if (alpha != 1.0)
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < m; i++) B_(i, j) *= alpha;
+  /* M(p, q) = op(A)(p, q) */
+#define M_(p, q) (tr ? A_(q, p) : A_(p, q))
+  int fwd = (lower == tr);
+  for (int t = 0; t < n; t++) {
+    int j = fwd ? t : n - 1 - t;
+    if (!unit) {
+      double r = 1.0 / M_(j, j);
+      for (int i = 0; i < m; i++) B_(i, j) *= r;
+    }
+    int q0 = fwd ? j + 1 : 0, q1 = fwd ? n : j;
+    for (int q = q0; q < q1; q++) {
+      double mq = M_(j, q);
+      if (mq != 0.0)
+        for (int i = 0; i < m; i++) B_(i, q) -= mq * B_(i, j);
+    }
+  }
+#undef M_
+#undef B_
+}
+
 void ora_trsv(char uplo, char trans, char diag, int n, const double *a, int lda, double *x) {
   ora_trsm_left(uplo, trans, diag, n, 1, 1.0, a, lda, x, n > 0 ? n : 1);
 }
@@ -314,4 +378,54 @@ void ora_lu_solve(int n, int nrhs, const double *lu, int lda, const int *piv, do
 void ora_chol_solve(int n, int nrhs, const double *l, int lda, double *b, int ldb) {
   ora_trsm_left('L', 'N', 'N', n, nrhs, 1.0, l, lda, b, ldb); /* PLA_Pos_def.c:75-90 */
   ora_trsm_left('L', 'T', 'N', n, nrhs, 1.0, l, lda, b, ldb);
+}
+
+/* ------------------------------------------------------------------ */
+/* Application interface on one process: every block is local, so the */
+/* reference's puts reduce to PLA_Obj_add_to_local_contents /          */
+/* PLA_Obj_add_from_local_contents (API/PLA_API_util.c:1519).  The     */
+/* reference adds the buffer unscaled; alpha here is the documented    */
+/* axpy scalar (equal for alpha = 1).  The product is rounded before   */
+/* the add (no fused multiply-add).                                    */
+
+/* PLA_API_axpy_matrix_to_global, API/PLA_API.c:390-588:
+   A(dr + i, dc + j) += alpha * buf(i, j). */
+void ora_api_to_global(int m, int n, double alpha, const double *buf, int ldb, double *a, int lda, int dr, int dc) {
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) {
+      volatile double t = alpha * buf[(size_t)j * ldb + i];
+      A_(dr + i, dc + j) += t;
+    }
+}
+
+/* PLA_API_axpy_global_to_matrix, API/PLA_API.c:593-792:
+   buf(i, j) += alpha * A(dr + i, dc + j). */
+void ora_api_from_global(int m, int n, double alpha, const double *a, int lda, int dr, int dc, double *buf,
+                         int ldb) {
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < m; i++) {
+      volatile double t = alpha * A_(dr + i, dc + j);
+      buf[(size_t)j * ldb + i] += t;
+    }
+}
+
+/* PLA_API_multi_axpy_matrix_to_global / _global_to_matrix,
+   API/PLA_API.c:797-911: sub-block (i, j) starts at local row
+   sum(rsz[0..i)) and column sum(csz[0..j)) and goes to (rd[i], cd[j]);
+   blocks in row-major order of (i, j). */
+void ora_api_multi(int to_global, int nr, int nc, const int *rsz, const int *csz, double alpha, double *buf,
+                   int ldb, double *a, int lda, const int *rd, const int *cd) {
+  int li = 0;
+  for (int i = 0; i < nr; i++) {
+    int lj = 0;
+    for (int j = 0; j < nc; j++) {
+      double *sub = buf + (size_t)lj * ldb + li;
+      if (to_global)
+        ora_api_to_global(rsz[i], csz[j], alpha, sub, ldb, a, lda, rd[i], cd[j]);
+      else
+        ora_api_from_global(rsz[i], csz[j], alpha, a, lda, rd[i], cd[j], sub, ldb);
+      lj += csz[j];
+    }
+    li += rsz[i];
+  }
 }

@@ -23,7 +23,11 @@
  *   ora_chol_unblocked / ora_chol_blocked     : numpy.linalg.cholesky, and
  *       the failure index against exact leading-minor determinants
  *   ora_laswp                                 : an explicit permutation
- *   ora_trsv / ora_trsm_left / ora_gemm       : numpy solve / matmul
+ *   ora_trsv / ora_trsm_left / ora_trsm_right : scipy solve_triangular
+ *   ora_gemm / ora_syrk                       : numpy matmul (ora_syrk also:
+ *       the opposite triangle is exactly beta C)
+ *   ora_api_*                                 : numpy slice arithmetic,
+ *       bit-exact (one rounded product, one rounded add per element)
  * No function here is "parity unpinned".
  */
 #ifndef PLA_ORACLE_H
@@ -67,9 +71,24 @@ void ora_trsv(char uplo, char trans, char diag, int n, const double *a, int lda,
 /* B <- alpha * op(A)^{-1} B, A n x n triangular, B n x nrhs. */
 void ora_trsm_left(char uplo, char trans, char diag, int n, int nrhs, double alpha,
                    const double *a, int lda, double *b, int ldb);
+/* B <- alpha B op(A)^{-1}, A n x n triangular, B m x n. */
+void ora_trsm_right(char uplo, char trans, char diag, int m, int n, double alpha, const double *a, int lda,
+                    double *b, int ldb);
+/* all of C <- beta C, then the uplo triangle += alpha op(A) op(A)^T (PLA_Syrk) */
+void ora_syrk(char uplo, char trans, int n, int k, double alpha, const double *a, int lda, double beta, double *c,
+              int ldc);
 /* C <- alpha op(A) op(B) + beta C */
 void ora_gemm(char ta, char tb, int m, int n, int k, double alpha, const double *a, int lda,
               const double *b, int ldb, double beta, double *c, int ldc);
+
+/* Application interface on one process (API/PLA_API.c:390-911):
+   A(dr:, dc:) += alpha buf, buf += alpha A(dr:, dc:), and the multi-block
+   forms (to_global != 0: buffer into A). */
+void ora_api_to_global(int m, int n, double alpha, const double *buf, int ldb, double *a, int lda, int dr, int dc);
+void ora_api_from_global(int m, int n, double alpha, const double *a, int lda, int dr, int dc, double *buf,
+                         int ldb);
+void ora_api_multi(int to_global, int nr, int nc, const int *rsz, const int *csz, double alpha, double *buf,
+                   int ldb, double *a, int lda, const int *rd, const int *cd);
 
 /* PLA_General_solve (SOLVERS/LU/PLA_General.c:67-119) on already-factored A:
    apply pivots to B, then L (unit) and U solves. */

@@ -11,6 +11,8 @@
 //   PLA_Apply_pivots_to_rows  SOLVERS/LU/PLA_Apply_pivots_new.c:13
 //   PLA_Trsv    BLAS2/PLA_Trsv.c:15     PLA_Trsm   BLAS3/PLA_Trsm.c:13
 //   PLA_Gemm    BLAS3/PLA_Gemm.c:13     PLA_Gemv   BLAS2/PLA_Gemv.c:13
+//   PLA_Syrk    BLAS3/PLA_Syrk.c:13     trsm_right PLA_Trsm_right_lower|upper
+//               (BLAS3/PLA_Trsm_right_lower.c:13, BLAS3/PLA_Trsm_right_upper.c:13)
 //
 // Recursion replaces the reference's fixed-size inner blocking: every split
 // puts the bulk of the flops into dgemm_kernel and leaves only <= 64-wide
@@ -57,19 +59,36 @@ int trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs,
   return trsm_left(s, lower, trans, unit, n0, nrhs, T00, ldt, B, ldb, abort_flag);
 }
 
+// X op(T) = X0 over column blocks: op(T) = [M00 M01; M10 M11].  Upper-type
+// op(T) (M10 = 0) solves X0 first, then X1 -= X0 M01; lower-type (M01 = 0)
+// solves X1 first, then X0 -= X1 M10.
+int trsm_right(hipStream_t s, bool lower, bool trans, bool unit, int nrows, int n, const double *T, long ldt,
+               double *X, long ldx, const int *abort_flag) {
+  if (nrows <= 0 || n <= 0) return PLA_SUCCESS;
+  const int leaf = trsm_leaf_size();
+  if (n <= leaf) return k_trsm_right_leaf(s, lower, trans, unit, nrows, n, T, ldt, X, ldx, abort_flag);
+  const int n0 = split(n, leaf), n1 = n - n0;
+  const double *T11 = T + (long)n0 * ldt + n0;
+  double *X1 = X + (long)n0 * ldx;
+  int rc;
+  if (lower == trans) {
+    // M01 = T01 (upper) or T10^T (lower, transposed)
+    const double *M01 = trans ? T + n0 : T + (long)n0 * ldt;
+    if ((rc = trsm_right(s, lower, trans, unit, nrows, n0, T, ldt, X, ldx, abort_flag))) return rc;
+    if ((rc = k_gemm(s, 0, false, trans, nrows, n1, n0, -1.0, X, ldx, M01, ldt, 1.0, X1, ldx, abort_flag)))
+      return rc;
+    return trsm_right(s, lower, trans, unit, nrows, n1, T11, ldt, X1, ldx, abort_flag);
+  }
+  // M10 = T10 (lower) or T01^T (upper, transposed)
+  const double *M10 = trans ? T + (long)n0 * ldt : T + n0;
+  if ((rc = trsm_right(s, lower, trans, unit, nrows, n1, T11, ldt, X1, ldx, abort_flag))) return rc;
+  if ((rc = k_gemm(s, 0, false, trans, nrows, n0, n1, -1.0, X1, ldx, M10, ldt, 1.0, X, ldx, abort_flag))) return rc;
+  return trsm_right(s, lower, trans, unit, nrows, n0, T, ldt, X, ldx, abort_flag);
+}
+
 int trsm_rltn(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
               const int *abort_flag) {
-  if (nrows <= 0 || b <= 0) return PLA_SUCCESS;
-  const int leaf = trsm_leaf_size();
-  if (b <= leaf) return k_trsm_rltn_leaf(s, nrows, b, L, ldl, X, ldx, abort_flag);
-  const int b0 = split(b, leaf), b1 = b - b0;
-  int rc;
-  if ((rc = trsm_rltn(s, nrows, b0, L, ldl, X, ldx, abort_flag))) return rc;
-  // X1 -= X0 * L10^T
-  if ((rc = k_gemm(s, 0, false, true, nrows, b1, b0, -1.0, X, ldx, L + b0, ldl, 1.0, X + (long)b0 * ldx,
-                   ldx, abort_flag)))
-    return rc;
-  return trsm_rltn(s, nrows, b1, L + (long)b0 * ldl + b0, ldl, X + (long)b0 * ldx, ldx, abort_flag);
+  return trsm_right(s, true, true, false, nrows, b, L, ldl, X, ldx, abort_flag);
 }
 
 struct panel_rec {
@@ -282,11 +301,8 @@ int PLA_Trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, 
     if ((rc = trsm_left(s, lower, tr, unit, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
   } else if (side == PLA_SIDE_RIGHT) {
     if (B->width != n) return fail(PLA_ERR_DIMENSIONS, "PLA_Trsm: B has %d columns, A is %d", B->width, n);
-    if (!(lower && tr && !unit))
-      return fail(PLA_ERR_UNSUPPORTED,
-                  "PLA_Trsm: right side is implemented for lower, transposed, non-unit A only");
     if ((rc = k_scale2d(s, B->length, B->width, al, b, ld(B)))) return rc;
-    if ((rc = trsm_rltn(s, B->length, n, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = trsm_right(s, lower, tr, unit, B->length, n, a, ld(A), b, ld(B), nullptr))) return rc;
   } else {
     return fail(PLA_ERR_INVALID_PARAM, "PLA_Trsm: invalid side");
   }
@@ -333,6 +349,39 @@ int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Ob
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(B, (void **)&b)) || (rc = dev_ptr(C, (void **)&c)))
     return rc;
   if ((rc = k_gemm(ctx().stream, 0, ta, tb, m, n, ak, al, a, ld(A), b, ld(B), be, c, ld(C), nullptr)))
+    return rc;
+  return written(C);
+}
+
+// C <- beta C, then the uplo triangle of C += alpha op(A) op(A)^T
+// (BLAS3/PLA_Syrk_panpan.c:13-95: PLA_Local_scal(beta, C) scales all of C,
+// both triangles, before the rank-nb updates of the uplo triangle).
+// PLA_NO_TRANS / PLA_CONJ: op(A) = A, n x k; PLA_TRANS / PLA_CONJ_TRANS:
+// op(A) = A^T, A k x n (real data: conjugation is the identity).
+int PLA_Syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Obj C) {
+  int rc;
+  if ((rc = check_double(A, "PLA_Syrk")) || (rc = check_double(C, "PLA_Syrk"))) return rc;
+  if (uplo != PLA_LOWER_TRIANGULAR && uplo != PLA_UPPER_TRIANGULAR)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Syrk: invalid uplo");
+  if (trans != PLA_NO_TRANSPOSE && trans != PLA_TRANSPOSE && trans != PLA_CONJUGATE_TRANSPOSE &&
+      trans != PLA_CONJUGATE)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Syrk: invalid trans");
+  if (A->objtype != PLA_MATRIX || C->objtype != PLA_MATRIX)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Syrk: A and C must be PLA_MATRIX objects");
+  const int n = C->length;
+  if (C->width != n) return fail(PLA_ERR_DIMENSIONS, "PLA_Syrk: C is %d x %d, not square", C->length, C->width);
+  const bool tr = (trans == PLA_TRANSPOSE || trans == PLA_CONJUGATE_TRANSPOSE);
+  if ((tr ? A->width : A->length) != n)
+    return fail(PLA_ERR_DIMENSIONS, "PLA_Syrk: A is %d x %d for a %d x %d C", A->length, A->width, n, n);
+  const int k = tr ? A->length : A->width;
+  double al = 1.0, be = 1.0;
+  if ((alpha && (rc = scalar_value(alpha, &al))) || (beta && (rc = scalar_value(beta, &be)))) return rc;
+  double *a, *c;
+  if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(C, (void **)&c))) return rc;
+  hipStream_t s = ctx().stream;
+  if ((rc = k_scale2d(s, n, n, be, c, ld(C)))) return rc;
+  if (k > 0 && al != 0.0 &&
+      (rc = k_syrk(s, uplo == PLA_LOWER_TRIANGULAR, tr, n, k, al, a, ld(A), 1.0, c, ld(C))))
     return rc;
   return written(C);
 }

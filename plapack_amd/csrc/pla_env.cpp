@@ -320,6 +320,7 @@ int PLA_Finalize(void) {
   if (c.stream_tr) (void)hipStreamSynchronize(c.stream_tr);
   if (c.stream_panel) (void)hipStreamSynchronize(c.stream_panel);
   if (c.stream_trail) (void)hipStreamSynchronize(c.stream_trail);
+  pla::api_release();
   pla::release_workspaces();
   g_initialized = 0;
   c.initialized = false;

@@ -103,11 +103,12 @@ int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
 constexpr int PLA_NUM_WS = 48;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
-// 29-33 GMRES, 34-42 mixed Cholesky
+// 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging
 enum {
   WS_SRES_PART = 24, WS_SRES_MAX, WS_ONES, WS_NORM_R, WS_MDOT,
   WS_GM_V = 29, WS_GM_W, WS_GM_H, WS_GM_T, WS_GM_Z,
-  WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R
+  WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R,
+  WS_API_0 = 43, WS_API_1
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
@@ -157,14 +158,28 @@ int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, doubl
            const int *abort_flag);
 int k_syrk_lower(hipStream_t s, int role, int n, int k, double alpha, const double *X, long ldx,
                  double beta, double *C, long ldc, const int *abort_flag);
+// lower (upper) triangle of the n x n C <- beta C + alpha op(X) op(X)^T,
+// op(X) = X (n x k) or, trans, X^T (X k x n)
+int k_syrk(hipStream_t s, bool lower, bool trans, int n, int k, double alpha, const double *X, long ldx, double beta,
+           double *C, long ldc);
 // rows i and i + piv[i] (i < np) of an nrows x ncols block; entries that point
 // outside the block are ignored
 int k_laswp(hipStream_t s, double *A, long lda, int nrows, int ncols, int np, const int *piv);
 int k_trsm_leaf(hipStream_t s, bool lower, bool trans, bool unit, int b, int ncols, const double *T,
                 long ldt, double *B, long ldb, const int *abort_flag);
-int k_trsm_rltn_leaf(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
-                     const int *abort_flag);
+// X <- X inv(op(T)), X nrows x b, T b x b (b <= 64)
+int k_trsm_right_leaf(hipStream_t s, bool lower, bool trans, bool unit, int nrows, int b, const double *T, long ldt,
+                      double *X, long ldx, const int *abort_flag);
 int k_potrf_leaf(hipStream_t s, int b, double *A, long lda, int offset, int *info);
+// application-interface transfers of a packed M x N block (leading dimension
+// M) to / from its place in an object (maps or offsets r0, c0)
+enum { API_F64 = 1, API_F32 = 2, API_I32 = 3 };
+int k_api_scatter_add(hipStream_t s, int elem, int M, int N, const void *alpha, const void *src, void *dst, long ldd,
+                      const int *rmap, const int *cmap, int r0, int c0);
+int k_api_gather(hipStream_t s, int elem, int M, int N, const void *src, long lds_, void *dst, const int *rmap,
+                 const int *cmap, int r0, int c0);
+void api_forget(PLA_Obj obj);  // drop a freed object from the open-object list
+void api_release();            // free the staging buffers (PLA_Finalize)
 int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D, long ldd,
              const int *abort_flag);
 int k_scale2d(hipStream_t s, int m, int n, double alpha, double *D, long ldd);
@@ -234,6 +249,9 @@ int k_axpy_cols(hipStream_t s, int n, int nrhs, const double *D, long ldd, doubl
 // B <- op(T)^{-1} B, T n x n triangular, B n x nrhs.
 int trsm_left(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *T,
               long ldt, double *B, long ldb, const int *abort_flag);
+// X <- X inv(op(T)), X nrows x n, T n x n triangular.
+int trsm_right(hipStream_t s, bool lower, bool trans, bool unit, int nrows, int n, const double *T, long ldt,
+               double *X, long ldx, const int *abort_flag);
 // X <- X inv(L)^T, X nrows x b, L b x b lower non-unit.
 int trsm_rltn(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
               const int *abort_flag);

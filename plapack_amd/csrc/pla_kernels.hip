@@ -19,8 +19,8 @@
 //   laswp_kernel      PLA_Apply_pivots_to_rows (PLA_Apply_pivots_new.c:13-179)
 //   trsm_leaf_kernel  PLA_Local_trsm / dtrsm (Local_BLAS/PLA_Local_trsm.c:67),
 //                     left side, one thread per right-hand side column.
-//   trsm_rltn_leaf    dtrsm("R","L","T","N") of PLA_Chol.c:193-195, one
-//                     thread per row.
+//   trsm_right_leaf_kernel  dtrsm("R", ...) of PLA_Local_trsm and of
+//                     PLA_Chol.c:193-195 ("R","L","T","N"), one thread per row.
 //   potrf_leaf_kernel PLA_dchol (SOLVERS/Cholesky/PLA_Local_chol.c:103-143)
 //   fill_drand48      TEST/create_problem.c:31-52 (drand48()*2-1, column-major)
 //
@@ -44,8 +44,9 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // B fragment as the A operand and vice versa, so D is the transposed C tile
 // and lane l holds C(l%16, l/16 + 4r): 16 consecutive lanes cover 16
 // consecutive rows of one column (128-byte coalesced loads and stores of C).
-// LOWER: only the lower triangle (row >= col) of C is read or written and
-// tiles entirely above the diagonal exit at once (SYRK).
+// TRI: 0 the whole of C; 1 (2) only the lower (upper) triangle, row >= col
+// (row <= col), of C is read or written and tiles entirely outside it exit at
+// once (SYRK).
 // ROLE only separates kernel symbols (the trailing update is ROLE 1) so that
 // profiles can attribute time to the LU/Cholesky trailing update.
 //
@@ -88,8 +89,13 @@ __device__ __forceinline__ void gemm_mfma_step(const gemm_smem &sm, int cur, int
   }
 }
 
-// acc = sgn * beta * C over the tile (0 outside C or, for LOWER, above the diagonal)
-template <bool LOWER>
+template <int TRI>
+__device__ __forceinline__ bool tri_in(int gi, int gj) {
+  return TRI == 0 || (TRI == 1 ? gi >= gj : gi <= gj);
+}
+
+// acc = sgn * beta * C over the tile (0 outside C or outside the TRI triangle)
+template <int TRI>
 __device__ __forceinline__ void gemm_load_c(int m, int n, double beta, double sgn, const double *__restrict__ C,
                                             long ldc, int i0, int j0, dbl4 (&acc)[4][4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -103,12 +109,12 @@ __device__ __forceinline__ void gemm_load_c(int m, int n, double beta, double sg
         const int gi = i0 + wm * 64 + bi * 16 + l16;
         const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
         double v = 0.0;
-        if (beta != 0.0 && gi < m && gj < n && (!LOWER || gi >= gj)) v = beta * C[(long)gj * ldc + gi];
+        if (beta != 0.0 && gi < m && gj < n && tri_in<TRI>(gi, gj)) v = beta * C[(long)gj * ldc + gi];
         acc[bi][bj][r] = sgn * v;
       }
 }
 
-template <bool LOWER>
+template <int TRI>
 __device__ __forceinline__ void gemm_store_c(int m, int n, double sgn, double *__restrict__ C, long ldc, int i0,
                                              int j0, const dbl4 (&acc)[4][4]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -121,7 +127,7 @@ __device__ __forceinline__ void gemm_store_c(int m, int n, double sgn, double *_
       for (int r = 0; r < 4; r++) {
         const int gi = i0 + wm * 64 + bi * 16 + l16;
         const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
-        if (gi < m && gj < n && (!LOWER || gi >= gj)) C[(long)gj * ldc + gi] = sgn * acc[bi][bj][r];
+        if (gi < m && gj < n && tri_in<TRI>(gi, gj)) C[(long)gj * ldc + gi] = sgn * acc[bi][bj][r];
       }
 }
 
@@ -130,7 +136,7 @@ __device__ __forceinline__ void gemm_store_c(int m, int n, double sgn, double *_
 // pairs (2p, 2p+1), p = tid % 64, at depths tid / 64 + 4e; a depth-contiguous
 // one (A transposed, or B) as depth pairs (2q, 2q+1), q = tid % 8, of indices
 // tid / 8 + 32e; e = 0..3.
-template <bool TA, bool TB, bool LOWER>
+template <bool TA, bool TB, int TRI>
 __device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int k, double sgn,
                                                const double *__restrict__ A, long lda, const double *__restrict__ B,
                                                long ldb, double beta, double *__restrict__ C, long ldc, int i0,
@@ -201,7 +207,7 @@ __device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int 
     if (nkf > 0) load(0);
     else load_tail(0);
   }
-  gemm_load_c<LOWER>(m, n, beta, sgn, C, ldc, i0, j0, acc);
+  gemm_load_c<TRI>(m, n, beta, sgn, C, ldc, i0, j0, acc);
   if (nk > 0) store(0);
   __syncthreads();
   int kt = 0;
@@ -244,11 +250,11 @@ __device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int 
     kt++;
   }
   if (nk > 0) gemm_mfma_step(sm, kt & 1, wm, wn, l16, lq, acc);
-  gemm_store_c<LOWER>(m, n, sgn, C, ldc, i0, j0, acc);
+  gemm_store_c<TRI>(m, n, sgn, C, ldc, i0, j0, acc);
 }
 
 // Any tile, any alpha: element loads with bounds checks, A scaled by alpha.
-template <bool TA, bool TB, bool LOWER>
+template <bool TA, bool TB, int TRI>
 __device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, int k, double alpha,
                                                   const double *__restrict__ A, long lda,
                                                   const double *__restrict__ B, long ldb, double beta,
@@ -256,7 +262,7 @@ __device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, i
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
   dbl4 acc[4][4];
-  gemm_load_c<LOWER>(m, n, beta, 1.0, C, ldc, i0, j0, acc);
+  gemm_load_c<TRI>(m, n, beta, 1.0, C, ldc, i0, j0, acc);
   double ra[8], rb[8];
   auto load_tile = [&](int kt) {
     const int kb = kt * GBK;
@@ -314,11 +320,11 @@ __device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, i
     if (kt + 1 < nk) store_tile(cur ^ 1);
     __syncthreads();
   }
-  gemm_store_c<LOWER>(m, n, 1.0, C, ldc, i0, j0, acc);
+  gemm_store_c<TRI>(m, n, 1.0, C, ldc, i0, j0, acc);
 }
 
 // fast: alpha is +-1 and every operand lane offset fits 32 bits (host check)
-template <bool TA, bool TB, bool LOWER, int ROLE>
+template <bool TA, bool TB, int TRI, int ROLE>
 __global__ __launch_bounds__(256, 2) void dgemm_kernel(
     int m, int n, int k, double alpha, const double *__restrict__ A, long lda,
     const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
@@ -338,12 +344,12 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
   const int tm = first_m + within % gsz;
   const int tn = within / gsz;
   const int i0 = tm * GBM, j0 = tn * GBN;
-  if (LOWER && i0 + GBM <= j0) return;
+  if ((TRI == 1 && i0 + GBM <= j0) || (TRI == 2 && j0 + GBN <= i0)) return;
   __shared__ gemm_smem sm;
   if (fast && i0 + GBM <= m && j0 + GBN <= n)
-    gemm_tile_fast<TA, TB, LOWER>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
+    gemm_tile_fast<TA, TB, TRI>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
   else
-    gemm_tile_checked<TA, TB, LOWER>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
+    gemm_tile_checked<TA, TB, TRI>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
 }
 
 // ---------------------------------------------------------------------------
@@ -897,17 +903,25 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int b, int ncols, const 
     if (i < b) x[i] = v[i];
 }
 
-// X <- X * inv(L^T), X: nrows x b (b <= 64), L lower b x b non-unit.
-// One thread per row of X; netlib dtrsm("R","L","T","N") order: scale column
-// k by 1/L(k,k), then subtract L(j,k) * X(:,k) from the later columns j.
-__global__ __launch_bounds__(256) void trsm_rltn_leaf_kernel(int nrows, int b, const double *__restrict__ Lm,
-                                                            long ldl, double *__restrict__ X, long ldx,
-                                                            const int *__restrict__ abort_flag) {
+// Right-side triangular solve of a b x b leaf (b <= 64): X op(T) = X0 for
+// nrows rows, one thread per row (the rows are independent).  M = op(T) is
+// staged in LDS as sM[i][j] = M(i, j); upper-type M (FWD) is solved over the
+// columns left to right, x_j = (x0_j - sum_{i<j} x_i M(i,j)) / M(j,j), lower-
+// type M right to left.  A non-unit diagonal is applied as a multiply by its
+// reciprocal, the order of dtrsm's right-side loops.  Every lane reads the
+// same sM entry at a time (LDS broadcast).
+template <bool FWD, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_right_leaf_kernel(int nrows, int b, const double *__restrict__ T,
+                                                              long ldt, int trans, double *__restrict__ X, long ldx,
+                                                              const int *__restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ double sL[TRSM_LEAF][TRSM_LEAF + 1];
+  __shared__ double sM[TRSM_LEAF][TRSM_LEAF + 1];
   for (int e = threadIdx.x; e < b * b; e += 256) {
-    const int kcol = e / b, i = e - kcol * b;
-    sL[i][kcol] = Lm[(long)kcol * ldl + i];
+    const int col = e / b, row = e - col * b;  // T(row, col), read down the columns
+    if (trans)
+      sM[col][row] = T[(long)col * ldt + row];
+    else
+      sM[row][col] = T[(long)col * ldt + row];
   }
   __syncthreads();
   const int r = blockIdx.x * 256 + threadIdx.x;
@@ -916,12 +930,18 @@ __global__ __launch_bounds__(256) void trsm_rltn_leaf_kernel(int nrows, int b, c
 #pragma unroll
   for (int c = 0; c < TRSM_LEAF; c++) v[c] = (c < b) ? X[(long)c * ldx + r] : 0.0;
 #pragma unroll
-  for (int kk = 0; kk < TRSM_LEAF; kk++) {
+  for (int q = 0; q < TRSM_LEAF; q++) {
+    const int kk = FWD ? q : TRSM_LEAF - 1 - q;
     if (kk < b) {
-      v[kk] *= 1.0 / sL[kk][kk];
+      if (!UNIT) v[kk] *= 1.0 / sM[kk][kk];
+      if (FWD) {
 #pragma unroll
-      for (int j = kk + 1; j < TRSM_LEAF; j++)
-        if (j < b) v[j] = fma(-sL[j][kk], v[kk], v[j]);
+        for (int j = kk + 1; j < TRSM_LEAF; j++)
+          if (j < b) v[j] = fma(-sM[kk][j], v[kk], v[j]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < kk; j++) v[j] = fma(-sM[kk][j], v[kk], v[j]);
+      }
     }
   }
 #pragma unroll
@@ -983,6 +1003,42 @@ __global__ __launch_bounds__(256) void copy2d_kernel(int m, int n, const double 
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
     const int j = (int)(e / m), i = (int)(e - (long)j * m);
     D[(long)j * ldd + i] = S[(long)j * lds_ + i];
+  }
+}
+
+// Application-interface transfers (API/PLA_API.c:390-792).  A packed M x N
+// block (column-major, leading dimension M) travels between host and HBM in
+// one DMA copy; these kernels move it between the packed form and its place in
+// the global object.  Local row i / column j of the block lands on global row
+// rmap[i] / column cmap[j] (or r0 + i / c0 + j without a map), so many
+// sub-blocks of a multi-axpy go in one launch.  Rows run over x (coalesced
+// within a column), columns over y.
+template <typename T>
+__global__ __launch_bounds__(256) void api_scatter_add_kernel(int M, int N, T alpha, const T *__restrict__ src,
+                                                              T *__restrict__ dst, long ldd,
+                                                              const int *__restrict__ rmap,
+                                                              const int *__restrict__ cmap, int r0, int c0) {
+#pragma clang fp contract(off)  // alpha * x rounded, then added: no fused multiply-add
+  for (int j = blockIdx.y; j < N; j += gridDim.y) {
+    const long gj = cmap ? cmap[j] : c0 + j;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < M; i += gridDim.x * 256) {
+      const long gi = rmap ? rmap[i] : r0 + i;
+      T *d = dst + gj * ldd + gi;
+      *d = *d + alpha * src[(long)j * M + i];
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void api_gather_kernel(int M, int N, const T *__restrict__ src, long lds_,
+                                                         T *__restrict__ dst, const int *__restrict__ rmap,
+                                                         const int *__restrict__ cmap, int r0, int c0) {
+  for (int j = blockIdx.y; j < N; j += gridDim.y) {
+    const long gj = cmap ? cmap[j] : c0 + j;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < M; i += gridDim.x * 256) {
+      const long gi = rmap ? rmap[i] : r0 + i;
+      dst[(long)j * M + i] = src[gj * lds_ + gi];
+    }
   }
 }
 
@@ -1086,7 +1142,7 @@ inline unsigned grid1d(long total, int per_block) {
 
 namespace pla {
 
-template <bool TA, bool TB, bool LOWER, int ROLE>
+template <bool TA, bool TB, int TRI, int ROLE>
 static void launch_gemm(hipStream_t s, int m, int n, int k, double alpha, const double *A, long lda,
                         const double *B, long ldb, double beta, double *C, long ldc,
                         const int *abort_flag) {
@@ -1096,7 +1152,7 @@ static void launch_gemm(hipStream_t s, int m, int n, int k, double alpha, const 
   // of at most (31 ld + 128) doubles from a uniform base
   const long lim = (1L << 32) / 8 - 128;
   const int fast = (alpha == 1.0 || alpha == -1.0) && 31 * lda < lim && 31 * ldb < lim;
-  hipLaunchKernelGGL((dgemm_kernel<TA, TB, LOWER, ROLE>), dim3(nblk), dim3(256), 0, s, m, n, k, alpha,
+  hipLaunchKernelGGL((dgemm_kernel<TA, TB, TRI, ROLE>), dim3(nblk), dim3(256), 0, s, m, n, k, alpha,
                      A, lda, B, ldb, beta, C, ldc, tm, tn, abort_flag, fast);
 }
 
@@ -1118,15 +1174,15 @@ int k_gemm(hipStream_t s, int role, bool ta, bool tb, int m, int n, int k, doubl
   if ((long)((m + GBM - 1) / GBM) * ((n + GBN - 1) / GBN) > (1L << 31) - 8)
     return fail(PLA_ERR_DIMENSIONS, "gemm: too many tiles");
   if (role == 1 && !ta && !tb)
-    launch_gemm<false, false, false, 1>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+    launch_gemm<false, false, 0, 1>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
   else if (!ta && !tb)
-    launch_gemm<false, false, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+    launch_gemm<false, false, 0, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
   else if (!ta && tb)
-    launch_gemm<false, true, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+    launch_gemm<false, true, 0, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
   else if (ta && !tb)
-    launch_gemm<true, false, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+    launch_gemm<true, false, 0, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
   else
-    launch_gemm<true, true, false, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
+    launch_gemm<true, true, 0, 0>(s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, abort_flag);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
@@ -1135,9 +1191,27 @@ int k_syrk_lower(hipStream_t s, int role, int n, int k, double alpha, const doub
                  double beta, double *C, long ldc, const int *abort_flag) {
   if (n <= 0) return PLA_SUCCESS;
   if (role == 1)
-    launch_gemm<false, true, true, 1>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, abort_flag);
+    launch_gemm<false, true, 1, 1>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, abort_flag);
   else
-    launch_gemm<false, true, true, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, abort_flag);
+    launch_gemm<false, true, 1, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, abort_flag);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_syrk(hipStream_t s, bool lower, bool trans, int n, int k, double alpha, const double *X, long ldx, double beta,
+           double *C, long ldc) {
+  if (n <= 0) return PLA_SUCCESS;
+  if ((long)((n + GBM - 1) / GBM) * ((n + GBN - 1) / GBN) > (1L << 31) - 8)
+    return fail(PLA_ERR_DIMENSIONS, "syrk: too many tiles");
+  // C (n x n) += alpha op(X) op(X)^T with op(X) = X (n x k) or X^T (X k x n)
+  if (lower && !trans)
+    launch_gemm<false, true, 1, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, nullptr);
+  else if (lower)
+    launch_gemm<true, false, 1, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, nullptr);
+  else if (!trans)
+    launch_gemm<false, true, 2, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, nullptr);
+  else
+    launch_gemm<true, false, 2, 0>(s, n, n, k, alpha, X, ldx, X, ldx, beta, C, ldc, nullptr);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
@@ -1173,12 +1247,24 @@ int k_trsm_leaf(hipStream_t s, bool lower, bool trans, bool unit, int b, int nco
   return PLA_SUCCESS;
 }
 
-int k_trsm_rltn_leaf(hipStream_t s, int nrows, int b, const double *L, long ldl, double *X, long ldx,
-                     const int *abort_flag) {
+int k_trsm_right_leaf(hipStream_t s, bool lower, bool trans, bool unit, int nrows, int b, const double *T, long ldt,
+                      double *X, long ldx, const int *abort_flag) {
   if (nrows <= 0 || b <= 0) return PLA_SUCCESS;
-  if (b > TRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "trsm_rltn leaf: b = %d", b);
-  hipLaunchKernelGGL(trsm_rltn_leaf_kernel, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, b, L,
-                     ldl, X, ldx, abort_flag);
+  if (b > TRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "trsm right leaf: b = %d", b);
+  const dim3 grid((nrows + 255) / 256), blk(256);
+  const bool fwd = (lower == trans);  // op(T) upper-type
+  if (fwd && unit)
+    hipLaunchKernelGGL((trsm_right_leaf_kernel<true, true>), grid, blk, 0, s, nrows, b, T, ldt, (int)trans, X, ldx,
+                       abort_flag);
+  else if (fwd)
+    hipLaunchKernelGGL((trsm_right_leaf_kernel<true, false>), grid, blk, 0, s, nrows, b, T, ldt, (int)trans, X, ldx,
+                       abort_flag);
+  else if (unit)
+    hipLaunchKernelGGL((trsm_right_leaf_kernel<false, true>), grid, blk, 0, s, nrows, b, T, ldt, (int)trans, X, ldx,
+                       abort_flag);
+  else
+    hipLaunchKernelGGL((trsm_right_leaf_kernel<false, false>), grid, blk, 0, s, nrows, b, T, ldt, (int)trans, X,
+                       ldx, abort_flag);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
@@ -1189,6 +1275,55 @@ int k_potrf_leaf(hipStream_t s, int b, double *A, long lda, int offset, int *inf
   hipLaunchKernelGGL(potrf_leaf_kernel, dim3(1), dim3(64), 0, s, b, A, lda, offset, info);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
+}
+
+// grid of the API transfer kernels: enough row blocks to cover M (at most
+// 64), columns over y (at most 65535, grid-stride beyond)
+static dim3 api_grid(int M, int N) {
+  const int gx = (M + 255) / 256 < 64 ? (M + 255) / 256 : 64;
+  const int gy = N < 65535 ? N : 65535;
+  return dim3(gx > 0 ? gx : 1, gy > 0 ? gy : 1);
+}
+
+template <typename T>
+static int api_scatter_t(hipStream_t s, int M, int N, const void *alpha, const void *src, void *dst, long ldd,
+                         const int *rmap, const int *cmap, int r0, int c0) {
+  const T a = *(const T *)alpha;
+  hipLaunchKernelGGL((api_scatter_add_kernel<T>), api_grid(M, N), dim3(256), 0, s, M, N, a, (const T *)src,
+                     (T *)dst, ldd, rmap, cmap, r0, c0);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+template <typename T>
+static int api_gather_t(hipStream_t s, int M, int N, const void *src, long lds_, void *dst, const int *rmap,
+                        const int *cmap, int r0, int c0) {
+  hipLaunchKernelGGL((api_gather_kernel<T>), api_grid(M, N), dim3(256), 0, s, M, N, (const T *)src, lds_, (T *)dst,
+                     rmap, cmap, r0, c0);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_api_scatter_add(hipStream_t s, int elem, int M, int N, const void *alpha, const void *src, void *dst, long ldd,
+                      const int *rmap, const int *cmap, int r0, int c0) {
+  if (M <= 0 || N <= 0) return PLA_SUCCESS;
+  switch (elem) {
+    case API_F64: return api_scatter_t<double>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
+    case API_F32: return api_scatter_t<float>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
+    case API_I32: return api_scatter_t<int>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
+  }
+  return fail(PLA_ERR_UNSUPPORTED, "api scatter: element kind %d", elem);
+}
+
+int k_api_gather(hipStream_t s, int elem, int M, int N, const void *src, long lds_, void *dst, const int *rmap,
+                 const int *cmap, int r0, int c0) {
+  if (M <= 0 || N <= 0) return PLA_SUCCESS;
+  switch (elem) {
+    case API_F64: return api_gather_t<double>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
+    case API_F32: return api_gather_t<float>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
+    case API_I32: return api_gather_t<int>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
+  }
+  return fail(PLA_ERR_UNSUPPORTED, "api gather: element kind %d", elem);
 }
 
 int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D, long ldd,

@@ -262,6 +262,7 @@ int PLA_Create_constants_conf_to(PLA_Obj obj, PLA_Obj *minus_one, PLA_Obj *zero,
 
 int PLA_Obj_free(PLA_Obj *obj) {
   if (!obj) return PLA_SUCCESS;
+  if (*obj) pla::api_forget(*obj);
   free_view(*obj);
   *obj = nullptr;
   return PLA_SUCCESS;

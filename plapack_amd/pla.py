@@ -42,6 +42,19 @@ PLA_OP_ALL_ALG = 87
 PLA_UNIT_DIAG = 101
 PLA_NONUNIT_DIAG = 102
 PLA_SUCCESS = 0
+# error codes (negative; PLA_Last_error() holds the message)
+PLA_ERR_NOT_INITIALIZED = -1001
+PLA_ERR_INVALID_OBJECT = -1002
+PLA_ERR_INVALID_PARAM = -1003
+PLA_ERR_UNSUPPORTED = -1004
+PLA_ERR_DIMENSIONS = -1005
+PLA_ERR_DEVICE = -1006
+PLA_ERR_NO_MEMORY = -1007
+PLA_ERR_MULTI_PROCESS = -1008
+PLA_MODE_CLOSED = 71
+PLA_MODE_OPEN = 72
+PLA_STATE_API_ACTIVE = 73
+PLA_STATE_NORMAL = 74
 PLA_ALIGN_FIRST = -200
 PLA_MIXED_FP16 = 1
 PLA_MIXED_BF16 = 2
@@ -123,6 +136,22 @@ _SIGS = {
     "PLA_Trsm": [_i, _i, _i, _i, _obj, _obj, _obj],
     "PLA_Gemv": [_i, _obj, _obj, _obj, _obj, _obj],
     "PLA_Gemm": [_i, _i, _obj, _obj, _obj, _obj, _obj],
+    "PLA_Syrk": [_i, _i, _obj, _obj, _obj, _obj],
+    "PLA_API_begin": [],
+    "PLA_API_end": [],
+    "PLA_API_state": [_pi],
+    "PLA_Obj_API_open": [_obj],
+    "PLA_Obj_API_close": [_obj],
+    "PLA_Obj_API_mode": [_obj, _pi],
+    "PLA_Obj_API_sync": [_obj],
+    "PLA_API_axpy_vector_to_global": [_i, _p, _p, _i, _obj, _i],
+    "PLA_API_axpy_global_to_vector": [_i, _p, _obj, _i, _p, _i],
+    "PLA_API_multi_axpy_vector_to_global": [_i, _p, _p, _p, _i, _obj, _p],
+    "PLA_API_multi_axpy_global_to_vector": [_i, _p, _p, _obj, _p, _p, _i],
+    "PLA_API_axpy_matrix_to_global": [_i, _i, _p, _p, _i, _obj, _i, _i],
+    "PLA_API_axpy_global_to_matrix": [_i, _i, _p, _obj, _i, _i, _p, _i],
+    "PLA_API_multi_axpy_matrix_to_global": [_i, _i, _p, _p, _p, _p, _i, _obj, _p, _p],
+    "PLA_API_multi_axpy_global_to_matrix": [_i, _i, _p, _p, _p, _obj, _p, _p, _p, _i],
     "PLA_Matrix_infinity_norm": [_obj, _obj],
     "PLA_Matrix_one_norm": [_obj, _obj],
     "PLA_General_solve": [_obj, _obj, _obj],

@@ -31,7 +31,8 @@ def test_constants_match_reference_values():
               "PLA_TRANSPOSE": 22, "PLA_UNIT_DIAG": 101, "PLA_NONUNIT_DIAG": 102,
               "PLA_SIDE_LEFT": 43, "PLA_SIDE_RIGHT": 44, "PLA_OP_PAN_PAN": 83, "PLA_OP_ALL_ALG": 87,
               "PLA_MATRIX": 13, "PLA_MVECTOR": 12, "PLA_MSCALAR": 14, "PLA_ALL_ROWS": -31,
-              "PLA_ALIGN_FIRST": -200}
+              "PLA_ALIGN_FIRST": -200, "PLA_MODE_CLOSED": 71, "PLA_MODE_OPEN": 72,
+              "PLA_STATE_API_ACTIVE": 73, "PLA_STATE_NORMAL": 74}
     src = "#define PLA_NO_MPI_H\n#include <stdio.h>\n#include \"PLA.h\"\nint main(void){\n"
     for k in expect:
         src += f'printf("{k} %d\\n", (int)({k}));\n'

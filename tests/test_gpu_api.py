@@ -1,0 +1,257 @@
+"""GPU parity of the application interface (API/PLA_API.c) through the C ABI:
+host buffers added into objects in HBM and pieces of objects added into host
+buffers, against the oracle's restatement (ora_api_*), bit-exact (each element
+receives one rounded product alpha * x and one rounded add).
+
+Covers the reference's own usage (EXAMPLES/LU/create_problem.c:132-183: a
+matrix filled through PLA_API_axpy_matrix_to_global in 10-column blocks, a
+vector through PLA_API_axpy_vector_to_global; EXAMPLES/LU/process_answer.c:28-82:
+a vector read back through PLA_API_axpy_global_to_vector), multi-block forms
+with disjoint and overlapping targets, transfers larger than one staging slot,
+MPI_FLOAT / MPI_INT objects, host mirrors, and the state errors the reference
+aborts on."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from plapack_amd import pla
+
+pytestmark = pytest.mark.gpu
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ints(v):
+    return np.ascontiguousarray(v, dtype=np.int32)
+
+
+def _d(x):
+    return ctypes.byref(ctypes.c_double(x))
+
+
+@pytest.fixture
+def api(pla_session):
+    pla.call("PLA_API_begin")
+    yield pla_session
+    state = ctypes.c_int()
+    pla.call("PLA_API_state", ctypes.byref(state))
+    if state.value == pla.PLA_STATE_API_ACTIVE:
+        pla.lib().PLA_API_end()
+
+
+def test_reference_example_fill_and_read_back(api, ora):
+    """create_problem.c's API fill of A (n x n, 10-column blocks) and x, then
+    process_answer.c's read-back of x."""
+    s = api
+    n = 1000
+    A, x = s.matrix(n, n), s.mvector(n)
+    pla.call("PLA_Obj_set_to_zero", A)
+    pla.call("PLA_Obj_set_to_zero", x)
+    pla.call("PLA_Obj_API_open", A)
+    pla.call("PLA_Obj_API_open", x)
+    full = ora.uniform(n, n + 1, 3)
+    for j in range(0, n, 10):
+        w = min(10, n - j)
+        blk = np.asfortranarray(full[:, j:j + w])
+        pla.call("PLA_API_axpy_matrix_to_global", n, w, _d(1.0), _ptr(blk), n, A, 0, j)
+    xv = np.ascontiguousarray(full[:, n])
+    pla.call("PLA_API_axpy_vector_to_global", n, _d(1.0), _ptr(xv), 1, x, 0)
+    pla.call("PLA_Obj_API_close", A)
+    pla.call("PLA_Obj_API_close", x)
+    pla.call("PLA_API_end")
+    assert np.array_equal(pla.get_contents(A), full[:, :n])
+    assert np.array_equal(pla.get_contents(x)[:, 0], full[:, n])
+    # read back x into a zeroed local vector
+    pla.call("PLA_API_begin")
+    pla.call("PLA_Obj_API_open", x)
+    locx = np.zeros(n)
+    pla.call("PLA_API_axpy_global_to_vector", n, _d(1.0), x, 0, _ptr(locx), 1)
+    pla.call("PLA_Obj_API_sync", x)
+    assert np.array_equal(locx, full[:, n])
+    pla.call("PLA_Obj_API_close", x)
+    pla.call("PLA_API_end")
+    pla.free(A, x)
+
+
+@pytest.mark.parametrize("m,n,dr,dc,alpha", [(1, 1, 0, 0, 1.0), (77, 13, 5, 100, -0.5), (300, 129, 0, 0, 3.0),
+                                             (64, 1, 236, 128, 0.25)])
+def test_matrix_axpy_both_directions(api, ora, m, n, dr, dc, alpha):
+    s = api
+    rng = np.random.default_rng(m + n + dr)
+    a0 = rng.standard_normal((300, 257))
+    buf = np.asfortranarray(rng.standard_normal((m + 3, n)))  # local leading dimension m + 3
+    A = pla.from_numpy(s, a0)
+    pla.call("PLA_Obj_API_open", A)
+    pla.call("PLA_API_axpy_matrix_to_global", m, n, _d(alpha), _ptr(buf), m + 3, A, dr, dc)
+    pla.call("PLA_Obj_API_sync", A)
+    _, ref_a = ora.api_multi(True, [m], [n], alpha, buf[:m], a0, [dr], [dc])
+    assert np.array_equal(pla.get_contents(A), ref_a)
+    loc = np.asfortranarray(rng.standard_normal((m + 3, n)))
+    loc0 = loc.copy()
+    pla.call("PLA_API_axpy_global_to_matrix", m, n, _d(-2.0), A, dr, dc, _ptr(loc), m + 3)
+    ref_b, _ = ora.api_multi(False, [m], [n], -2.0, loc0[:m], ref_a, [dr], [dc])
+    assert np.array_equal(loc[:m], ref_b)
+    assert np.array_equal(loc[m:], loc0[m:])  # rows past m of the local buffer untouched
+    pla.call("PLA_Obj_API_close", A)
+    pla.free(A)
+
+
+def test_transfers_larger_than_a_staging_slot(api, ora):
+    """3000 x 1500 doubles (36 MB) go in several chunks, both directions."""
+    s = api
+    rng = np.random.default_rng(11)
+    m, n = 3000, 1500
+    a0 = rng.standard_normal((m + 10, n + 4))
+    buf = np.asfortranarray(rng.standard_normal((m, n)))
+    A = pla.from_numpy(s, a0)
+    pla.call("PLA_Obj_API_open", A)
+    pla.call("PLA_API_axpy_matrix_to_global", m, n, _d(0.5), _ptr(buf), m, A, 7, 3)
+    loc = np.asfortranarray(rng.standard_normal((m, n)))
+    loc0 = loc.copy()
+    # the read sees the add queued before it (same object, same stream)
+    pla.call("PLA_API_axpy_global_to_matrix", m, n, _d(1.0), A, 7, 3, _ptr(loc), m)
+    pla.call("PLA_Obj_API_close", A)
+    _, ref_a = ora.api_multi(True, [m], [n], 0.5, buf, a0, [7], [3])
+    ref_b, _ = ora.api_multi(False, [m], [n], 1.0, loc0, ref_a, [7], [3])
+    assert np.array_equal(pla.get_contents(A), ref_a)
+    assert np.array_equal(loc, ref_b)
+    pla.free(A)
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+def test_multi_matrix_axpy(api, ora, overlap):
+    """Disjoint targets go in one launch; overlapping ones block by block."""
+    s = api
+    rng = np.random.default_rng(12 + overlap)
+    a0 = rng.standard_normal((200, 150))
+    rsz, csz = [30, 0, 45, 17], [20, 33, 9]
+    rd = [150, 7, 60, 100] if not overlap else [150, 7, 60, 70]
+    cd = [120, 0, 50] if not overlap else [120, 0, 20]
+    M, N = sum(rsz), sum(csz)
+    buf = np.asfortranarray(rng.standard_normal((M + 2, N)))
+    A = pla.from_numpy(s, a0)
+    pla.call("PLA_Obj_API_open", A)
+    pla.call("PLA_API_multi_axpy_matrix_to_global", 4, 3, _ptr(_ints(rsz)), _ptr(_ints(csz)), _d(-1.5), _ptr(buf),
+             M + 2, A, _ptr(_ints(rd)), _ptr(_ints(cd)))
+    loc = np.asfortranarray(rng.standard_normal((M + 2, N)))
+    loc0 = loc.copy()
+    pla.call("PLA_API_multi_axpy_global_to_matrix", 4, 3, _ptr(_ints(rsz)), _ptr(_ints(csz)), _d(0.75), A,
+             _ptr(_ints(rd)), _ptr(_ints(cd)), _ptr(loc), M + 2)
+    pla.call("PLA_Obj_API_close", A)
+    _, ref_a = ora.api_multi(True, rsz, csz, -1.5, buf, a0, rd, cd)
+    ref_b, _ = ora.api_multi(False, rsz, csz, 0.75, loc0, ref_a, rd, cd)
+    assert np.array_equal(pla.get_contents(A), ref_a)
+    assert np.array_equal(loc, ref_b)
+    pla.free(A)
+
+
+@pytest.mark.parametrize("inc", [1, 3])
+def test_vector_and_multi_vector_axpy(api, ora, inc):
+    s = api
+    rng = np.random.default_rng(13 + inc)
+    n = 500
+    x0 = rng.standard_normal(n)
+    x = s.mvector(n)
+    pla.set_contents(x, x0)
+    pla.call("PLA_Obj_API_open", x)
+    v = rng.standard_normal(120 * inc)
+    pla.call("PLA_API_axpy_vector_to_global", 120, _d(2.0), _ptr(v), inc, x, 300)
+    sizes, displs = [10, 50, 0, 60], [0, 440, 5, 100]
+    mv = rng.standard_normal(120 * inc)
+    pla.call("PLA_API_multi_axpy_vector_to_global", 4, _ptr(_ints(sizes)), _d(-1.0), _ptr(mv), inc, x,
+             _ptr(_ints(displs)))
+    loc = rng.standard_normal(120 * inc)
+    loc0 = loc.copy()
+    pla.call("PLA_API_multi_axpy_global_to_vector", 4, _ptr(_ints(sizes)), _d(0.5), x, _ptr(_ints(displs)),
+             _ptr(loc), inc)
+    pla.call("PLA_Obj_API_close", x)
+    col = lambda a: a[::inc].reshape(-1, 1)
+    _, ref = ora.api_multi(True, [120], [1], 2.0, col(v), x0.reshape(-1, 1), [300], [0])
+    _, ref = ora.api_multi(True, sizes, [1], -1.0, col(mv), ref, displs, [0])
+    ref_loc, _ = ora.api_multi(False, sizes, [1], 0.5, col(loc0), ref, displs, [0])
+    assert np.array_equal(pla.get_contents(x), ref)
+    assert np.array_equal(loc[::inc], ref_loc[:, 0])
+    if inc > 1:  # elements between the strided ones untouched
+        assert np.array_equal(np.delete(loc, np.s_[::inc]), np.delete(loc0, np.s_[::inc]))
+    pla.free(x)
+
+
+def test_float_and_int_objects(api):
+    s = api
+    fa = s.matrix(50, 40, pla.MPI_FLOAT)
+    ia = s.matrix(50, 40, pla.MPI_INT)
+    pla.call("PLA_Obj_set_to_zero", fa)
+    pla.call("PLA_Obj_set_to_zero", ia)
+    pla.set_contents(ia, np.arange(2000).reshape(50, 40))
+    pla.call("PLA_Obj_API_open", fa)
+    pla.call("PLA_Obj_API_open", ia)
+    fb = np.asfortranarray(np.random.default_rng(1).standard_normal((10, 20)).astype(np.float32))
+    ib = np.asfortranarray(np.arange(200, dtype=np.int32).reshape(10, 20))
+    pla.call("PLA_API_axpy_matrix_to_global", 10, 20, ctypes.byref(ctypes.c_float(2.0)), _ptr(fb), 10, fa, 40, 20)
+    pla.call("PLA_API_axpy_matrix_to_global", 10, 20, ctypes.byref(ctypes.c_int(-3)), _ptr(ib), 10, ia, 0, 0)
+    iloc = np.zeros((10, 20), dtype=np.int32, order="F")
+    pla.call("PLA_API_axpy_global_to_matrix", 10, 20, ctypes.byref(ctypes.c_int(2)), ia, 0, 0, _ptr(iloc), 10)
+    pla.call("PLA_Obj_API_close", fa)
+    pla.call("PLA_Obj_API_close", ia)
+    fref = np.zeros((50, 40), np.float32)
+    fref[40:, 20:] += np.float32(2.0) * fb
+    assert np.array_equal(pla.get_contents(fa), fref)
+    iref = np.arange(2000, dtype=np.int32).reshape(50, 40)
+    iref[:10, :20] += -3 * ib
+    assert np.array_equal(pla.get_contents(ia), iref)
+    assert np.array_equal(iloc, 2 * iref[:10, :20])
+    pla.free(fa, ia)
+
+
+def test_host_mirror_sees_closed_adds(api):
+    """A local-buffer mirror taken before the adds holds them after close."""
+    s = api
+    A = pla.from_numpy(s, np.ones((20, 20)))
+    mirror = ctypes.c_void_p()
+    pla.call("PLA_Obj_local_buffer", A, ctypes.byref(mirror))
+    pla.call("PLA_Obj_API_open", A)
+    buf = np.full((20, 20), 2.0, order="F")
+    pla.call("PLA_API_axpy_matrix_to_global", 20, 20, None, _ptr(buf), 20, A, 0, 0)  # NULL alpha: 1
+    pla.call("PLA_Obj_API_close", A)
+    host = np.ctypeslib.as_array(ctypes.cast(mirror, ctypes.POINTER(ctypes.c_double)), shape=(400,))
+    assert np.array_equal(host, np.full(400, 3.0))
+    pla.free(A)
+
+
+def test_states_and_errors(pla_session):
+    s = pla_session
+    L = pla.lib()
+    A, S = s.matrix(10, 10), s.mscalar()
+    state, mode = ctypes.c_int(), ctypes.c_int()
+    pla.call("PLA_API_state", ctypes.byref(state))
+    assert state.value == pla.PLA_STATE_NORMAL
+    assert L.PLA_API_end() == pla.PLA_ERR_INVALID_PARAM            # API not active
+    assert L.PLA_Obj_API_open(A) == pla.PLA_ERR_INVALID_PARAM      # API not active
+    pla.call("PLA_API_begin")
+    assert L.PLA_API_begin() == pla.PLA_ERR_INVALID_PARAM          # already active
+    pla.call("PLA_API_state", ctypes.byref(state))
+    assert state.value == pla.PLA_STATE_API_ACTIVE
+    buf = np.zeros((10, 10), order="F")
+    # object not open
+    assert L.PLA_API_axpy_matrix_to_global(2, 2, None, _ptr(buf), 10, A, 0, 0) == pla.PLA_ERR_INVALID_PARAM
+    pla.call("PLA_Obj_API_open", A)
+    pla.call("PLA_Obj_API_mode", A, ctypes.byref(mode))
+    assert mode.value == pla.PLA_MODE_OPEN
+    assert L.PLA_Obj_API_open(A) == pla.PLA_ERR_INVALID_PARAM      # already open
+    assert L.PLA_API_axpy_matrix_to_global(5, 5, None, _ptr(buf), 10, A, 6, 0) == pla.PLA_ERR_DIMENSIONS
+    assert L.PLA_API_axpy_global_to_matrix(5, 5, None, A, 0, -1, _ptr(buf), 10) == pla.PLA_ERR_DIMENSIONS
+    assert L.PLA_API_axpy_matrix_to_global(5, 5, None, _ptr(buf), 4, A, 0, 0) == pla.PLA_ERR_INVALID_PARAM
+    pla.call("PLA_Obj_API_open", S)
+    assert L.PLA_API_axpy_matrix_to_global(1, 1, None, _ptr(buf), 1, S, 0, 0) == pla.PLA_ERR_INVALID_PARAM
+    pla.call("PLA_Obj_API_close", S)
+    assert L.PLA_API_end() == pla.PLA_ERR_INVALID_PARAM            # A still open
+    pla.call("PLA_Obj_API_close", A)
+    assert L.PLA_Obj_API_close(A) == pla.PLA_ERR_INVALID_PARAM     # not open
+    pla.call("PLA_Obj_API_mode", A, ctypes.byref(mode))
+    assert mode.value == pla.PLA_MODE_CLOSED
+    pla.call("PLA_API_end")
+    assert np.array_equal(pla.get_contents(A), np.zeros((10, 10)))  # failed calls changed nothing
+    pla.free(A, S)

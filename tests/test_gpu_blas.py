@@ -1,6 +1,7 @@
 """GPU parity of the building blocks beneath PLA_LU / PLA_Chol, called through
 the C ABI: the drand48 fill (bit-exact against the oracle), PLA_Gemm (FP64
-MFMA kernel, every transpose case and ragged edges), PLA_Trsm / PLA_Trsv,
+MFMA kernel, every transpose case and ragged edges), PLA_Syrk, PLA_Trsm (both
+sides, every uplo / trans / diag) / PLA_Trsv,
 PLA_Apply_pivots_to_rows (bit-exact), views and copies.
 
 Tolerances: FP64 results agree with the oracle to c * k * eps * scale, the
@@ -105,19 +106,96 @@ def test_trsm_left_vs_oracle(pla_session, ora, uplo, trans, diag, n, nrhs):
     pla.free(T, B, al)
 
 
-def test_trsm_right_lower_trans_vs_oracle(pla_session, ora):
+@pytest.mark.parametrize("uplo", [pla.PLA_LOWER_TRIANGULAR, pla.PLA_UPPER_TRIANGULAR])
+@pytest.mark.parametrize("trans", [pla.PLA_NO_TRANSPOSE, pla.PLA_TRANSPOSE])
+@pytest.mark.parametrize("diag", [pla.PLA_UNIT_DIAG, pla.PLA_NONUNIT_DIAG])
+@pytest.mark.parametrize("m,n", [(90, 150), (1, 64), (300, 65), (513, 200)])
+def test_trsm_right_vs_oracle(pla_session, ora, uplo, trans, diag, m, n):
+    """B <- alpha B op(T)^-1: 64-column leaves (one thread per row) under a
+    column recursion whose updates are the FP64 GEMM."""
     s = pla_session
-    rng = np.random.default_rng(3)
-    n, m = 150, 90
-    l = np.tril(rng.uniform(-1, 1, (n, n))) / n + np.eye(n)
-    x = rng.uniform(-1, 1, (m, n))
-    L, X, al = _obj(s, l), _obj(s, x), s.scalar(1.0)
-    pla.call("PLA_Trsm", pla.PLA_SIDE_RIGHT, pla.PLA_LOWER_TRIANGULAR, pla.PLA_TRANSPOSE,
-             pla.PLA_NONUNIT_DIAG, al, L, X)
-    got = pla.get_contents(X)
-    ref = ora.trsm_left("L", "N", "N", l, x.T).T  # X L^T = B  <=>  L X^T = B^T
-    assert np.abs(got - ref).max() <= 64 * n * EPS
-    pla.free(L, X, al)
+    rng = np.random.default_rng(m * 5 + n)
+    t = rng.uniform(-1, 1, (n, n)) / n + np.eye(n)
+    b = rng.uniform(-1, 1, (m, n))
+    T, B, al = _obj(s, t), _obj(s, b), s.scalar(-0.5)
+    pla.call("PLA_Trsm", pla.PLA_SIDE_RIGHT, uplo, trans, diag, al, T, B)
+    got = pla.get_contents(B)
+    ref = ora.trsm_right("L" if uplo == pla.PLA_LOWER_TRIANGULAR else "U",
+                         "N" if trans == pla.PLA_NO_TRANSPOSE else "T",
+                         "U" if diag == pla.PLA_UNIT_DIAG else "N", t, b, alpha=-0.5)
+    assert np.abs(got - ref).max() <= 64 * n * EPS * max(1.0, np.abs(ref).max())
+    pla.free(T, B, al)
+
+
+def test_trsm_right_reads_only_its_triangle(pla_session, ora):
+    """NaN in the strict opposite triangle and (unit) diagonal of T is never read."""
+    s = pla_session
+    rng = np.random.default_rng(8)
+    n, m = 130, 70
+    t = rng.uniform(-1, 1, (n, n)) / n + np.eye(n)
+    b = rng.uniform(-1, 1, (m, n))
+    ref = ora.trsm_right("U", "T", "U", t, b)
+    t[np.tril(np.ones((n, n), bool), 0)] = np.nan
+    T, B = _obj(s, t), _obj(s, b)
+    pla.call("PLA_Trsm", pla.PLA_SIDE_RIGHT, pla.PLA_UPPER_TRIANGULAR, pla.PLA_TRANSPOSE, pla.PLA_UNIT_DIAG,
+             None, T, B)
+    got = pla.get_contents(B)
+    assert np.isfinite(got).all()
+    assert np.abs(got - ref).max() <= 64 * n * EPS * max(1.0, np.abs(ref).max())
+    pla.free(T, B)
+
+
+@pytest.mark.parametrize("uplo", [pla.PLA_LOWER_TRIANGULAR, pla.PLA_UPPER_TRIANGULAR])
+@pytest.mark.parametrize("trans", [pla.PLA_NO_TRANSPOSE, pla.PLA_TRANSPOSE])
+@pytest.mark.parametrize("n,k,alpha,beta", [(1, 1, 1.0, 1.0), (100, 33, 0.75, -2.0), (257, 300, -1.0, 1.0),
+                                            (640, 200, 1.0, 0.5), (384, 0, 1.0, 3.0), (130, 64, -1.0, 0.0)])
+def test_syrk_vs_oracle(pla_session, ora, uplo, trans, n, k, alpha, beta):
+    """PLA_Syrk: all of C scaled by beta, the uplo triangle += alpha op(A) op(A)^T;
+    n = 640 / 257 put interior 128 x 128 tiles on the alpha = +-1 fast path."""
+    s = pla_session
+    rng = np.random.default_rng(n + 3 * k)
+    a = rng.uniform(-1, 1, (n, k) if trans == pla.PLA_NO_TRANSPOSE else (k, n))
+    c = rng.uniform(-1, 1, (n, n))
+    A, C, al, be = _obj(s, a), _obj(s, c), s.scalar(alpha), s.scalar(beta)
+    pla.call("PLA_Syrk", uplo, trans, al, A, be, C)
+    got = pla.get_contents(C)
+    lo = uplo == pla.PLA_LOWER_TRIANGULAR
+    ref = ora.syrk("L" if lo else "U", "N" if trans == pla.PLA_NO_TRANSPOSE else "T", alpha, a, beta, c)
+    tri = np.tril(np.ones((n, n), bool)) if lo else np.triu(np.ones((n, n), bool))
+    assert np.abs(got[tri] - ref[tri]).max() <= 8 * (k + 1) * EPS * max(1.0, np.abs(ref).max())
+    assert np.array_equal(got[~tri], ref[~tri])  # beta * C, a single rounding
+    pla.free(A, C, al, be)
+
+
+def test_syrk_on_a_view_leaves_outside_untouched(pla_session, ora):
+    """The Cholesky trailing-update shape: C = A22 inside a larger matrix, A = A21."""
+    s = pla_session
+    rng = np.random.default_rng(9)
+    full = rng.uniform(-1, 1, (700, 700))
+    F = _obj(s, full)
+    tl, tr, bl, br = (ctypes.c_void_p() for _ in range(4))
+    pla.call("PLA_Obj_split_4", F, 300, 300, ctypes.byref(tl), ctypes.byref(tr), ctypes.byref(bl),
+             ctypes.byref(br))
+    mone, one = s.scalar(-1.0), s.scalar(1.0)
+    pla.call("PLA_Syrk", pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, mone, bl, one, br)
+    got = pla.get_contents(F)
+    ref = full.copy()
+    ref[300:, 300:] = ora.syrk("L", "N", -1.0, full[300:, :300], 1.0, full[300:, 300:])
+    assert np.abs(got - ref).max() <= 8 * 300 * EPS * 300
+    assert np.array_equal(got[:300, :], full[:300, :])
+    assert np.array_equal(np.triu(got[300:, 300:], 1), np.triu(full[300:, 300:], 1))
+    pla.free(tl, tr, bl, br, mone, one, F)
+
+
+def test_syrk_argument_errors(pla_session):
+    s = pla_session
+    A, C, R = s.matrix(10, 4), s.matrix(10, 10), s.matrix(10, 9)
+    one = s.scalar(1.0)
+    assert pla.lib().PLA_Syrk(pla.PLA_LOWER_TRIANGULAR, pla.PLA_TRANSPOSE, one, A, one, C) == pla.PLA_ERR_DIMENSIONS
+    assert pla.lib().PLA_Syrk(pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, one, A, one, R) == pla.PLA_ERR_DIMENSIONS
+    assert pla.lib().PLA_Syrk(12345, pla.PLA_NO_TRANSPOSE, one, A, one, C) == pla.PLA_ERR_INVALID_PARAM
+    assert pla.lib().PLA_Syrk(pla.PLA_UPPER_TRIANGULAR, 0, one, A, one, C) == pla.PLA_ERR_INVALID_PARAM
+    pla.free(A, C, R, one)
 
 
 @pytest.mark.parametrize("uplo", [pla.PLA_LOWER_TRIANGULAR, pla.PLA_UPPER_TRIANGULAR])

@@ -202,6 +202,79 @@ def test_trsm_left_vs_scipy(ora, uplo, trans, diag):
     assert np.allclose(x, ref, rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["U", "N"])
+def test_trsm_right_vs_scipy(ora, uplo, trans, diag):
+    """X op(T) = alpha B  <=>  op(T)^T X^T = alpha B^T."""
+    rng = np.random.default_rng(5)
+    n, m = 29, 7
+    t = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    b = rng.standard_normal((m, n))
+    x = ora.trsm_right(uplo, trans, diag, t, b, alpha=-1.5)
+    ref = scipy.linalg.solve_triangular(t, -1.5 * b.T, lower=(uplo == "L"),
+                                        trans="N" if trans == "T" else "T",
+                                        unit_diagonal=(diag == "U")).T
+    assert np.allclose(x, ref, rtol=1e-12, atol=1e-12)
+    # the strict opposite triangle of T is never read
+    t2 = t.copy()
+    mask = np.triu(np.ones((n, n), bool), 1) if uplo == "L" else np.tril(np.ones((n, n), bool), -1)
+    t2[mask] = np.nan
+    if diag == "U":
+        np.fill_diagonal(t2, np.nan)
+    assert np.array_equal(ora.trsm_right(uplo, trans, diag, t2, b, alpha=-1.5), x)
+
+
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_syrk_vs_numpy(ora, uplo, trans):
+    rng = np.random.default_rng(6)
+    n, k = 19, 11
+    a = rng.standard_normal((n, k) if trans == "N" else (k, n))
+    c = rng.standard_normal((n, n))
+    got = ora.syrk(uplo, trans, 0.75, a, -2.0, c)
+    opa = a if trans == "N" else a.T
+    full = 0.75 * opa @ opa.T - 2.0 * c
+    tri = np.tril(np.ones((n, n), bool)) if uplo == "L" else np.triu(np.ones((n, n), bool))
+    assert np.allclose(got[tri], full[tri], rtol=1e-13, atol=1e-13)
+    # PLA_Syrk_panpan scales all of C by beta before the triangle update
+    assert np.array_equal(got[~tri], -2.0 * c[~tri])
+    # beta = 0 is a multiply: NaN in C survives in the untouched triangle
+    c[~tri] = np.nan
+    got0 = ora.syrk(uplo, trans, 1.0, a, 0.0, c)
+    assert np.isnan(got0[~tri]).all()
+
+
+def test_api_multi_vs_numpy_slices(ora):
+    """Multi-block axpys: sub-block (i, j) of the local buffer starts at the
+    running sums of the sizes and lands at (rd[i], cd[j]); overlapping targets
+    receive every add in block order."""
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((40, 30))
+    buf = rng.standard_normal((12, 9))
+    rsz, csz, rd, cd = [5, 0, 7], [4, 5], [30, 2, 10], [20, 1]
+    nb, na = ora.api_multi(True, rsz, csz, -0.5, buf, a, rd, cd)
+    ref = a.copy()
+    ref[30:35, 20:24] += -0.5 * buf[0:5, 0:4]
+    ref[30:35, 1:6] += -0.5 * buf[0:5, 4:9]
+    ref[10:17, 20:24] += -0.5 * buf[5:12, 0:4]
+    ref[10:17, 1:6] += -0.5 * buf[5:12, 4:9]
+    assert np.array_equal(na, ref) and np.array_equal(nb, buf)
+    nb, na = ora.api_multi(False, rsz, csz, 2.0, buf, a, rd, cd)
+    refb = buf.copy()
+    refb[0:5, 0:4] += 2.0 * a[30:35, 20:24]
+    refb[0:5, 4:9] += 2.0 * a[30:35, 1:6]
+    refb[5:12, 0:4] += 2.0 * a[10:17, 20:24]
+    refb[5:12, 4:9] += 2.0 * a[10:17, 1:6]
+    assert np.array_equal(nb, refb) and np.array_equal(na, a)
+    # overlapping row targets: both blocks add into rows 3..5, in order
+    nb, na = ora.api_multi(True, [4, 4], [3], 1.0, buf[:8, :3], np.zeros((10, 5)), [2, 3], [1])
+    ref = np.zeros((10, 5))
+    ref[2:6, 1:4] += buf[0:4, 0:3]
+    ref[3:7, 1:4] += buf[4:8, 0:3]
+    assert np.array_equal(na, ref)
+
+
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 def test_gemm_vs_numpy(ora, ta, tb):
     rng = np.random.default_rng(4)
