@@ -276,11 +276,20 @@ int PLA_API_multi_axpy_global_to_matrix(int nsub_row, int nsub_col, int *size_ro
    relative to row j.  Returns PLA_SUCCESS; like the reference, a zero pivot
    is not flagged (the factors then hold inf/nan). */
 int PLA_LU(PLA_Obj A, PLA_Obj pivots);
+/* SOLVERS/LINPACK/PLA_LU_linpack.c:13: PLA_LU whose interchanges reach only
+   the columns from each panel on -- the multipliers of a panel keep the row
+   order of their own step (LINPACK's dgefa layout); on [A b] (n x (n+1)) the
+   last column ends as inv(L) P b, ready for PLA_Trsv with U */
+int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots);
 /* SOLVERS/Cholesky/PLA_Chol.c:13.  Lower only (the reference aborts on
    upper).  Returns 0, or the 1-based global index k+i+1 of the first
    non-positive pivot (PLA_Chol.c:181-185); the diagonal block that failed
    is left as it was before its factorisation, as in the reference. */
 int PLA_Chol(int uplo, PLA_Obj A);
+/* SOLVERS/Cholesky/PLA_Chol_simple.c:13: lower Cholesky with block size
+   nb_alg (<= 0: A unchanged); returns 0 or the 1-based index of the first
+   non-positive pivot, as PLA_Chol (the reference's variant always returns 0) */
+int PLA_Chol_simple(int nb_alg, PLA_Obj A);
 /* SOLVERS/LU/PLA_Apply_pivots_new.c:13 -- forward sequential row swaps */
 int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots);
 /* BLAS2/PLA_Trsv.c:15 */

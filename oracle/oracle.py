@@ -45,6 +45,7 @@ def lib():
         L.ora_idamax.argtypes = [_i, _p, _i]
         L.ora_lu_unblocked.argtypes = [_i, _i, _p, _i, _p]
         L.ora_lu_blocked.argtypes = [_i, _i, _p, _i, _i, _p]
+        L.ora_lu_linpack.argtypes = [_i, _i, _p, _i, _i, _p]
         L.ora_laswp.argtypes = [_i, _p, _i, _i, _i, _p]
         L.ora_chol_unblocked.restype = _i
         L.ora_chol_unblocked.argtypes = [_i, _p, _i]
@@ -107,6 +108,15 @@ def lu_blocked(a, nb=128):
     m, n = a.shape
     piv = np.zeros(min(m, n), dtype=np.int32)
     lib().ora_lu_blocked(m, n, _ptr(a), max(m, 1), nb, _ptr(piv))
+    return a, piv
+
+
+def lu_linpack(a, nb=128):
+    """PLA_LU_linpack (ora_lu_linpack): returns (factors, pivots)."""
+    a = _f(a).copy(order="F")
+    m, n = a.shape
+    piv = np.zeros(min(m, n), dtype=np.int32)
+    lib().ora_lu_linpack(m, n, _ptr(a), max(m, 1), nb, _ptr(piv))
     return a, piv
 
 

@@ -171,7 +171,10 @@ void ora_syrk(char uplo, char trans, int n, int k, double alpha, const double *a
 }
 
 /* PLA_LU_right, SOLVERS/LU/PLA_LU.c:37-212, on a 1x1 template. */
-void ora_lu_blocked(int m, int n, double *a, int lda, int nb, int *piv) {
+/* left = 1: PLA_LU_right; left = 0: PLA_LU_linpack_right
+   (SOLVERS/LINPACK/PLA_LU_linpack.c:27-171), whose interchanges reach only
+   A_BR, the columns from the panel on (:121-123). */
+static void lu_blocked(int m, int n, double *a, int lda, int nb, int *piv, int left) {
   int mn = m < n ? m : n;
   double *panel = (double *)malloc(sizeof(double) * (size_t)m * (size_t)(nb > 0 ? nb : 1));
   for (int k = 0; k < mn; k += nb) {
@@ -201,8 +204,12 @@ void ora_lu_blocked(int m, int n, double *a, int lda, int nb, int *piv) {
           for (int i = j + 1; i < rows; i++) pcc[i] += pc[i] * temp;
       }
     }
-    /* PLA_Apply_pivots_to_rows(A_B, piv): every column of rows k..m, :152 */
-    ora_laswp(n, &A_(k, 0), lda, 0, b, piv + k);
+    /* PLA_Apply_pivots_to_rows(A_B, piv): every column of rows k..m, :152
+       (LINPACK variant: columns k..n only) */
+    if (left)
+      ora_laswp(n, &A_(k, 0), lda, 0, b, piv + k);
+    else
+      ora_laswp(n - k, &A_(k, k), lda, 0, b, piv + k);
     /* panel write-back, :160-165 */
     for (int j = 0; j < b; j++)
       memcpy(&A_(k, k + j), panel + (size_t)j * ldp, sizeof(double) * rows);
@@ -218,6 +225,10 @@ void ora_lu_blocked(int m, int n, double *a, int lda, int nb, int *piv) {
   }
   free(panel);
 }
+
+void ora_lu_blocked(int m, int n, double *a, int lda, int nb, int *piv) { lu_blocked(m, n, a, lda, nb, piv, 1); }
+
+void ora_lu_linpack(int m, int n, double *a, int lda, int nb, int *piv) { lu_blocked(m, n, a, lda, nb, piv, 0); }
 
 /* ------------------------------------------------------------------ */
 /* Triangular solves                                                  */

@@ -17,6 +17,8 @@
  *   ora_srand48/ora_drand48/ora_fill_uniform : glibc srand48/drand48 via libc
  *   ora_lcg_skip                              : stepping the generator k times
  *   ora_idamax                                : numpy argmax(|x|) (first max)
+ *   ora_lu_linpack                            : ora_lu_blocked's U, pivots and
+ *       appended column bit for bit; its L after the later interchanges
  *   ora_lu_unblocked / ora_lu_blocked         : scipy LAPACK getrf pivots,
  *       exact rational elimination (tiny n, incl. exact ties), and the
  *       backward-error bound ||PA-LU|| / (n ||A|| eps) < 16
@@ -53,6 +55,9 @@ int ora_idamax(int n, const double *x, int incx);
 void ora_lu_unblocked(int m, int n, double *a, int lda, int *piv);
 /* PLA_LU_right: blocked right-looking LU (SOLVERS/LU/PLA_LU.c:37-212). */
 void ora_lu_blocked(int m, int n, double *a, int lda, int nb, int *piv);
+/* PLA_LU_linpack (SOLVERS/LINPACK/PLA_LU_linpack.c:27-171): as ora_lu_blocked,
+   but each panel's interchanges reach only the columns from the panel on. */
+void ora_lu_linpack(int m, int n, double *a, int lda, int nb, int *piv);
 
 /* PLA_Apply_pivots_to_rows: forward sequential interchanges of rows
    r0+i <-> r0+i+piv[i], i = 0..npiv-1, over ncols columns

@@ -129,6 +129,29 @@ int PLA_Chol(int uplo, PLA_Obj A) {
   return info;
 }
 
+// SOLVERS/Cholesky/PLA_Chol_simple.c:13: lower Cholesky with the caller's
+// algorithmic block size (the reference composes PLA_Local_chol, PLA_Trsm and
+// PLA_Syrk on views; the same three steps run here on the device path).
+// nb_alg <= 0 leaves A unchanged, as the reference's loop then ends at once.
+// A non-positive pivot returns its 1-based index, as PLA_Chol does (the
+// reference's simple variant returns 0 and leaves the block it failed on
+// factored as far as it got).
+int PLA_Chol_simple(int nb_alg, PLA_Obj A) {
+  if (!valid(A)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Chol_simple: invalid A");
+  if (A->base->datatype != MPI_DOUBLE)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol_simple: only MPI_DOUBLE matrices are supported");
+  const int n = A->length;
+  if (A->width != n) return fail(PLA_ERR_DIMENSIONS, "PLA_Chol_simple: A is %d x %d", A->length, A->width);
+  if (n == 0 || nb_alg <= 0) return PLA_SUCCESS;
+  double *a;
+  int rc = dev_ptr(A, (void **)&a);
+  if (rc) return rc;
+  int info = 0;
+  if ((rc = chol_factor(ctx().stream, n, a, ld(A), nb_alg, &info))) return rc;
+  if ((rc = written(A))) return rc;
+  return info;
+}
+
 // Factor A = L L^T, then solve L L^T X = B in place (PLA_Pos_def.c:69-90).
 // Returns the PLA_Chol failure index (and leaves B untouched) if A is not
 // positive definite.

@@ -41,7 +41,7 @@ struct lu_bufs {
 };
 
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
-                           long lda, int *piv, int nb, const lu_bufs &bf, hipEvent_t ev_ready,
+                           long lda, int *piv, int nb, bool swap_left, const lu_bufs &bf, hipEvent_t ev_ready,
                            hipEvent_t ev_panel, hipEvent_t ev_step);
 
 // Blocked right-looking LU of the m x n column-major device matrix a, with
@@ -53,7 +53,11 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
 //                       -> A12 <- inv(L11) A12 -> A22[:, next panel] -= A21 A12
 //                       -> signal -> A22[:, rest] -= A21 A12
 // look-ahead stream:    wait(signal) -> factor panel k+1 -> signal(panel k+1)
-int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, int nb) {
+//
+// swap_left = false is the LINPACK variant (SOLVERS/LINPACK/PLA_LU_linpack.c:
+// 121-123 interchanges only A_BR, the columns from the panel on): the columns
+// of L left of a panel keep the row order of their own step.
+int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, int nb, bool swap_left) {
   if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
   const int stride = leaf_pairs_stride();
@@ -77,7 +81,7 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
-  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, bf, ev_ready, ev_panel, ev_step);
+  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, swap_left, bf, ev_ready, ev_panel, ev_step);
   // the main stream waits for both streams, also after an error
   const int rc_end = lu_streams_end(sMain, sP, sM);
   if (rc) return rc;
@@ -91,7 +95,7 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
 // update reads: sL waits for the end of step k-1 on sM (ev_step) before
 // permuting them.  Step k's composite list is comp + k * cstride.
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
-                           long lda, int *piv, int nb, const lu_bufs &bf, hipEvent_t ev_ready,
+                           long lda, int *piv, int nb, bool swap_left, const lu_bufs &bf, hipEvent_t ev_ready,
                            hipEvent_t ev_panel, hipEvent_t ev_step) {
   int *const pairs = bf.pairs, *const comp = bf.comp;
   const size_t pstride = bf.pstride;
@@ -126,7 +130,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
     const int nr = n - k - b, mr = m - k - b;
     const int *cp = comp + (size_t)step * cstride;
-    if (split_left) {
+    if (split_left && swap_left) {
       PLA_HIP(hipStreamWaitEvent(sL, ev_panel, 0));
       if (k > 0) PLA_HIP(hipStreamWaitEvent(sL, ev_step, 0));  // step k-1 no longer reads panel k-1's L21
       if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
@@ -159,8 +163,8 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     }
     if ((rc = right_part(n1, nr - n1))) return rc;
     // left of the panel last: nothing reads L before the factorization ends
-    if (!split_left && (rc = k_panel_swap(sM, a + k, lda, k, 0, 0, cp))) return rc;
-    if (split_left) PLA_HIP(hipEventRecord(ev_step, sM));
+    if (swap_left && !split_left && (rc = k_panel_swap(sM, a + k, lda, k, 0, 0, cp))) return rc;
+    if (swap_left && split_left) PLA_HIP(hipEventRecord(ev_step, sM));
   }
   return PLA_SUCCESS;
 }
@@ -169,7 +173,7 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
 
 extern "C" {
 
-int PLA_LU(PLA_Obj A, PLA_Obj pivots) {
+static int lu_entry(PLA_Obj A, PLA_Obj pivots, bool swap_left) {
   int rc = check_lu_args(A, pivots);
   if (rc) return rc;
   const int m = A->length, n = A->width;
@@ -179,10 +183,18 @@ int PLA_LU(PLA_Obj A, PLA_Obj pivots) {
   int *piv;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(pivots, (void **)&piv))) return rc;
   const int nb = choose_nb(mn, NB_LU);
-  if ((rc = lu_factor(ctx().stream, m, n, a, ld(A), piv, nb))) return rc;
+  if ((rc = lu_factor(ctx().stream, m, n, a, ld(A), piv, nb, swap_left))) return rc;
   if ((rc = written(A))) return rc;
   return written(pivots);
 }
+
+int PLA_LU(PLA_Obj A, PLA_Obj pivots) { return lu_entry(A, pivots, true); }
+
+// SOLVERS/LINPACK/PLA_LU_linpack.c:13: the same U, pivots and trailing
+// columns as PLA_LU (on the n x (n+1) appended system [A b] the last column
+// ends as inv(L) P b); the multipliers of each panel stay in the row order of
+// their own step.
+int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots) { return lu_entry(A, pivots, false); }
 
 // Factor A (PLA_LU), then solve A X = B in place: interchanges on B, then the
 // unit-lower and upper triangular solves (PLA_General.c:80-111).

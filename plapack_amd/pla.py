@@ -130,6 +130,8 @@ _SIGS = {
     "PLA_Obj_vert_split_2": [_obj, _i, _pobj, _pobj],
     "PLA_Copy": [_obj, _obj],
     "PLA_LU": [_obj, _obj],
+    "PLA_LU_linpack": [_obj, _obj],
+    "PLA_Chol_simple": [_i, _obj],
     "PLA_Chol": [_i, _obj],
     "PLA_Apply_pivots_to_rows": [_obj, _obj],
     "PLA_Trsv": [_i, _i, _i, _obj, _obj],

@@ -79,6 +79,35 @@ def test_chol_failure_index_and_state(pla_session, ora, n, nb, bad):
     assert np.array_equal(np.triu(l, 1), np.triu(a, 1))
 
 
+@pytest.mark.parametrize("n,nb_alg", [(20, 4), (300, 32), (1000, 100), (777, 512)])
+def test_chol_simple_vs_oracle(pla_session, ora, n, nb_alg):
+    """PLA_Chol_simple (SOLVERS/Cholesky/PLA_Chol_simple.c:13) with the
+    caller's block size."""
+    a = _spd(n, n + 1)
+    A = pla.from_numpy(pla_session, a)
+    assert pla.call("PLA_Chol_simple", nb_alg, A) == 0
+    l = pla.get_contents(A)
+    lo, info_o = ora.chol_blocked(a, nb=nb_alg)
+    assert info_o == 0
+    assert np.abs(np.tril(l) - np.tril(lo)).max() <= 1e-13 * max(1.0, np.abs(a).max()) * np.sqrt(n)
+    assert np.array_equal(np.triu(l, 1), np.triu(a, 1))
+    pla.free(A)
+
+
+def test_chol_simple_edge_cases(pla_session, ora):
+    s = pla_session
+    a = _spd(50, 4)
+    A = pla.from_numpy(s, a)
+    assert pla.call("PLA_Chol_simple", 0, A) == 0  # nb_alg <= 0: the reference's loop ends at once
+    assert np.array_equal(pla.get_contents(A), a)
+    a[30, 30] = -1.0
+    pla.set_contents(A, a)
+    info = pla.call("PLA_Chol_simple", 8, A, ok=range(0, 1 << 30))
+    _, info_o = ora.chol_blocked(a, nb=8)
+    assert info == info_o and info > 0
+    pla.free(A)
+
+
 def test_chol_upper_is_unsupported(pla_session):
     A = pla.from_numpy(pla_session, np.eye(4))
     assert pla.lib().PLA_Chol(pla.PLA_UPPER_TRIANGULAR, A) < 0

@@ -66,6 +66,33 @@ def test_lu_shapes_vs_oracle(pla_session, ora, m, n, nb):
     assert ora.lu_scaled_residual(a, lu, piv) < 16
 
 
+@pytest.mark.parametrize("n,nb", [(300, 64), (1000, 128), (2049, 256)])
+def test_lu_linpack_appended_system(pla_session, ora, n, nb):
+    """PLA_LU_linpack on [A b] (SOLVERS/LINPACK/driver.c:108-157): pivots and
+    factors against the oracle's LINPACK variant, then PLA_Trsv with U on the
+    appended column solves A x = b."""
+    s = pla_session
+    ab = ora.uniform(n, n + 1, seed=n)
+    pla.call("PLA_Set_device_nb", nb)
+    A = pla.from_numpy(s, ab)
+    P = s.mvector(n, dtype=pla.MPI_INT)
+    pla.call("PLA_LU_linpack", A, P)
+    pla.call("PLA_Set_device_nb", 0)
+    f, piv = pla.get_contents(A), pla.get_contents(P)[:, 0]
+    f_o, piv_o = ora.lu_linpack(ab, nb=nb)
+    assert np.array_equal(piv, piv_o)
+    assert np.abs(f - f_o).max() <= 1e-7 * np.abs(ab).max()
+    # back substitution on the appended column, as the LINPACK driver does
+    tl, tr, bl, br = (ctypes.c_void_p() for _ in range(4))
+    pla.call("PLA_Obj_split_4", A, n, n, ctypes.byref(tl), ctypes.byref(tr), ctypes.byref(bl), ctypes.byref(br))
+    pla.call("PLA_Trsv", pla.PLA_UPPER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG, tl, tr)
+    x = pla.get_contents(tr)[:, 0]
+    a, b = ab[:, :n], ab[:, n]
+    eta = np.abs(a @ x - b).max() / (np.abs(a).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+    assert eta < 64 * n * np.finfo(np.float64).eps
+    pla.free(tl, tr, bl, br, A, P)
+
+
 def test_lu_zero_column_not_flagged(pla_session, ora):
     """Like the reference, a zero pivot is not detected: PLA_LU returns 0."""
     a = ora.uniform(64, 64, 2)

@@ -245,6 +245,35 @@ def test_syrk_vs_numpy(ora, uplo, trans):
     assert np.isnan(got0[~tri]).all()
 
 
+@pytest.mark.parametrize("m,n,nb", [(60, 61, 8), (50, 50, 16), (70, 40, 12)])
+def test_lu_linpack_vs_lu_blocked(ora, m, n, nb):
+    """The LINPACK variant: U, pivots and the columns right of the square part
+    equal PLA_LU's bit for bit; applying each later panel's interchanges to a
+    panel's multipliers gives PLA_LU's L; on [A b] the appended column is
+    inv(L) P b, so back substitution with U solves A x = b."""
+    a = ora.uniform(m, n, 21)
+    f1, p1 = ora.lu_blocked(a, nb)
+    f2, p2 = ora.lu_linpack(a, nb)
+    mn = min(m, n)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(np.triu(f1), np.triu(f2))
+    L2 = np.tril(f2[:, :mn], -1)
+    for k in range(0, mn, nb):
+        cols = slice(k, min(k + nb, mn))
+        blk = L2[:, cols]
+        for j in range(min(k + nb, mn), mn):  # later interchanges, in order
+            if p2[j]:
+                blk[[j, j + p2[j]]] = blk[[j + p2[j], j]]
+        L2[:, cols] = blk
+    assert np.array_equal(L2, np.tril(f1[:, :mn], -1))
+    if n == m + 1:
+        x = scipy.linalg.solve_triangular(np.triu(f2[:, :m]), f2[:, m])
+        assert np.abs(a[:, :m] @ x - a[:, m]).max() <= 1e-10 * np.abs(x).max()
+    # an interchange into the columns left of a panel would show in L's first block
+    if mn > nb and any(p2[nb:]):
+        assert not np.array_equal(f1[:, :nb], f2[:, :nb])
+
+
 def test_api_multi_vs_numpy_slices(ora):
     """Multi-block axpys: sub-block (i, j) of the local buffer starts at the
     running sums of the sizes and lands at (rd[i], cd[j]); overlapping targets
