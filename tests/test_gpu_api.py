@@ -11,6 +11,7 @@ with disjoint and overlapping targets, transfers larger than one staging slot,
 MPI_FLOAT / MPI_INT objects, host mirrors, and the state errors the reference
 aborts on."""
 import ctypes
+import subprocess
 
 import numpy as np
 import pytest
@@ -255,3 +256,20 @@ def test_states_and_errors(pla_session):
     pla.call("PLA_API_end")
     assert np.array_equal(pla.get_contents(A), np.zeros((10, 10)))  # failed calls changed nothing
     pla.free(A, S)
+
+
+def test_example_style_c_driver(tmp_path, ora):
+    """tests/drivers/test_api_driver.c: a C program filling A and b through the
+    API as EXAMPLES/LU/create_problem.c does, solving with PLA_General_solve and
+    reading x back as EXAMPLES/LU/process_answer.c does."""
+    from test_gpu_lu import _driver_exe
+    exe = _driver_exe("test_api_driver")
+    n = 1000
+    xfile = str(tmp_path / "x.bin")
+    out = subprocess.run([exe, str(n), xfile], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    resid = float(out.stdout.split("scaled residual")[1])
+    assert resid < 16 * n * np.finfo(np.float64).eps
+    x = np.fromfile(xfile, dtype=np.float64)
+    ab = ora.uniform(n, n + 1, 0)  # the driver's drand48 stream: A, then b
+    assert np.abs(x - np.linalg.solve(ab[:, :n], ab[:, n])).max() <= 1e-8 * np.abs(x).max()

@@ -178,16 +178,17 @@ def test_lu_solve_large_backward_error(pla_session, n, nb, repeat):
     pla.free(A, A0, ones, b, x, r, P, one, zero, mone, nrm)
 
 
-def _driver_exe():
+def _driver_exe(name="test_lu_driver"):
+    """Compile tests/drivers/<name>.c with mpicc against include/PLA.h and the library."""
     mpicc = shutil.which("mpicc") or "/opt/conda/bin/mpicc"
     if not os.path.exists(mpicc):
         pytest.skip("mpicc not available")
     out = os.path.join(ROOT, "tests", "_build")
     os.makedirs(out, exist_ok=True)
-    exe = os.path.join(out, "test_lu_driver")
+    exe = os.path.join(out, name)
     libdir = os.path.join(ROOT, "plapack_amd")
     subprocess.run([mpicc, "-O2", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(ROOT, "tests", "drivers", "test_lu_driver.c"), "-o", exe,
+                    os.path.join(ROOT, "tests", "drivers", name + ".c"), "-o", exe, "-lm",
                     # the system libstdc++ (needed by the HIP runtime) ahead of MPICH's own
                     "-L/usr/lib/x86_64-linux-gnu", "-Wl,-rpath-link,/usr/lib/x86_64-linux-gnu",
                     "-L", libdir, "-lpla_mi355x", "-Wl,-rpath," + libdir,
