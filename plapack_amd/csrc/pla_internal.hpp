@@ -221,14 +221,16 @@ int trsm_leaf_size();
 
 // ---- mixed precision and FP64-accurate solves (pla_mixed.hip) ----
 int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *comp);
-int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long lda,
-            const void *Bh, const void *Bl, long ldb, float *C, long ldc);
+// C (FP32) -= A B with A and B in the packed k-slice layout (pla_mixed.hip):
+// A rows 0..m-1 (slice stride sa), B columns boff..boff+n-1 (slice stride sb)
+int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long sa, const void *Bh,
+            const void *Bl, long sb, long boff, float *C, long ldc);
 int k_cvt_f64_f32(hipStream_t s, int m, int n, const double *A, long lda, float *W, long ldw);
 int k_cvt_f32_f64(hipStream_t s, int m, int n, const float *W, long ldw, double *P, long ldp);
-int k_panel_out(hipStream_t s, int prec, int rows, int b, const double *P, long ldp, void *Lh, void *Ll,
-                int ldl);
-int k_upanel_out(hipStream_t s, int prec, int b, int nr, const double *U, long ldu, float *W, long ldw,
-                 void *Uh, void *Ul, int ldh);
+int k_panel_out(hipStream_t s, int prec, int rows, int b, int kpad, const double *P, long ldp, void *Lh, void *Ll,
+                long sl);
+int k_upanel_out(hipStream_t s, int prec, int b, int kpad, int nr, const double *U, long ldu, float *W, long ldw,
+                 void *Uh, void *Ul, long su, long col0);
 // Sync-free blocked triangular solve op(A) X = X, X n x nrhs (nrhs <= 16).
 // Dinv: inverted diagonal blocks (nullptr: computed into a workspace).
 int trsv_sf(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *A, long lda,

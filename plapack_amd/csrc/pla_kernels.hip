@@ -18,7 +18,7 @@
 //                     reciprocal-multiply, PLA_Local_ger a rank-1 update.
 //   laswp_kernel      PLA_Apply_pivots_to_rows (PLA_Apply_pivots_new.c:13-179)
 //   trsm_leaf_kernel  PLA_Local_trsm / dtrsm (Local_BLAS/PLA_Local_trsm.c:67),
-//                     left side, one thread per right-hand side column.
+//                     left side, a wave per group of columns, a lane per row.
 //   trsm_right_leaf_kernel  dtrsm("R", ...) of PLA_Local_trsm and of
 //                     PLA_Chol.c:193-195 ("R","L","T","N"), one thread per row.
 //   potrf_leaf_kernel PLA_dchol (SOLVERS/Cholesky/PLA_Local_chol.c:103-143)
@@ -849,11 +849,28 @@ __global__ __launch_bounds__(256) void laswp_kernel(double *__restrict__ A, long
 }
 
 // ---------------------------------------------------------------------------
-// Left triangular solve of a b x b block (b <= 64), one thread per column of
-// B: B <- op(T)^{-1} B.  The triangle is staged in LDS.  Operation order per
-// element follows the netlib dtrsm loops (column axpys for NoTrans, dot
-// products for Trans; a divide by the diagonal for NonUnit).
+// Triangular-solve leaves: blocks of at most 64 rows / columns.
 constexpr int TRSM_LEAF = 64;
+
+// Left-side triangular solve of a b x b leaf (b <= 64): B <- op(T)^-1 B for
+// ncols columns.  A wave takes NC columns at a time with lane i owning row i
+// of each, so every column is read and written as one coalesced 512-byte
+// row of lanes (a thread per column would touch one cache line -- and for a
+// wide matrix one page -- per lane and element).  Lane i keeps row i of
+// M = op(T) in registers (staged through LDS); the solve is dtrsm's column
+// order: at step k lane k finalizes x_k (divided by M(k,k) unless unit), it
+// is broadcast with v_readlane, and the lanes of the rows still open
+// subtract M(i,k) x_k.  Forward (lower-type M) takes k upwards, backward
+// downwards; each row receives its updates in the same order, with the same
+// fused multiply-adds, as a dot product over k would.
+constexpr int TRSM_NC = 4;
+
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 template <bool LOWER, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(256) void trsm_leaf_kernel(int b, int ncols, const double *__restrict__ T,
@@ -866,41 +883,44 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int b, int ncols, const 
     sT[i][kcol] = T[(long)kcol * ldt + i];
   }
   __syncthreads();
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncols) return;
-  double *x = B + (long)c * ldb;
-  double v[TRSM_LEAF];
+  constexpr bool FORWARD = (LOWER != TRANS);  // op(T) lower-type
+  const int lane = threadIdx.x & 63;
+  // row `lane` of M = op(T), strictly on the solved side of the diagonal
+  double mrow[TRSM_LEAF];
 #pragma unroll
-  for (int i = 0; i < TRSM_LEAF; i++) v[i] = (i < b) ? x[i] : 0.0;
-  // effective matrix M = op(T): lower-type (forward) when LOWER != TRANS
-  constexpr bool FORWARD = (LOWER != TRANS);
-  if (FORWARD) {
-#pragma unroll
-    for (int i = 0; i < TRSM_LEAF; i++) {
-      if (i < b) {
-        double t = v[i];
-#pragma unroll
-        for (int kk = 0; kk < i; kk++) t = fma(-(TRANS ? sT[kk][i] : sT[i][kk]), v[kk], t);
-        if (!UNIT) t = t / sT[i][i];
-        v[i] = t;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int ii = TRSM_LEAF - 1; ii >= 0; ii--) {
-      if (ii < b) {
-        double t = v[ii];
-#pragma unroll
-        for (int kk = TRSM_LEAF - 1; kk > ii; kk--)
-          if (kk < b) t = fma(-(TRANS ? sT[kk][ii] : sT[ii][kk]), v[kk], t);
-        if (!UNIT) t = t / sT[ii][ii];
-        v[ii] = t;
-      }
-    }
+  for (int k = 0; k < TRSM_LEAF; k++) {
+    const bool in = lane < b && k < b && (FORWARD ? k < lane : k > lane);
+    mrow[k] = in ? (TRANS ? sT[k][lane] : sT[lane][k]) : 0.0;
   }
+  const double diag = (!UNIT && lane < b) ? sT[lane][lane] : 1.0;
+  const int nwaves = gridDim.x * 4;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  for (int c0 = wave * TRSM_NC; c0 < ncols; c0 += nwaves * TRSM_NC) {
+    double x[TRSM_NC];
 #pragma unroll
-  for (int i = 0; i < TRSM_LEAF; i++)
-    if (i < b) x[i] = v[i];
+    for (int q = 0; q < TRSM_NC; q++)
+      x[q] = (c0 + q < ncols && lane < b) ? B[(long)(c0 + q) * ldb + lane] : 0.0;
+#pragma unroll
+    for (int s = 0; s < TRSM_LEAF; s++) {
+      const int k = FORWARD ? s : TRSM_LEAF - 1 - s;
+      if (k < b) {
+        if (!UNIT && lane == k) {
+#pragma unroll
+          for (int q = 0; q < TRSM_NC; q++) x[q] = x[q] / diag;
+        }
+        double xk[TRSM_NC];
+#pragma unroll
+        for (int q = 0; q < TRSM_NC; q++) xk[q] = lane_bcast(x[q], k);
+        if (FORWARD ? lane > k : lane < k) {
+#pragma unroll
+          for (int q = 0; q < TRSM_NC; q++) x[q] = fma(-mrow[k], xk[q], x[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TRSM_NC; q++)
+      if (c0 + q < ncols && lane < b) B[(long)(c0 + q) * ldb + lane] = x[q];
+  }
 }
 
 // Right-side triangular solve of a b x b leaf (b <= 64): X op(T) = X0 for
@@ -1228,7 +1248,11 @@ int k_trsm_leaf(hipStream_t s, bool lower, bool trans, bool unit, int b, int nco
                 long ldt, double *B, long ldb, const int *abort_flag) {
   if (b <= 0 || ncols <= 0) return PLA_SUCCESS;
   if (b > TRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "trsm leaf: b = %d > %d", b, TRSM_LEAF);
-  dim3 grid((ncols + 255) / 256), blk(256);
+  // four waves per workgroup, TRSM_NC columns per wave and pass; at most 512
+  // workgroups (grid-stride beyond)
+  const long groups = ((long)ncols + TRSM_NC - 1) / TRSM_NC;
+  const long wgs = (groups + 3) / 4;
+  dim3 grid((unsigned)(wgs < 512 ? wgs : 512)), blk(256);
 #define PLA_TRSM_CASE(LO, TR, UN)                                                                     \
   if (lower == LO && trans == TR && unit == UN) {                                                     \
     hipLaunchKernelGGL((trsm_leaf_kernel<LO, TR, UN>), grid, blk, 0, s, b, ncols, T, ldt, B, ldb,     \

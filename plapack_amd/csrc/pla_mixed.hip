@@ -63,25 +63,38 @@ __device__ inline f4v mfma16(bhalf8 a, bhalf8 b, f4v c) {
 
 // ---------------------------------------------------------------------------
 // C (m x n FP32, column-major) -= A (m x k) * B (k x n), A and B in FP16/BF16
-// with k contiguous: row i of A at Ah + i*lda, column j of B at Bh + j*ldb.
-// k is a multiple of 32; lda, ldb multiples of 8.  128 x 128 block tile, 4
-// waves of 64 x 64 as 4 x 4 MFMA 16x16x32 tiles, K staged 32 at a time
-// through double-buffered LDS.  An LDS row is 32 elements (64 bytes, four
-// 16-byte chunks); chunk c of row r sits at chunk c ^ ((r >> 1) & 3), which
-// makes the fragment reads (16 lanes: 16 rows, one chunk) and the staging
-// writes (4 lanes per row) free of bank conflicts -- measured on MI355X with
-// SQ_LDS_BANK_CONFLICT, where rows padded to 40 elements cost 4 conflict
-// cycles per read.  MFMA 16x16x32 layout (lane l): A row l%16,
+// in the packed k-slice layout below.  128 x 128 block tile, 4 waves of
+// 64 x 64 as 4 x 4 MFMA 16x16x32 tiles, K staged 32 at a time through
+// double-buffered LDS.  MFMA 16x16x32 layout (lane l): A row l%16,
 // k = 8*(l/16)..+7; B column l%16, same k; result lane l holds
 // D[4*(l/16) + r][l%16].
+//
+// Packed k-slice layout of an operand with `index` rows of A (columns of B):
+// k-slice t (k = 32t .. 32t+31) is a block of slice_stride elements; inside
+// it, index r owns 32 consecutive elements (64 bytes, four 16-byte chunks),
+// chunk c (k = 32t + 8c ..) stored at chunk position c ^ ((r >> 1) & 3).  A
+// 128-index x 32-k tile is then one contiguous 8 KB block: the staging loads
+// are fully used 128-byte lines, and its image in LDS, chunk for chunk, is
+// the swizzled tile whose fragment reads (16 lanes: 16 indices, one chunk)
+// and staging writes are free of bank conflicts (measured with
+// SQ_LDS_BANK_CONFLICT; rows padded to 40 elements cost 4 conflict cycles per
+// read).
 constexpr int HBM = 128, HBN = 128, HKS = 32;
 
+// Offset of element (index r, depth kk) in the packed k-slice layout.
+__device__ __forceinline__ long kslice_off(long r, int kk, long slice_stride) {
+  const int c = (kk >> 3) & 3;
+  return (long)(kk >> 5) * slice_stride + r * 32 + ((c ^ (int)((r >> 1) & 3)) << 3) + (kk & 7);
+}
+
+// A: rows 0..m-1 of a packed operand (slice stride sa); B: columns
+// boff..boff+n-1 of a packed operand (slice stride sb, boff a multiple of 8).
 template <typename H, bool SPLIT>
 __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, const H *__restrict__ Ah,
-                                                       const H *__restrict__ Al, long lda,
+                                                       const H *__restrict__ Al, long sa,
                                                        const H *__restrict__ Bh, const H *__restrict__ Bl,
-                                                       long ldb, float *__restrict__ C, long ldc, int tiles_m,
-                                                       int tiles_n, int vec_ok) {
+                                                       long sb, long boff, float *__restrict__ C, long ldc,
+                                                       int tiles_m, int tiles_n, int vec_ok) {
   typedef typename hvec<H>::type V;
   constexpr int NS = SPLIT ? 2 : 1;
   __shared__ V As[2][NS][HBM * 4];
@@ -109,34 +122,36 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
     for (int bj = 0; bj < 4; bj++) acc[bi][bj] = f4v{0.f, 0.f, 0.f, 0.f};
 
   // Global loads run one K step ahead of the MFMAs (staged through registers).
+  // Thread tid moves 16-byte chunks tid and tid + 256 of each 8 KB tile; the
+  // chunk's LDS slot is its position in the tile.
   V ra[NS][2], rb[NS][2];
   const V zero = V{};
   auto load_tile = [&](int kt) {
-    const int kb = kt * HKS;
 #pragma unroll
     for (int e = 0; e < 2; e++) {
-      const int c = tid + 256 * e, row = c >> 2, kc = c & 3;
+      const int c = tid + 256 * e, row = c >> 2;
       const int gi = i0 + row, gj = j0 + row;
+      const long ao = (long)kt * sa + (long)i0 * 32 + c * 8;
+      const long bo = (long)kt * sb + (boff + j0) * 32 + c * 8;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         const H *Ap = s ? Al : Ah;
         const H *Bp = s ? Bl : Bh;
-        ra[s][e] = gi < m ? *(const V *)(Ap + (long)gi * lda + kb + kc * 8) : zero;
-        rb[s][e] = gj < n ? *(const V *)(Bp + (long)gj * ldb + kb + kc * 8) : zero;
+        ra[s][e] = gi < m ? *(const V *)(Ap + ao) : zero;
+        rb[s][e] = gj < n ? *(const V *)(Bp + bo) : zero;
       }
     }
   };
-  // the swizzle depends on the lane only: staging rows (tid >> 2) + 64 e,
-  // fragment rows 16 bi + l16 (+ 64 wm / wn)
-  const int wchunk = (tid >> 2) * 4 + ((tid & 3) ^ ((tid >> 3) & 3));
+  // fragment rows 16 bi + l16 (+ 64 wm / wn): chunk lq of the row sits at
+  // lq ^ ((l16 >> 1) & 3) (tile rows start at multiples of 8)
   const int rchunk = l16 * 4 + (lq ^ ((l16 >> 1) & 3));
   auto store_tile = [&](int buf) {
 #pragma unroll
     for (int e = 0; e < 2; e++) {
 #pragma unroll
       for (int s = 0; s < NS; s++) {
-        As[buf][s][wchunk + 256 * e] = ra[s][e];
-        Bs[buf][s][wchunk + 256 * e] = rb[s][e];
+        As[buf][s][tid + 256 * e] = ra[s][e];
+        Bs[buf][s][tid + 256 * e] = rb[s][e];
       }
     }
   };
@@ -224,35 +239,40 @@ __device__ inline void to_low(float v, H *hi, H *lo, long idx) {
   if (SPLIT) lo[idx] = (H)(v - (float)h);
 }
 
-// Rows of a factored panel below its diagonal block (P: FP64, rows x b, ldp)
-// as a row-major low-precision copy Lh[i*ldl + c], rounded through FP32 as
-// stored in W; columns b..ldl-1 are zero.
+// Rows of a factored panel below its diagonal block (P: FP64, rows x b, ldp),
+// rounded through FP32 as stored in W, as a packed k-slice copy (index = row,
+// depth = column; slice stride sl; depths b .. kpad-1 zero).  Consecutive
+// threads write consecutive packed elements.
 template <typename H, bool SPLIT>
-__global__ __launch_bounds__(256) void panel_out_kernel(int rows, int b, const double *__restrict__ P, long ldp,
-                                                        H *__restrict__ Lh, H *__restrict__ Ll, int ldl) {
-  const long total = (long)rows * ldl;
+__global__ __launch_bounds__(256) void panel_out_kernel(int rows, int b, int kpad, const double *__restrict__ P,
+                                                        long ldp, H *__restrict__ Lh, H *__restrict__ Ll, long sl) {
+  const long per_slice = (long)rows * 32, total = per_slice * (kpad / 32);
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const int i = (int)(e / ldl), c = (int)(e - (long)i * ldl);
+    const int t = (int)(e / per_slice);
+    const long w = e - (long)t * per_slice;
+    const int i = (int)(w >> 5), pos = (int)(w & 31);
+    const int c = t * 32 + ((((pos >> 3) ^ ((i >> 1) & 3))) << 3) + (pos & 7);
     const float v = c < b ? (float)P[(long)c * ldp + i] : 0.f;
-    to_low<H, SPLIT>(v, Lh, Ll, (long)i * ldl + c);
+    to_low<H, SPLIT>(v, Lh, Ll, (long)t * sl + w);
   }
 }
 
-// U12 (FP64, b x nr, ldu) back into W (FP32) and a column-major low-precision
-// copy Uh[j*ldh + r] (rows b..ldh-1 zero).
+// U12 (FP64, b x nr, ldu) back into W (FP32) and as a packed k-slice copy of
+// columns col0 .. col0 + nr - 1 (index = column, depth = row; slice stride
+// su; depths b .. kpad-1 zero).
 template <typename H, bool SPLIT>
-__global__ __launch_bounds__(256) void upanel_out_kernel(int b, int nr, const double *__restrict__ U, long ldu,
-                                                         float *__restrict__ W, long ldw, H *__restrict__ Uh,
-                                                         H *__restrict__ Ul, int ldh) {
-  const long total = (long)nr * ldh;
+__global__ __launch_bounds__(256) void upanel_out_kernel(int b, int kpad, int nr, const double *__restrict__ U,
+                                                         long ldu, float *__restrict__ W, long ldw,
+                                                         H *__restrict__ Uh, H *__restrict__ Ul, long su, long col0) {
+  const long total = (long)nr * kpad;
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const int j = (int)(e / ldh), r = (int)(e - (long)j * ldh);
+    const int j = (int)(e / kpad), r = (int)(e - (long)j * kpad);
     float v = 0.f;
     if (r < b) {
       v = (float)U[(long)j * ldu + r];
       W[(long)j * ldw + r] = v;
     }
-    to_low<H, SPLIT>(v, Uh, Ul, (long)j * ldh + r);
+    to_low<H, SPLIT>(v, Uh, Ul, kslice_off(col0 + j, r, su));
   }
 }
 
@@ -664,38 +684,27 @@ namespace pla {
 
 // ---- launchers ---------------------------------------------------------------
 
-int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long lda,
-            const void *Bh, const void *Bl, long ldb, float *C, long ldc) {
+int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long sa, const void *Bh,
+            const void *Bl, long sb, long boff, float *C, long ldc) {
   if (m <= 0 || n <= 0 || k <= 0) return PLA_SUCCESS;
-  if (k % HKS || lda % 8 || ldb % 8) return fail(PLA_ERR_INVALID_PARAM, "hgemm: k %% 32 / ld %% 8");
+  if (k % HKS || sa % 256 || sb % 256 || boff % 8)
+    return fail(PLA_ERR_INVALID_PARAM, "hgemm: k %% 32, slice strides %% 256, column offset %% 8");
+  if (sa < (long)m * 32 || sb < (boff + n) * 32) return fail(PLA_ERR_INVALID_PARAM, "hgemm: slice too small");
   const int tm = (m + HBM - 1) / HBM, tn = (n + HBN - 1) / HBN;
   const int nblk = ((tm * tn + 7) / 8) * 8;
   const int vec_ok = (ldc % 4 == 0) && (((uintptr_t)C & 15) == 0);
   dim3 grid(nblk), blk(256);
+#define PLA_HG(H, SP)                                                                                             \
+  hipLaunchKernelGGL((hgemm_kernel<H, SP>), grid, blk, 0, s, m, n, k, (const H *)Ah, (const H *)Al, sa,           \
+                     (const H *)Bh, (const H *)Bl, sb, boff, C, ldc, tm, tn, vec_ok)
   switch (prec) {
-    case PLA_MIXED_FP16:
-      hipLaunchKernelGGL((hgemm_kernel<_Float16, false>), grid, blk, 0, s, m, n, k, (const _Float16 *)Ah,
-                         (const _Float16 *)Al, lda, (const _Float16 *)Bh, (const _Float16 *)Bl, ldb, C, ldc, tm,
-                         tn, vec_ok);
-      break;
-    case PLA_MIXED_FP16X3:
-      hipLaunchKernelGGL((hgemm_kernel<_Float16, true>), grid, blk, 0, s, m, n, k, (const _Float16 *)Ah,
-                         (const _Float16 *)Al, lda, (const _Float16 *)Bh, (const _Float16 *)Bl, ldb, C, ldc, tm,
-                         tn, vec_ok);
-      break;
-    case PLA_MIXED_BF16:
-      hipLaunchKernelGGL((hgemm_kernel<__bf16, false>), grid, blk, 0, s, m, n, k, (const __bf16 *)Ah,
-                         (const __bf16 *)Al, lda, (const __bf16 *)Bh, (const __bf16 *)Bl, ldb, C, ldc, tm, tn,
-                         vec_ok);
-      break;
-    case PLA_MIXED_BF16X3:
-      hipLaunchKernelGGL((hgemm_kernel<__bf16, true>), grid, blk, 0, s, m, n, k, (const __bf16 *)Ah,
-                         (const __bf16 *)Al, lda, (const __bf16 *)Bh, (const __bf16 *)Bl, ldb, C, ldc, tm, tn,
-                         vec_ok);
-      break;
-    default:
-      return fail(PLA_ERR_INVALID_PARAM, "hgemm: precision %d", prec);
+    case PLA_MIXED_FP16: PLA_HG(_Float16, false); break;
+    case PLA_MIXED_FP16X3: PLA_HG(_Float16, true); break;
+    case PLA_MIXED_BF16: PLA_HG(__bf16, false); break;
+    case PLA_MIXED_BF16X3: PLA_HG(__bf16, true); break;
+    default: return fail(PLA_ERR_INVALID_PARAM, "hgemm: precision %d", prec);
   }
+#undef PLA_HG
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
@@ -717,12 +726,13 @@ int k_cvt_f32_f64(hipStream_t s, int m, int n, const float *W, long ldw, double 
 static bool is_bf16(int prec) { return prec == PLA_MIXED_BF16 || prec == PLA_MIXED_BF16X3; }
 static bool is_split(int prec) { return prec == PLA_MIXED_FP16X3 || prec == PLA_MIXED_BF16X3; }
 
-int k_panel_out(hipStream_t s, int prec, int rows, int b, const double *P, long ldp, void *Lh, void *Ll,
-                int ldl) {
+int k_panel_out(hipStream_t s, int prec, int rows, int b, int kpad, const double *P, long ldp, void *Lh, void *Ll,
+                long sl) {
   if (rows <= 0) return PLA_SUCCESS;
-  dim3 grid(grid1((long)rows * ldl)), blk(256);
+  if (kpad % 32 || kpad < b || sl < (long)rows * 32) return fail(PLA_ERR_INVALID_PARAM, "panel_out: layout");
+  dim3 grid(grid1((long)rows * kpad)), blk(256);
 #define PLA_PO(H, SP) \
-  hipLaunchKernelGGL((panel_out_kernel<H, SP>), grid, blk, 0, s, rows, b, P, ldp, (H *)Lh, (H *)Ll, ldl)
+  hipLaunchKernelGGL((panel_out_kernel<H, SP>), grid, blk, 0, s, rows, b, kpad, P, ldp, (H *)Lh, (H *)Ll, sl)
   if (is_bf16(prec)) {
     if (is_split(prec)) PLA_PO(__bf16, true); else PLA_PO(__bf16, false);
   } else {
@@ -733,12 +743,14 @@ int k_panel_out(hipStream_t s, int prec, int rows, int b, const double *P, long 
   return PLA_SUCCESS;
 }
 
-int k_upanel_out(hipStream_t s, int prec, int b, int nr, const double *U, long ldu, float *W, long ldw, void *Uh,
-                 void *Ul, int ldh) {
+int k_upanel_out(hipStream_t s, int prec, int b, int kpad, int nr, const double *U, long ldu, float *W, long ldw,
+                 void *Uh, void *Ul, long su, long col0) {
   if (nr <= 0) return PLA_SUCCESS;
-  dim3 grid(grid1((long)nr * ldh)), blk(256);
-#define PLA_UO(H, SP) \
-  hipLaunchKernelGGL((upanel_out_kernel<H, SP>), grid, blk, 0, s, b, nr, U, ldu, W, ldw, (H *)Uh, (H *)Ul, ldh)
+  if (kpad % 32 || kpad < b || su < (col0 + nr) * 32) return fail(PLA_ERR_INVALID_PARAM, "upanel_out: layout");
+  dim3 grid(grid1((long)nr * kpad)), blk(256);
+#define PLA_UO(H, SP)                                                                                           \
+  hipLaunchKernelGGL((upanel_out_kernel<H, SP>), grid, blk, 0, s, b, kpad, nr, U, ldu, W, ldw, (H *)Uh, (H *)Ul, \
+                     su, col0)
   if (is_bf16(prec)) {
     if (is_split(prec)) PLA_UO(__bf16, true); else PLA_UO(__bf16, false);
   } else {

@@ -24,7 +24,7 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   nb = (nb / 32) * 32;
   if (nb < 32) nb = 32;
   if (nb > 1024) nb = 1024;
-  const int nbp = nb;  // low-precision copies are padded to nb columns/rows (a multiple of 32)
+  const int nbp = nb;  // depth of the low-precision copies (a multiple of 32)
   const int esz = 2;   // bytes of one FP16 / BF16 element
   const bool split = prec_split(prec);
   const int stride = leaf_pairs_stride();
@@ -32,10 +32,14 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   // L11 from panel k's copy while panel k+1 is factored into the other
   double *P64 = (double *)workspace(sizeof(double) * 2 * (size_t)n * nb, WS_P64);
   double *U64 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_U64);
-  char *Lh = (char *)workspace((size_t)2 * n * nbp * esz, WS_LH);
-  char *Ll = split ? (char *)workspace((size_t)2 * n * nbp * esz, WS_LL) : nullptr;
-  char *Uh = (char *)workspace((size_t)n * nbp * esz, WS_UH);
-  char *Ul = split ? (char *)workspace((size_t)n * nbp * esz, WS_UL) : nullptr;
+  // the low-precision copies are packed k-slices of 32 (k_hgemm): n indices
+  // (rows of L, columns of U) per slice, nbp / 32 slices
+  const long slice = ((long)n * 32 + 255) / 256 * 256;
+  const size_t lbuf = (size_t)slice * (nbp / 32) * esz;  // one copy (one parity of the L copies)
+  char *Lh = (char *)workspace(2 * lbuf, WS_LH);
+  char *Ll = split ? (char *)workspace(2 * lbuf, WS_LL) : nullptr;
+  char *Uh = (char *)workspace(lbuf, WS_UH);
+  char *Ul = split ? (char *)workspace(lbuf, WS_UL) : nullptr;
   const size_t pstride = (size_t)LU_MAX_LEAVES * stride;  // one parity of the leaves' pair lists
   int *pairs = (int *)workspace(sizeof(int) * 2 * pstride, WS_PAIRS, true);
   const int cstride = panel_comp_stride();
@@ -43,7 +47,6 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   // of a step may still be reading theirs while later panels are composed)
   int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(n, n, nb) * cstride, WS_COMP);
   if (!P64 || !U64 || !Lh || !Uh || !pairs || !comp || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
-  const size_t lbuf = (size_t)n * nbp * esz;  // one parity of the L copies
   static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
@@ -72,7 +75,8 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       if ((r = k_panel_compose(sP, pairs + par * pstride, nleaves[par], comp + (size_t)step * cstride)))
         return r;
       if ((r = k_cvt_f64_f32(sP, mp, b, P64, mp, Wk, ldw))) return r;
-      return k_panel_out(sP, prec, mp - b, b, P64 + b, mp, Lh + par * lbuf, split ? Ll + par * lbuf : nullptr, nbp);
+      return k_panel_out(sP, prec, mp - b, b, (b + 31) / 32 * 32, P64 + b, mp, Lh + par * lbuf,
+                         split ? Ll + par * lbuf : nullptr, slice);
     };
     {
       const int b = width_at(0);
@@ -107,12 +111,12 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
         if ((r = k_panel_swap_f32(sM, W + k, ldw, 0, kn + c0, kn + c0 + nc, cp))) return r;
         float *W12c = W12 + (long)c0 * ldw;
         double *U64c = U64 + (size_t)c0 * b;
-        char *Uhc = Uh + (size_t)c0 * nbp * esz, *Ulc = split ? Ul + (size_t)c0 * nbp * esz : nullptr;
         if ((r = k_cvt_f32_f64(sM, b, nc, W12c, ldw, U64c, b))) return r;
         if ((r = trsm_left(sM, true, false, true, b, nc, P64_of(par), n - k, U64c, b, nullptr))) return r;
-        if ((r = k_upanel_out(sM, prec, b, nc, U64c, b, W12c, ldw, Uhc, Ulc, nbp))) return r;
+        if ((r = k_upanel_out(sM, prec, b, kpad, nc, U64c, b, W12c, ldw, Uh, split ? Ul : nullptr, slice, c0)))
+          return r;
         timing_begin(sM, 3);
-        r = k_hgemm(sM, prec, mr, nc, kpad, L_h, L_l, nbp, Uhc, Ulc, nbp, W12c + b, ldw);
+        r = k_hgemm(sM, prec, mr, nc, kpad, L_h, L_l, slice, Uh, split ? Ul : nullptr, slice, c0, W12c + b, ldw);
         timing_end(sM, 3, 2.0 * mr * (double)nc * b);
         return r;
       };
