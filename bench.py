@@ -135,7 +135,7 @@ def timing_query(pla, role):
 # HBM bytes per launch of the trailing-update kernels at n=65536 (rocprofv3
 # --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the scale factors measured on a
 # copy of known size): profiles/trailing_gemm_traffic_n65536.md
-TRAFFIC_N65536 = {"hgemm": 9.95e9, "dgemm": 5.84e10}
+TRAFFIC_N65536 = {"hgemm": 9.33e9, "dgemm": 5.91e10}
 
 
 def roofline(kernel, launches, ms, flops, peak, mfma_per_flop=1, traffic=None):
