@@ -46,6 +46,8 @@ def lib():
         L.ora_lu_unblocked.argtypes = [_i, _i, _p, _i, _p]
         L.ora_lu_blocked.argtypes = [_i, _i, _p, _i, _i, _p]
         L.ora_lu_linpack.argtypes = [_i, _i, _p, _i, _i, _p]
+        L.ora_zlu_blocked.argtypes = [_i, _i, _p, _i, _i, _p]
+        L.ora_zlu_solve.argtypes = [_i, _i, _p, _i, _p, _p, _i]
         L.ora_laswp.argtypes = [_i, _p, _i, _i, _i, _p]
         L.ora_chol_unblocked.restype = _i
         L.ora_chol_unblocked.argtypes = [_i, _p, _i]
@@ -118,6 +120,37 @@ def lu_linpack(a, nb=128):
     piv = np.zeros(min(m, n), dtype=np.int32)
     lib().ora_lu_linpack(m, n, _ptr(a), max(m, 1), nb, _ptr(piv))
     return a, piv
+
+
+def _z(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.complex128))
+
+
+def zlu_blocked(a, nb=128):
+    """MPI_DOUBLE_COMPLEX PLA_LU (ora_zlu_blocked): returns (factors, pivots)."""
+    a = _z(a).copy(order="F")
+    m, n = a.shape
+    piv = np.zeros(min(m, n), dtype=np.int32)
+    lib().ora_zlu_blocked(m, n, _ptr(a), max(m, 1), nb, _ptr(piv))
+    return a, piv
+
+
+def zlu_solve(lu, piv, b):
+    lu = _z(lu)
+    b = _z(b).copy(order="F")
+    if b.ndim == 1:
+        b = b.reshape(-1, 1, order="F")
+    n, nrhs = b.shape
+    lib().ora_zlu_solve(n, nrhs, _ptr(lu), max(lu.shape[0], 1), _ptr(np.ascontiguousarray(piv, dtype=np.int32)),
+                        _ptr(b), max(n, 1))
+    return b
+
+
+def uniform_z(m, n, seed=0, skip=0):
+    """TEST/create_problem.c for MPI_DOUBLE_COMPLEX: drand48()*2-1 for the real
+    then the imaginary part of each element, column by column."""
+    r = uniform(2 * m, n, seed, skip)
+    return np.asfortranarray(r[0::2] + 1j * r[1::2])
 
 
 def laswp(a, piv, r0=0):

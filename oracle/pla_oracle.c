@@ -11,6 +11,7 @@
  */
 #include "pla_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -440,3 +441,105 @@ void ora_api_multi(int to_global, int nr, int nc, const int *rsz, const int *csz
     li += rsz[i];
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* MPI_DOUBLE_COMPLEX LU: PLA_LU_right / PLA_LU_right_mv on complex    */
+/* data.  The pivot is izamax's (BLAS1/PLA_Iamax.c:176-230: the first   */
+/* maximum of |re| + |im|), a_21 is scaled by the reciprocal of the     */
+/* pivot (Local_BLAS/PLA_Local_BLAS1.c:240-250, computed here as        */
+/* conj(z) / (re^2 + im^2)), the rank-1 update is zgeru                 */
+/* (INCLUDE/PLA_BLAS.h:407), and the trailing steps ztrsm / zgemm.     */
+
+typedef double complex zc;
+#define Z_(i, j) z[(size_t)(j) * (size_t)lda + (size_t)(i)]
+
+static int izamax(int n, const zc *x) {
+  if (n <= 0) return -1;
+  int best = 0;
+  double dmax = fabs(creal(x[0])) + fabs(cimag(x[0]));
+  for (int i = 1; i < n; i++) {
+    double v = fabs(creal(x[i])) + fabs(cimag(x[i]));
+    if (v > dmax) { dmax = v; best = i; }
+  }
+  return best;
+}
+
+static zc zrecip(zc v) {
+  double d = creal(v) * creal(v) + cimag(v) * cimag(v);
+  return creal(v) / d - I * (cimag(v) / d);
+}
+
+void ora_zlu_blocked(int m, int n, void *av, int lda, int nb, int *piv) {
+  zc *z = (zc *)av;
+  int mn = m < n ? m : n;
+  for (int k = 0; k < mn; k += nb) {
+    int rows = m - k, b = mn - k < nb ? mn - k : nb;
+    /* PLA_LU_right_mv on the panel (its own columns only) */
+    for (int j = 0; j < b; j++) {
+      int idx = izamax(rows - j, &Z_(k + j, k + j));
+      piv[k + j] = idx;
+      if (idx)
+        for (int c = 0; c < b; c++) {
+          zc t = Z_(k + j, k + c);
+          Z_(k + j, k + c) = Z_(k + j + idx, k + c);
+          Z_(k + j + idx, k + c) = t;
+        }
+      zc inv = zrecip(Z_(k + j, k + j));
+      for (int i = k + j + 1; i < m; i++) Z_(i, k + j) *= inv;
+      for (int c = j + 1; c < b; c++) {
+        zc temp = -Z_(k + j, k + c);
+        if (temp != 0.0)
+          for (int i = k + j + 1; i < m; i++) Z_(i, k + c) += Z_(i, k + j) * temp;
+      }
+    }
+    /* interchanges on every column outside the panel */
+    for (int j = 0; j < b; j++)
+      if (piv[k + j]) {
+        for (int c = 0; c < n; c++) {
+          if (c >= k && c < k + b) continue;
+          zc t = Z_(k + j, c);
+          Z_(k + j, c) = Z_(k + j + piv[k + j], c);
+          Z_(k + j + piv[k + j], c) = t;
+        }
+      }
+    int nr = n - k - b;
+    if (nr > 0) {
+      /* A_12 <- inv(L_11) A_12 (ztrsm, unit lower, column axpys) */
+      for (int c = 0; c < nr; c++)
+        for (int l = 0; l < b; l++) {
+          zc x = Z_(k + l, k + b + c);
+          if (x != 0.0)
+            for (int i = l + 1; i < b; i++) Z_(k + i, k + b + c) -= x * Z_(k + i, k + l);
+        }
+      /* A_22 <- A_22 - A_21 A_12 (zgemm, netlib order) */
+      for (int c = 0; c < nr; c++)
+        for (int l = 0; l < b; l++) {
+          zc temp = -Z_(k + l, k + b + c);
+          if (temp != 0.0)
+            for (int i = k + b; i < m; i++) Z_(i, k + b + c) += temp * Z_(i, k + l);
+        }
+    }
+  }
+}
+
+/* PLA_General_solve on complex factors: interchanges on B, then the unit
+   lower and the upper triangular solves (ztrsm column order, division by
+   the diagonal). */
+void ora_zlu_solve(int n, int nrhs, const void *luv, int lda, const int *piv, void *bv, int ldb) {
+  const zc *z = (const zc *)luv;
+  zc *bb = (zc *)bv;
+  for (int c = 0; c < nrhs; c++) {
+    zc *x = bb + (size_t)c * ldb;
+    for (int i = 0; i < n; i++)
+      if (piv[i]) { zc t = x[i]; x[i] = x[i + piv[i]]; x[i + piv[i]] = t; }
+    for (int k = 0; k < n; k++)
+      if (x[k] != 0.0)
+        for (int i = k + 1; i < n; i++) x[i] -= x[k] * Z_(i, k);
+    for (int k = n - 1; k >= 0; k--)
+      if (x[k] != 0.0) {
+        x[k] /= Z_(k, k);
+        for (int i = 0; i < k; i++) x[i] -= x[k] * Z_(i, k);
+      }
+  }
+}
+#undef Z_

@@ -28,6 +28,8 @@
  *   ora_trsv / ora_trsm_left / ora_trsm_right : scipy solve_triangular
  *   ora_gemm / ora_syrk                       : numpy matmul (ora_syrk also:
  *       the opposite triangle is exactly beta C)
+ *   ora_zlu_blocked / ora_zlu_solve           : scipy LAPACK zgetrf pivots and
+ *       factors; solve residuals
  *   ora_api_*                                 : numpy slice arithmetic,
  *       bit-exact (one rounded product, one rounded add per element)
  * No function here is "parity unpinned".
@@ -94,6 +96,11 @@ void ora_api_from_global(int m, int n, double alpha, const double *a, int lda, i
                          int ldb);
 void ora_api_multi(int to_global, int nr, int nc, const int *rsz, const int *csz, double alpha, double *buf,
                    int ldb, double *a, int lda, const int *rd, const int *cd);
+
+/* MPI_DOUBLE_COMPLEX PLA_LU (interleaved re/im, column-major): izamax
+   pivots (|re| + |im|), zgeru updates; and PLA_General_solve on its factors. */
+void ora_zlu_blocked(int m, int n, void *a, int lda, int nb, int *piv);
+void ora_zlu_solve(int n, int nrhs, const void *lu, int lda, const int *piv, void *b, int ldb);
 
 /* PLA_General_solve (SOLVERS/LU/PLA_General.c:67-119) on already-factored A:
    apply pivots to B, then L (unit) and U solves. */

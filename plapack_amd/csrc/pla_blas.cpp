@@ -250,7 +250,8 @@ int PLA_Kernel_timing_query(int role, long *launches, double *total_ms, double *
 }
 
 int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots) {
-  int rc = check_double(A, "PLA_Apply_pivots_to_rows");
+  const bool cplx = valid(A) && A->base->datatype == MPI_DOUBLE_COMPLEX;
+  int rc = cplx ? PLA_SUCCESS : check_double(A, "PLA_Apply_pivots_to_rows");
   if (rc) return rc;
   if (!valid(pivots) || pivots->base->datatype != MPI_INT)
     return fail(PLA_ERR_INVALID_PARAM, "PLA_Apply_pivots_to_rows: pivots must be an MPI_INT object");
@@ -272,7 +273,9 @@ int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots) {
     if (hp[i] < 0 || i + hp[i] >= A->length)
       return fail(PLA_ERR_INVALID_PARAM, "PLA_Apply_pivots_to_rows: pivot %d (= %d) leaves the matrix", i,
                   hp[i]);
-  if ((rc = k_laswp(ctx().stream, a, ld(A), A->length, A->width, np, p))) return rc;
+  if ((rc = cplx ? k_zlaswp(ctx().stream, a, ld(A), A->length, A->width, np, p)
+                 : k_laswp(ctx().stream, a, ld(A), A->length, A->width, np, p)))
+    return rc;
   return written(A);
 }
 

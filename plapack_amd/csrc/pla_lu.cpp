@@ -1,4 +1,5 @@
-// pla_lu.cpp -- PLA_LU and PLA_General_solve on a 1x1 template, device resident.
+// pla_lu.cpp -- PLA_LU and PLA_General_solve on a 1x1 template, device resident
+// (MPI_DOUBLE here; MPI_DOUBLE_COMPLEX objects go to pla_zlu.cpp).
 //
 // Reference: SOLVERS/LU/PLA_LU.c:13-212 (blocked right-looking LU with
 // partial pivoting), SOLVERS/LU/PLA_General.c:67-119 (factor + solve).
@@ -18,8 +19,8 @@ using namespace pla;
 static int check_lu_args(PLA_Obj A, PLA_Obj pivots) {
   // SOLVERS/LU/PLA_LU_enter_exit.c:17-105
   if (!valid(A)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_LU: invalid A");
-  if (A->base->datatype != MPI_DOUBLE)
-    return fail(PLA_ERR_UNSUPPORTED, "PLA_LU: only MPI_DOUBLE matrices are supported");
+  if (A->base->datatype != MPI_DOUBLE && A->base->datatype != MPI_DOUBLE_COMPLEX)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_LU: only MPI_DOUBLE and MPI_DOUBLE_COMPLEX matrices are supported");
   if (A->objtype != PLA_MATRIX) return fail(PLA_ERR_INVALID_PARAM, "PLA_LU: A must be a PLA_MATRIX");
   if (!valid(pivots)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_LU: invalid pivots");
   if (pivots->base->datatype != MPI_INT)
@@ -182,6 +183,13 @@ static int lu_entry(PLA_Obj A, PLA_Obj pivots, bool swap_left) {
   double *a;
   int *piv;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(pivots, (void **)&piv))) return rc;
+  if (A->base->datatype == MPI_DOUBLE_COMPLEX) {
+    if (!swap_left) return fail(PLA_ERR_UNSUPPORTED, "PLA_LU_linpack: MPI_DOUBLE_COMPLEX is not supported");
+    const int nbz = ctx().nb_device > 0 ? ctx().nb_device : (ctx().nb_user > 0 ? ctx().nb_user : 256);
+    if ((rc = zlu_factor(ctx().stream, m, n, a, ld(A), piv, nbz))) return rc;
+    if ((rc = written(A))) return rc;
+    return written(pivots);
+  }
   const int nb = choose_nb(mn, NB_LU);
   if ((rc = lu_factor(ctx().stream, m, n, a, ld(A), piv, nb, swap_left))) return rc;
   if ((rc = written(A))) return rc;
@@ -201,8 +209,8 @@ int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots) { return lu_entry(A, pivots, false
 int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
   int rc = PLA_LU(A, ipiv);
   if (rc) return rc;
-  if (!valid(B) || B->base->datatype != MPI_DOUBLE)
-    return fail(PLA_ERR_INVALID_OBJECT, "PLA_General_solve: invalid B");
+  if (!valid(B) || B->base->datatype != A->base->datatype)
+    return fail(PLA_ERR_INVALID_OBJECT, "PLA_General_solve: invalid B (or not of A's datatype)");
   const int n = A->length;
   if (A->width != n || B->length != n)
     return fail(PLA_ERR_DIMENSIONS, "PLA_General_solve: A %d x %d, B %d x %d", A->length, A->width,
@@ -213,6 +221,10 @@ int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
       (rc = dev_ptr(ipiv, (void **)&piv)))
     return rc;
   hipStream_t s = ctx().stream;
+  if (A->base->datatype == MPI_DOUBLE_COMPLEX) {
+    if ((rc = zlu_solve(s, n, B->width, a, ld(A), piv, b, ld(B)))) return rc;
+    return written(B);
+  }
   if ((rc = k_laswp(s, b, ld(B), n, B->width, n, piv))) return rc;
   if (B->width <= trsv_max_rhs()) {
     if ((rc = trsv_sf(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;

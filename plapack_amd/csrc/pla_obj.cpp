@@ -389,16 +389,19 @@ int PLA_Obj_set_local_contents(int trans, int rows_in_buf, int cols_in_buf, void
 
 int PLA_Obj_fill_drand48(PLA_Obj obj, long seed, int64_t skip) {
   if (!valid(obj)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Obj_fill_drand48: invalid obj");
-  if (obj->base->datatype != MPI_DOUBLE)
-    return fail(PLA_ERR_UNSUPPORTED, "PLA_Obj_fill_drand48: only MPI_DOUBLE objects");
+  const bool cplx = obj->base->datatype == MPI_DOUBLE_COMPLEX;
+  if (obj->base->datatype != MPI_DOUBLE && !cplx)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_Obj_fill_drand48: only MPI_DOUBLE and MPI_DOUBLE_COMPLEX objects");
   if (skip < 0) return fail(PLA_ERR_INVALID_PARAM, "PLA_Obj_fill_drand48: negative skip");
   void *p;
   int rc = dev_ptr(obj, &p);
   if (rc) return rc;
   const uint64_t state0 = ((((uint64_t)(uint32_t)seed) << 16) | 0x330EULL) & ((1ULL << 48) - 1);
-  if ((rc = k_fill_drand48(ctx().stream, obj->length, obj->width, (double *)p, ld(obj), state0,
-                           (uint64_t)skip)))
-    return rc;
+  // a complex element is two consecutive draws, real part first (TEST/create_problem.c:60-65):
+  // the stream fills the 2m x n real view of the doubles
+  const int mr = cplx ? 2 * obj->length : obj->length;
+  const long ldr = cplx ? 2L * ld(obj) : ld(obj);
+  if ((rc = k_fill_drand48(ctx().stream, mr, obj->width, (double *)p, ldr, state0, (uint64_t)skip))) return rc;
   return written(obj);
 }
 

@@ -23,6 +23,7 @@ MPI_COMM_SELF = 0x44000001
 MPI_INT = 0x4C000405
 MPI_FLOAT = 0x4C00040A
 MPI_DOUBLE = 0x4C00080B
+MPI_DOUBLE_COMPLEX = 1275072546
 
 # constants (INCLUDE/PLA_define.h:11-102)
 PLA_MVECTOR = 12
@@ -292,7 +293,8 @@ def datatype(o):
 
 
 def _np_dtype(dt):
-    return {MPI_DOUBLE: np.float64, MPI_INT: np.int32, MPI_FLOAT: np.float32}[dt]
+    return {MPI_DOUBLE: np.float64, MPI_INT: np.int32, MPI_FLOAT: np.float32,
+            MPI_DOUBLE_COMPLEX: np.complex128}[dt]
 
 
 def set_contents(o, arr):
@@ -319,7 +321,7 @@ def from_numpy(session, arr):
     a = np.asarray(arr)
     if a.ndim == 1:
         a = a.reshape(-1, 1)
-    dt = MPI_INT if a.dtype.kind in "iu" else MPI_DOUBLE
+    dt = MPI_INT if a.dtype.kind in "iu" else (MPI_DOUBLE_COMPLEX if a.dtype.kind == "c" else MPI_DOUBLE)
     o = session.matrix(a.shape[0], a.shape[1], dt)
     set_contents(o, a)
     return o

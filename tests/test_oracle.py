@@ -274,6 +274,40 @@ def test_lu_linpack_vs_lu_blocked(ora, m, n, nb):
         assert not np.array_equal(f1[:, :nb], f2[:, :nb])
 
 
+@pytest.mark.parametrize("m,n,nb", [(40, 40, 8), (60, 35, 16), (35, 60, 16), (97, 97, 32)])
+def test_zlu_blocked_vs_lapack(ora, m, n, nb):
+    """MPI_DOUBLE_COMPLEX PLA_LU: izamax pivots (|re| + |im|) and zgeru updates
+    agree with LAPACK zgetrf (scipy), which uses the same pivot rule."""
+    a = ora.uniform_z(m, n, seed=m + n)
+    f, piv = ora.zlu_blocked(a, nb=nb)
+    lu, pp = scipy.linalg.lu_factor(a, check_finite=False)
+    mn = min(m, n)
+    assert np.array_equal(piv, pp[:mn] - np.arange(mn))
+    assert np.abs(f - lu).max() <= 1e-12 * np.abs(a).max()
+
+
+def test_zlu_exact_pivot_rule(ora):
+    """|re| + |im| (not the modulus) picks the pivot: 3+3i (6) beats 5 (5) though
+    |5| > |3+3i|; a tie keeps the first row."""
+    a = np.array([[1.0, 1.0], [5.0, 2.0], [3 + 3j, 0.0], [-6j, 1.0]], dtype=complex)
+    _, piv = ora.zlu_blocked(a.copy(), nb=2)
+    assert piv[0] == 2  # rows 2 and 3 tie at 6: the first
+    b = np.array([[1.0, 0.0], [5.0, 1.0], [3 + 3j, 2.0]], dtype=complex)
+    _, piv = ora.zlu_blocked(b, nb=2)
+    assert piv[0] == 2
+
+
+def test_zlu_solve_and_stream(ora):
+    n = 80
+    ab = ora.uniform_z(n, n + 1, seed=2)
+    # create_problem's complex stream: real then imaginary part of each element
+    r = ora.uniform(2 * n, n + 1, 2)
+    assert np.array_equal(ab.real, r[0::2]) and np.array_equal(ab.imag, r[1::2])
+    f, piv = ora.zlu_blocked(ab[:, :n], nb=16)
+    x = ora.zlu_solve(f, piv, ab[:, n])[:, 0]
+    assert np.abs(x - np.linalg.solve(ab[:, :n], ab[:, n])).max() <= 1e-10 * np.abs(x).max()
+
+
 def test_api_multi_vs_numpy_slices(ora):
     """Multi-block axpys: sub-block (i, j) of the local buffer starts at the
     running sums of the sizes and lands at (rd[i], cd[j]); overlapping targets
