@@ -270,8 +270,9 @@ int PLA_API_multi_axpy_global_to_matrix(int nsub_row, int nsub_col, int *size_ro
                                         int local_ldim);
 
 /* ---- factor and solve path ---- */
-/* SOLVERS/LU/PLA_LU.c:13.  A: m x n MPI_DOUBLE matrix, overwritten with L
-   (unit, below the diagonal) and U; pivots: MPI_INT mvector of length
+/* SOLVERS/LU/PLA_LU.c:13.  A: m x n MPI_DOUBLE or MPI_DOUBLE_COMPLEX matrix
+   (complex pivots: the first maximum of |re| + |im|, izamax's rule),
+   overwritten with L (unit, below the diagonal) and U; pivots: MPI_INT mvector of length
    min(m,n) receiving, for each row j, the 0-based offset of its pivot row
    relative to row j.  Returns PLA_SUCCESS; like the reference, a zero pivot
    is not flagged (the factors then hold inf/nan). */

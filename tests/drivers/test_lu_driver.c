@@ -4,7 +4,8 @@
  * <mpi.h>, linked with libpla_mi355x.so.
  *
  * stdin (as TEST/test_lu.c:40-131): mesh rows cols, nb_distr, nb_alg, error,
- * parameters, sequential, then pairs "datatype n" until datatype -1.
+ * parameters, sequential, then pairs "datatype n" until datatype -1 (1: double,
+ * 3: double complex).
  * The matrix is filled through PLA_Obj_local_buffer on the host exactly as
  * TEST/create_problem.c:31-52 does (srand48(0), drand48()*2-1, column-major);
  * the library synchronises its host mirror with HBM.  For every n the driver
@@ -17,15 +18,18 @@
 #include "PLA.h"
 
 static void create_problem(PLA_Obj A) {
-  int m, n, ldim, i, j;
+  int m, n, ldim, i, j, per = 1;
   double *buf;
+  MPI_Datatype dt;
   PLA_Obj_local_length(A, &m);
   PLA_Obj_local_width(A, &n);
   PLA_Obj_local_ldim(A, &ldim);
+  PLA_Obj_datatype(A, &dt);
+  if (dt == MPI_DOUBLE_COMPLEX) per = 2; /* real then imaginary part, as TEST/create_problem.c:60-65 */
   PLA_Obj_local_buffer(A, (void **)&buf);
   srand48(0);
   for (j = 0; j < n; j++)
-    for (i = 0; i < m; i++) buf[j * ldim + i] = drand48() * 2.0 - 1.0;
+    for (i = 0; i < m * per; i++) buf[(size_t)j * ldim * per + i] = drand48() * 2.0 - 1.0;
 }
 
 int main(int argc, char *argv[]) {
@@ -52,11 +56,12 @@ int main(int argc, char *argv[]) {
   PLA_Temp_create(nb_distr, 0, &templ);
   while (scanf("%d", &itype) == 1 && itype != -1) {
     if (scanf("%d", &n) != 1) break;
-    if (itype != 1) {
+    if (itype != 1 && itype != 3) {
       printf("datatype %d not supported\n", itype);
       continue;
     }
-    PLA_Matrix_create(MPI_DOUBLE, n, n, templ, PLA_ALIGN_FIRST, PLA_ALIGN_FIRST, &A);
+    PLA_Matrix_create(itype == 3 ? MPI_DOUBLE_COMPLEX : MPI_DOUBLE, n, n, templ, PLA_ALIGN_FIRST, PLA_ALIGN_FIRST,
+                      &A);
     PLA_Mvector_create(MPI_INT, n, 1, templ, PLA_ALIGN_FIRST, &pivots);
     create_problem(A);
     MPI_Barrier(MPI_COMM_WORLD);

@@ -197,7 +197,7 @@ def _driver_exe(name="test_lu_driver"):
     return exe
 
 
-def test_reference_style_c_driver(tmp_path):
+def test_reference_style_c_driver(tmp_path, ora):
     """A TEST/test_lu.c-style C program, host-filled through PLA_Obj_local_buffer."""
     exe = _driver_exe()
     pivfile = str(tmp_path / "piv.bin")
@@ -207,6 +207,12 @@ def test_reference_style_c_driver(tmp_path):
     assert "MFLOPS/node" in out.stdout
     piv = np.fromfile(pivfile, dtype=np.int32)
     assert np.array_equal(piv, np.load(os.path.join(GOLDEN, "lu_piv_n2048_seed0.npy")))
+    # TEST/test_lu.c's double-complex datatype (itype 3), n = 1000
+    out = subprocess.run([exe, pivfile], input="1 1\n128\n128\n1\n1\n0\n3 1000\n-1\n", capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    _, piv_z = ora.zlu_blocked(ora.uniform_z(1000, 1000, 0), nb=128)  # nb_alg 128 from stdin
+    assert np.array_equal(np.fromfile(pivfile, dtype=np.int32), piv_z)
     # a 2 x 1 mesh is refused with an error code
     out = subprocess.run([exe], input="2 1\n128\n128\n1\n1\n0\n-1\n", capture_output=True, text=True,
                          timeout=120)
