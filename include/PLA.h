@@ -282,8 +282,9 @@ int PLA_LU(PLA_Obj A, PLA_Obj pivots);
    order of their own step (LINPACK's dgefa layout); on [A b] (n x (n+1)) the
    last column ends as inv(L) P b, ready for PLA_Trsv with U */
 int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots);
-/* SOLVERS/Cholesky/PLA_Chol.c:13.  Lower only (the reference aborts on
-   upper).  Returns 0, or the 1-based global index k+i+1 of the first
+/* SOLVERS/Cholesky/PLA_Chol.c:13.  MPI_DOUBLE (A = L L^T) or
+   MPI_DOUBLE_COMPLEX (A = L L^H, zero imaginary diagonal).  Lower only (the
+   reference aborts on upper).  Returns 0, or the 1-based global index k+i+1 of the first
    non-positive pivot (PLA_Chol.c:181-185); the diagonal block that failed
    is left as it was before its factorisation, as in the reference. */
 int PLA_Chol(int uplo, PLA_Obj A);

@@ -48,6 +48,8 @@ def lib():
         L.ora_lu_linpack.argtypes = [_i, _i, _p, _i, _i, _p]
         L.ora_zlu_blocked.argtypes = [_i, _i, _p, _i, _i, _p]
         L.ora_zlu_solve.argtypes = [_i, _i, _p, _i, _p, _p, _i]
+        L.ora_zchol_blocked.restype = _i
+        L.ora_zchol_blocked.argtypes = [_i, _p, _i, _i]
         L.ora_laswp.argtypes = [_i, _p, _i, _i, _i, _p]
         L.ora_chol_unblocked.restype = _i
         L.ora_chol_unblocked.argtypes = [_i, _p, _i]
@@ -144,6 +146,13 @@ def zlu_solve(lu, piv, b):
     lib().ora_zlu_solve(n, nrhs, _ptr(lu), max(lu.shape[0], 1), _ptr(np.ascontiguousarray(piv, dtype=np.int32)),
                         _ptr(b), max(n, 1))
     return b
+
+
+def zchol_blocked(a, nb=128):
+    """MPI_DOUBLE_COMPLEX PLA_Chol, lower (ora_zchol_blocked): returns (factors, info)."""
+    a = _z(a).copy(order="F")
+    info = lib().ora_zchol_blocked(a.shape[0], _ptr(a), max(a.shape[0], 1), nb)
+    return a, info
 
 
 def uniform_z(m, n, seed=0, skip=0):

@@ -522,6 +522,48 @@ void ora_zlu_blocked(int m, int n, void *av, int lda, int nb, int *piv) {
   }
 }
 
+/* PLA_Chol_lower_large (SOLVERS/Cholesky/PLA_Chol.c:83-234) on complex data:
+   PLA_zchol on the diagonal block (PLA_Local_chol.c:210-232: sqrt of the real
+   diagonal, a zero imaginary part, the column scaled by the reciprocal, zher
+   on the trailing part), A21 <- A21 inv(L11)^H (ztrsm "R","L","C","N"),
+   A22 -= A21 A21^H (zherk; the updated diagonal keeps a zero imaginary part).
+   Returns 0 or k + j + 1 for the first non-positive real diagonal (a test
+   the reference's zchol leaves out and its dchol makes). */
+int ora_zchol_blocked(int n, void *av, int lda, int nb) {
+  zc *z = (zc *)av;
+  for (int k = 0; k < n; k += nb) {
+    int b = n - k < nb ? n - k : nb;
+    for (int j = 0; j < b; j++) {
+      double d = creal(Z_(k + j, k + j));
+      if (!(d > 0.0)) return k + j + 1;
+      double r = sqrt(d), inv = 1.0 / r;
+      Z_(k + j, k + j) = r;
+      for (int i = j + 1; i < b; i++) Z_(k + i, k + j) *= inv;
+      for (int c = j + 1; c < b; c++) {
+        zc y = conj(Z_(k + c, k + j));
+        for (int i = c; i < b; i++) Z_(k + i, k + c) -= Z_(k + i, k + j) * y;
+        Z_(k + c, k + c) = creal(Z_(k + c, k + c));
+      }
+    }
+    int r = n - k - b;
+    if (r <= 0) continue;
+    for (int i = k + b; i < n; i++)   /* each row of A21: x <- x inv(L11)^H */
+      for (int c = 0; c < b; c++) {
+        Z_(i, k + c) /= conj(Z_(k + c, k + c));
+        for (int j = c + 1; j < b; j++) Z_(i, k + j) -= conj(Z_(k + j, k + c)) * Z_(i, k + c);
+      }
+    for (int c = k + b; c < n; c++) { /* lower A22 -= A21 A21^H, column by column */
+      for (int l = k; l < k + b; l++) {
+        zc y = conj(Z_(c, l));
+        if (y != 0.0)
+          for (int i = c; i < n; i++) Z_(i, c) -= Z_(i, l) * y;
+      }
+      Z_(c, c) = creal(Z_(c, c));
+    }
+  }
+  return 0;
+}
+
 /* PLA_General_solve on complex factors: interchanges on B, then the unit
    lower and the upper triangular solves (ztrsm column order, division by
    the diagonal). */

@@ -30,6 +30,8 @@
  *       the opposite triangle is exactly beta C)
  *   ora_zlu_blocked / ora_zlu_solve           : scipy LAPACK zgetrf pivots and
  *       factors; solve residuals
+ *   ora_zchol_blocked                         : numpy.linalg.cholesky (complex),
+ *       failure index against exact leading-minor signs
  *   ora_api_*                                 : numpy slice arithmetic,
  *       bit-exact (one rounded product, one rounded add per element)
  * No function here is "parity unpinned".
@@ -101,6 +103,8 @@ void ora_api_multi(int to_global, int nr, int nc, const int *rsz, const int *csz
    pivots (|re| + |im|), zgeru updates; and PLA_General_solve on its factors. */
 void ora_zlu_blocked(int m, int n, void *a, int lda, int nb, int *piv);
 void ora_zlu_solve(int n, int nrhs, const void *lu, int lda, const int *piv, void *b, int ldb);
+/* MPI_DOUBLE_COMPLEX PLA_Chol (lower, A = L L^H): 0 or the failure index. */
+int ora_zchol_blocked(int n, void *a, int lda, int nb);
 
 /* PLA_General_solve (SOLVERS/LU/PLA_General.c:67-119) on already-factored A:
    apply pivots to B, then L (unit) and U solves. */

@@ -115,8 +115,8 @@ int PLA_Chol(int uplo, PLA_Obj A) {
   if (uplo != PLA_LOWER_TRIANGULAR)
     return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol: only uplo = PLA_LOWER_TRIANGULAR is implemented");
   if (!valid(A)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Chol: invalid A");
-  if (A->base->datatype != MPI_DOUBLE)
-    return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol: only MPI_DOUBLE matrices are supported");
+  if (A->base->datatype != MPI_DOUBLE && A->base->datatype != MPI_DOUBLE_COMPLEX)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol: only MPI_DOUBLE and MPI_DOUBLE_COMPLEX matrices are supported");
   const int n = A->length;
   if (A->width != n) return fail(PLA_ERR_DIMENSIONS, "PLA_Chol: A is %d x %d", A->length, A->width);
   if (n == 0) return PLA_SUCCESS;
@@ -124,6 +124,12 @@ int PLA_Chol(int uplo, PLA_Obj A) {
   int rc = dev_ptr(A, (void **)&a);
   if (rc) return rc;
   int info = 0;
+  if (A->base->datatype == MPI_DOUBLE_COMPLEX) {  // A = L L^H (pla_zfactor.cpp)
+    const int nbz = ctx().nb_device > 0 ? ctx().nb_device : (ctx().nb_user > 0 ? ctx().nb_user : 256);
+    if ((rc = zchol_factor(ctx().stream, n, a, ld(A), nbz, &info))) return rc;
+    if ((rc = written(A))) return rc;
+    return info;
+  }
   if ((rc = chol_factor(ctx().stream, n, a, ld(A), choose_nb(n, NB_CHOL), &info))) return rc;
   if ((rc = written(A))) return rc;
   return info;
@@ -138,8 +144,8 @@ int PLA_Chol(int uplo, PLA_Obj A) {
 // factored as far as it got).
 int PLA_Chol_simple(int nb_alg, PLA_Obj A) {
   if (!valid(A)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Chol_simple: invalid A");
-  if (A->base->datatype != MPI_DOUBLE)
-    return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol_simple: only MPI_DOUBLE matrices are supported");
+  if (A->base->datatype != MPI_DOUBLE && A->base->datatype != MPI_DOUBLE_COMPLEX)
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol_simple: only MPI_DOUBLE and MPI_DOUBLE_COMPLEX matrices are supported");
   const int n = A->length;
   if (A->width != n) return fail(PLA_ERR_DIMENSIONS, "PLA_Chol_simple: A is %d x %d", A->length, A->width);
   if (n == 0 || nb_alg <= 0) return PLA_SUCCESS;
@@ -147,7 +153,9 @@ int PLA_Chol_simple(int nb_alg, PLA_Obj A) {
   int rc = dev_ptr(A, (void **)&a);
   if (rc) return rc;
   int info = 0;
-  if ((rc = chol_factor(ctx().stream, n, a, ld(A), nb_alg, &info))) return rc;
+  if ((rc = A->base->datatype == MPI_DOUBLE_COMPLEX ? zchol_factor(ctx().stream, n, a, ld(A), nb_alg, &info)
+                                                    : chol_factor(ctx().stream, n, a, ld(A), nb_alg, &info)))
+    return rc;
   if ((rc = written(A))) return rc;
   return info;
 }
@@ -158,14 +166,18 @@ int PLA_Chol_simple(int nb_alg, PLA_Obj A) {
 int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B) {
   int rc = PLA_Chol(PLA_LOWER_TRIANGULAR, A);
   if (rc) return rc;
-  if (!valid(B) || B->base->datatype != MPI_DOUBLE)
-    return fail(PLA_ERR_INVALID_OBJECT, "PLA_Pos_def_solve: invalid B");
+  if (!valid(B) || B->base->datatype != A->base->datatype)
+    return fail(PLA_ERR_INVALID_OBJECT, "PLA_Pos_def_solve: invalid B (or not of A's datatype)");
   const int n = A->length;
   if (B->length != n)
     return fail(PLA_ERR_DIMENSIONS, "PLA_Pos_def_solve: B has %d rows, A is %d", B->length, n);
   double *a, *b;
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(B, (void **)&b))) return rc;
   hipStream_t s = ctx().stream;
+  if (A->base->datatype == MPI_DOUBLE_COMPLEX) {
+    if ((rc = zchol_solve(s, n, B->width, a, ld(A), b, ld(B)))) return rc;
+    return written(B);
+  }
   if (B->width <= trsv_max_rhs()) {
     if ((rc = trsv_sf(s, true, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
     if ((rc = trsv_sf(s, true, true, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;

@@ -13,8 +13,9 @@
 // already the real 2k x n matrix of its doubles with leading dimension 2 ld;
 // then Ahat B is the complex product, 8mnk real flops -- the complex flop
 // count -- on the MFMA kernel.  These kernels are the rest: the embedding,
-// a one-workgroup panel leaf, the sequential interchanges, and the
-// triangular-solve leaf.
+// a one-workgroup panel leaf, the sequential interchanges, the
+// triangular-solve leaves, the conjugate transpose, and the Cholesky
+// diagonal-block kernels.
 #include "pla_internal.hpp"
 
 namespace {
@@ -142,10 +143,10 @@ __global__ __launch_bounds__(ZLEAF_THREADS) void zleaf_kernel(zd *__restrict__ P
   }
 }
 
-// Left triangular-solve leaf for complex data, B <- M^-1 B with M lower unit
-// (LOWER) or upper non-unit, b <= ZTRSM_LEAF: a wave per column group, lane i
-// owning row i (the real leaf's scheme; complex rows take two registers, so
-// the leaf is 32 rows and half the lanes idle).
+// Left triangular-solve leaf for complex data, B <- M^-1 B with M lower or
+// upper, unit or not (dividing by the diagonal), b <= ZTRSM_LEAF: a wave per
+// column group, lane i owning row i (the real leaf's scheme; complex rows take
+// two registers, so the leaf is 32 rows and half the lanes idle).
 constexpr int ZTRSM_LEAF = 32;
 constexpr int ZTRSM_NC = 4;
 
@@ -156,7 +157,7 @@ __device__ __forceinline__ double lane_bcast_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <bool LOWER>
+template <bool LOWER, bool UNIT>
 __global__ __launch_bounds__(256) void ztrsm_leaf_kernel(int b, int ncols, const zd *__restrict__ T, long ldt,
                                                          zd *__restrict__ B, long ldb) {
   __shared__ zd sT[ZTRSM_LEAF][ZTRSM_LEAF + 1];
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void ztrsm_leaf_kernel(int b, int ncols, const
     const bool in = lane < b && k < b && (LOWER ? k < lane : k > lane);
     mrow[k] = in ? sT[lane][k] : zd{0.0, 0.0};
   }
-  const zd diag = (!LOWER && lane < b) ? sT[lane][lane] : zd{1.0, 0.0};
+  const zd diag = (!UNIT && lane < b) ? sT[lane][lane] : zd{1.0, 0.0};
   const int nwaves = gridDim.x * 4;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   for (int c0 = wave * ZTRSM_NC; c0 < ncols; c0 += nwaves * ZTRSM_NC) {
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(256) void ztrsm_leaf_kernel(int b, int ncols, const
     for (int s = 0; s < ZTRSM_LEAF; s++) {
       const int k = LOWER ? s : ZTRSM_LEAF - 1 - s;
       if (k < b) {
-        if (!LOWER && lane == k) {
+        if (!UNIT && lane == k) {
 #pragma unroll
           for (int q = 0; q < ZTRSM_NC; q++) x[q] = zdiv(x[q], diag);
         }
@@ -200,6 +201,149 @@ __global__ __launch_bounds__(256) void ztrsm_leaf_kernel(int b, int ncols, const
 #pragma unroll
     for (int q = 0; q < ZTRSM_NC; q++)
       if (c0 + q < ncols && lane < b) B[(long)(c0 + q) * ldb + lane] = x[q];
+  }
+}
+
+// B (n x m) = A^H for the complex m x n block A: 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(256) void zconjtrans_kernel(int m, int n, const zd *__restrict__ A, long lda,
+                                                         zd *__restrict__ B, long ldb) {
+  __shared__ zd tile[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int i = i0 + threadIdx.x, j = j0 + r;  // A(i, j), consecutive i per row of threads
+    if (i < m && j < n) tile[r][threadIdx.x] = A[(long)j * lda + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int j = j0 + threadIdx.x, i = i0 + r;  // B(j, i) = conj A(i, j)
+    if (i < m && j < n) {
+      const zd v = tile[threadIdx.x][r];
+      B[(long)i * ldb + j] = zd{v.re, -v.im};
+    }
+  }
+}
+
+// Lower Cholesky of a b x b Hermitian diagonal block (b <= ZPOTRF_LEAF) in
+// one workgroup, PLA_zchol's right-looking order
+// (SOLVERS/Cholesky/PLA_Local_chol.c:210-232): a(j,j) <- sqrt(re a(j,j)) with
+// a zero imaginary part, the column below scaled by its reciprocal, then the
+// Hermitian rank-1 update of the lower trailing block (zher: the updated
+// diagonal keeps a zero imaginary part).  A non-positive (or NaN) real
+// diagonal stops the block and stores offset + j + 1 in *info, as the real
+// path's dchol test does (the reference's zchol has no test).
+constexpr int ZPOTRF_LEAF = 32;
+
+__global__ __launch_bounds__(256) void zpotrf_leaf_kernel(int b, zd *__restrict__ A, long lda, int offset,
+                                                          int *__restrict__ info) {
+  if (*info) return;
+  __shared__ zd S[ZPOTRF_LEAF][ZPOTRF_LEAF + 1];
+  __shared__ int failed;
+  const int t = threadIdx.x;
+  for (int e = t; e < b * b; e += 256) {
+    const int j = e / b, i = e - j * b;
+    if (i >= j) S[i][j] = A[(long)j * lda + i];
+  }
+  if (t == 0) failed = 0;
+  __syncthreads();
+  for (int j = 0; j < b; j++) {
+    const double d = S[j][j].re;
+    if (!(d > 0.0)) {
+      if (t == 0) failed = j + 1;
+      break;
+    }
+    const double r = sqrt(d), inv = 1.0 / r;
+    __syncthreads();
+    if (t == 0) S[j][j] = zd{r, 0.0};
+    for (int i = j + 1 + t; i < b; i += 256) S[i][j] = zd{S[i][j].re * inv, S[i][j].im * inv};
+    __syncthreads();
+    // S(i, c) -= S(i, j) conj(S(c, j)) for j < c <= i
+    for (int e = t; e < b * b; e += 256) {
+      const int c = e / b, i = e - c * b;
+      if (c > j && i >= c) {
+        const zd x = S[i][j], y = S[c][j];
+        const zd u = zd{x.re * y.re + x.im * y.im, x.im * y.re - x.re * y.im};  // x conj(y)
+        S[i][c] = zd{S[i][c].re - u.re, i == c ? 0.0 : S[i][c].im - u.im};
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (failed) {
+    if (t == 0) *info = offset + failed;
+    return;  // the caller restores the block from its backup
+  }
+  for (int e = t; e < b * b; e += 256) {
+    const int j = e / b, i = e - j * b;
+    if (i >= j) A[(long)j * lda + i] = S[i][j];
+  }
+}
+
+// X <- X inv(L)^H, X nrows x b (b <= ZTRSM_LEAF), L lower non-unit: one thread
+// per row (the rows are independent), ztrsm("R","L","C","N")'s order: column k
+// divided by conj(L(k,k)), then conj(L(j,k)) x_k subtracted from columns j > k.
+__global__ __launch_bounds__(256) void ztrsm_rlc_leaf_kernel(int nrows, int b, const zd *__restrict__ L, long ldl,
+                                                             zd *__restrict__ X, long ldx,
+                                                             const int *__restrict__ info) {
+  if (info && *info) return;
+  __shared__ zd sL[ZTRSM_LEAF][ZTRSM_LEAF + 1];
+  for (int e = threadIdx.x; e < b * b; e += 256) {
+    const int c = e / b, i = e - c * b;
+    sL[i][c] = L[(long)c * ldl + i];
+  }
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nrows) return;
+  zd v[ZTRSM_LEAF];
+#pragma unroll
+  for (int c = 0; c < ZTRSM_LEAF; c++) v[c] = c < b ? X[(long)c * ldx + r] : zd{0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < ZTRSM_LEAF; k++) {
+    if (k < b) {
+      const zd d = sL[k][k];
+      v[k] = zdiv(v[k], zd{d.re, -d.im});
+#pragma unroll
+      for (int j = k + 1; j < ZTRSM_LEAF; j++)
+        if (j < b) {
+          const zd l = sL[j][k];
+          v[j] = zsub(v[j], zmul(zd{l.re, -l.im}, v[k]));
+        }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < ZTRSM_LEAF; c++)
+    if (c < b) X[(long)c * ldx + r] = v[c];
+}
+
+// Lower triangle of a diagonal block: C -= X X^H, C n x n (n <= ZHERK_DIAG),
+// X n x k; one thread per element (i >= j), the diagonal's imaginary part set
+// to zero (zher / zherk semantics).
+constexpr int ZHERK_DIAG = 256;
+
+__global__ __launch_bounds__(256) void zherk_diag_kernel(int n, int k, const zd *__restrict__ X, long ldx,
+                                                         zd *__restrict__ C, long ldc, const int *__restrict__ info) {
+  if (info && *info) return;
+  __shared__ zd xi[16][17], xj[16][17];
+  const int i = blockIdx.x * 16 + threadIdx.x, j = blockIdx.y * 16 + threadIdx.y;
+  if (blockIdx.x < blockIdx.y) return;  // tile entirely above the diagonal
+  double sr = 0.0, si = 0.0;
+  for (int l0 = 0; l0 < k; l0 += 16) {
+    // xi[a][l] = X(16 bx + a, l0 + l), xj[a][l] = X(16 by + a, l0 + l)
+    const int l = l0 + threadIdx.y;
+    const int ri = blockIdx.x * 16 + threadIdx.x, rj = blockIdx.y * 16 + threadIdx.x;
+    xi[threadIdx.x][threadIdx.y] = (ri < n && l < k) ? X[(long)l * ldx + ri] : zd{0.0, 0.0};
+    xj[threadIdx.x][threadIdx.y] = (rj < n && l < k) ? X[(long)l * ldx + rj] : zd{0.0, 0.0};
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const zd a = xi[threadIdx.x][q], c = xj[threadIdx.y][q];
+      sr += a.re * c.re + a.im * c.im;  // a conj(c)
+      si += a.im * c.re - a.re * c.im;
+    }
+    __syncthreads();
+  }
+  if (i < n && j < n && i >= j) {
+    zd *p = C + (long)j * ldc + i;
+    *p = zd{p->re - sr, i == j ? 0.0 : p->im - si};
   }
 }
 
@@ -239,15 +383,62 @@ int k_zleaf(hipStream_t s, void *P, long lda, int mp, int w, int pw, int pc0, in
 
 int ztrsm_leaf_size() { return ZTRSM_LEAF; }
 
-int k_ztrsm_leaf(hipStream_t s, bool lower, int b, int ncols, const void *T, long ldt, void *B, long ldb) {
+int k_ztrsm_leaf(hipStream_t s, bool lower, bool unit, int b, int ncols, const void *T, long ldt, void *B,
+                 long ldb) {
   if (b <= 0 || ncols <= 0) return PLA_SUCCESS;
   if (b > ZTRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "complex trsm leaf: b = %d", b);
   const long groups = ((long)ncols + ZTRSM_NC - 1) / ZTRSM_NC, wgs = (groups + 3) / 4;
   const dim3 grid((unsigned)(wgs < 512 ? wgs : 512)), blk(256);
-  if (lower)
-    hipLaunchKernelGGL((ztrsm_leaf_kernel<true>), grid, blk, 0, s, b, ncols, (const zd *)T, ldt, (zd *)B, ldb);
+  const zd *t = (const zd *)T;
+  zd *x = (zd *)B;
+  if (lower && unit)
+    hipLaunchKernelGGL((ztrsm_leaf_kernel<true, true>), grid, blk, 0, s, b, ncols, t, ldt, x, ldb);
+  else if (lower)
+    hipLaunchKernelGGL((ztrsm_leaf_kernel<true, false>), grid, blk, 0, s, b, ncols, t, ldt, x, ldb);
+  else if (unit)
+    hipLaunchKernelGGL((ztrsm_leaf_kernel<false, true>), grid, blk, 0, s, b, ncols, t, ldt, x, ldb);
   else
-    hipLaunchKernelGGL((ztrsm_leaf_kernel<false>), grid, blk, 0, s, b, ncols, (const zd *)T, ldt, (zd *)B, ldb);
+    hipLaunchKernelGGL((ztrsm_leaf_kernel<false, false>), grid, blk, 0, s, b, ncols, t, ldt, x, ldb);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_zconjtrans(hipStream_t s, int m, int n, const void *A, long lda, void *B, long ldb) {
+  if (m <= 0 || n <= 0) return PLA_SUCCESS;
+  const dim3 grid((m + 31) / 32, (n + 31) / 32), blk(32, 8);
+  if (grid.y > 65535) return fail(PLA_ERR_DIMENSIONS, "conjugate transpose: %d columns", n);
+  hipLaunchKernelGGL(zconjtrans_kernel, grid, blk, 0, s, m, n, (const zd *)A, lda, (zd *)B, ldb);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int zpotrf_leaf_size() { return ZPOTRF_LEAF; }
+
+int k_zpotrf_leaf(hipStream_t s, int b, void *A, long lda, int offset, int *info) {
+  if (b <= 0) return PLA_SUCCESS;
+  if (b > ZPOTRF_LEAF) return fail(PLA_ERR_INVALID_PARAM, "complex Cholesky leaf: b = %d", b);
+  hipLaunchKernelGGL(zpotrf_leaf_kernel, dim3(1), dim3(256), 0, s, b, (zd *)A, lda, offset, info);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_ztrsm_rlc_leaf(hipStream_t s, int nrows, int b, const void *L, long ldl, void *X, long ldx, const int *info) {
+  if (nrows <= 0 || b <= 0) return PLA_SUCCESS;
+  if (b > ZTRSM_LEAF) return fail(PLA_ERR_INVALID_PARAM, "complex right trsm leaf: b = %d", b);
+  hipLaunchKernelGGL(ztrsm_rlc_leaf_kernel, dim3((nrows + 255) / 256), dim3(256), 0, s, nrows, b, (const zd *)L,
+                     ldl, (zd *)X, ldx, info);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int zherk_diag_max() { return ZHERK_DIAG; }
+
+int k_zherk_diag(hipStream_t s, int n, int k, const void *X, long ldx, void *C, long ldc, const int *info) {
+  if (n <= 0) return PLA_SUCCESS;
+  if (n > ZHERK_DIAG) return fail(PLA_ERR_INVALID_PARAM, "complex herk diagonal block: n = %d", n);
+  const int t = (n + 15) / 16;
+  hipLaunchKernelGGL(zherk_diag_kernel, dim3(t, t), dim3(16, 16), 0, s, n, k, (const zd *)X, ldx, (zd *)C, ldc,
+                     info);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }

@@ -103,13 +103,13 @@ int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
 constexpr int PLA_NUM_WS = 48;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
-// 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45 complex
-// GEMM operand embedding
+// 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-47 complex
+// (GEMM operand embedding, conjugate transposes, Cholesky info word and block backup)
 enum {
   WS_SRES_PART = 24, WS_SRES_MAX, WS_ONES, WS_NORM_R, WS_MDOT,
   WS_GM_V = 29, WS_GM_W, WS_GM_H, WS_GM_T, WS_GM_Z,
   WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R,
-  WS_API_0 = 43, WS_API_1, WS_ZEMBED
+  WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
@@ -179,14 +179,23 @@ int k_api_scatter_add(hipStream_t s, int elem, int M, int N, const void *alpha, 
                       const int *rmap, const int *cmap, int r0, int c0);
 int k_api_gather(hipStream_t s, int elem, int M, int N, const void *src, long lds_, void *dst, const int *rmap,
                  const int *cmap, int r0, int c0);
-// ---- MPI_DOUBLE_COMPLEX LU (pla_complex.hip, pla_zlu.cpp) ----
+// ---- MPI_DOUBLE_COMPLEX LU (pla_complex.hip, pla_zfactor.cpp) ----
 // H (2m x 2n real, ldh) = the real embedding of the complex m x n block A
 int k_zembed(hipStream_t s, int m, int n, const void *A, long lda, double *H, long ldh);
 int k_zlaswp(hipStream_t s, void *A, long lda, int nrows, int ncols, int np, const int *piv);
 int zleaf_width();
 int k_zleaf(hipStream_t s, void *P, long lda, int mp, int w, int pw, int pc0, int *piv);
 int ztrsm_leaf_size();
-int k_ztrsm_leaf(hipStream_t s, bool lower, int b, int ncols, const void *T, long ldt, void *B, long ldb);
+int k_ztrsm_leaf(hipStream_t s, bool lower, bool unit, int b, int ncols, const void *T, long ldt, void *B,
+                 long ldb);
+int k_zconjtrans(hipStream_t s, int m, int n, const void *A, long lda, void *B, long ldb);  // B = A^H
+int zpotrf_leaf_size();
+int k_zpotrf_leaf(hipStream_t s, int b, void *A, long lda, int offset, int *info);
+int k_ztrsm_rlc_leaf(hipStream_t s, int nrows, int b, const void *L, long ldl, void *X, long ldx, const int *info);
+int zherk_diag_max();
+int k_zherk_diag(hipStream_t s, int n, int k, const void *X, long ldx, void *C, long ldc, const int *info);
+int zchol_factor(hipStream_t s, int n, void *a, long lda, int nb, int *info_out);
+int zchol_solve(hipStream_t s, int n, int nrhs, const void *l, long lda, void *b, long ldb);
 int zlu_factor(hipStream_t s, int m, int n, void *a, long lda, int *piv, int nb);
 int zlu_solve(hipStream_t s, int n, int nrhs, const void *lu, long lda, const int *piv, void *b, long ldb);
 void api_forget(PLA_Obj obj);  // drop a freed object from the open-object list

@@ -1,5 +1,5 @@
 // pla_lu.cpp -- PLA_LU and PLA_General_solve on a 1x1 template, device resident
-// (MPI_DOUBLE here; MPI_DOUBLE_COMPLEX objects go to pla_zlu.cpp).
+// (MPI_DOUBLE here; MPI_DOUBLE_COMPLEX objects go to pla_zfactor.cpp).
 //
 // Reference: SOLVERS/LU/PLA_LU.c:13-212 (blocked right-looking LU with
 // partial pivoting), SOLVERS/LU/PLA_General.c:67-119 (factor + solve).

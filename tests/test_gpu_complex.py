@@ -1,8 +1,11 @@
-"""GPU parity of the MPI_DOUBLE_COMPLEX LU path through the C ABI: PLA_LU
+"""GPU parity of the MPI_DOUBLE_COMPLEX paths through the C ABI: PLA_LU
 (SOLVERS/LU/PLA_LU.c on complex data, the default datatype of the reference's
-example drivers, EXAMPLES/LU/driver.c:110), PLA_General_solve, and
-PLA_Apply_pivots_to_rows, against the oracle's complex restatement
-(ora_zlu_blocked, pinned to LAPACK zgetrf in test_oracle.py).
+example drivers, EXAMPLES/LU/driver.c:110), PLA_General_solve,
+PLA_Apply_pivots_to_rows, PLA_Chol / PLA_Chol_simple (A = L L^H,
+SOLVERS/Cholesky/PLA_Chol.c:83-234; EXAMPLES/Cholesky/driver.c:17) and
+PLA_Pos_def_solve, against the oracle's complex restatements
+(ora_zlu_blocked pinned to LAPACK zgetrf, ora_zchol_blocked to
+numpy.linalg.cholesky, in test_oracle.py).
 
 Bar: pivots equal to the oracle's (integers); factors within the reference's
 sequential-check tolerance 1e-7 * max|A| of the oracle (in practice ~1e-13 is
@@ -118,5 +121,78 @@ def test_complex_unsupported_paths_fail_cleanly(pla_session):
     A = s.matrix(8, 8, pla.MPI_DOUBLE_COMPLEX)
     P = s.mvector(8, dtype=pla.MPI_INT)
     assert pla.lib().PLA_LU_linpack(A, P) == pla.PLA_ERR_UNSUPPORTED
-    assert pla.lib().PLA_Chol(pla.PLA_LOWER_TRIANGULAR, A) == pla.PLA_ERR_UNSUPPORTED
     pla.free(A, P)
+
+
+def _hpd(n, seed, delta=0.1):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    return x @ x.conj().T / n + delta * np.eye(n)
+
+
+def _zchol_gpu(s, a, nb, simple=False):
+    pla.call("PLA_Set_device_nb", nb)
+    A = pla.from_numpy(s, a)
+    if simple:
+        info = pla.call("PLA_Chol_simple", nb, A, ok=range(0, 1 << 30))
+    else:
+        info = pla.call("PLA_Chol", pla.PLA_LOWER_TRIANGULAR, A, ok=range(0, 1 << 30))
+    out = pla.get_contents(A)
+    pla.free(A)
+    pla.call("PLA_Set_device_nb", 0)
+    return out, info
+
+
+@pytest.mark.parametrize("n,nb", [(1, 32), (31, 32), (64, 32), (300, 64), (1000, 256), (2049, 256)])
+def test_zchol_vs_oracle(pla_session, ora, n, nb):
+    a = _hpd(n, n)
+    l, info = _zchol_gpu(pla_session, a, nb)
+    lo, info_o = ora.zchol_blocked(a, nb=nb)
+    assert info == 0 and info_o == 0
+    assert np.abs(np.tril(l) - np.tril(lo)).max() <= 1e-13 * max(1.0, np.abs(a).max()) * np.sqrt(n)
+    assert np.array_equal(np.triu(l, 1), np.triu(a, 1))  # the upper triangle is never written
+    assert np.all(np.diag(l).imag == 0)
+
+
+def test_zchol_simple(pla_session, ora):
+    a = _hpd(700, 5)
+    l, info = _zchol_gpu(pla_session, a, 100, simple=True)
+    lo, _ = ora.zchol_blocked(a, nb=100)
+    assert info == 0
+    assert np.abs(np.tril(l) - np.tril(lo)).max() <= 1e-12 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("n,nb,bad", [(40, 8, 25), (300, 64, 200), (700, 256, 0)])
+def test_zchol_failure_index_and_state(pla_session, ora, n, nb, bad):
+    """A non-positive pivot: its 1-based index, blocks before it factored, its
+    diagonal block as it was before its factorization."""
+    a = _hpd(n, 3)
+    a[bad, bad] = -5.0
+    l, info = _zchol_gpu(pla_session, a, nb)
+    lo, info_o = ora.zchol_blocked(a, nb=nb)
+    assert info == info_o and info > 0
+    k = ((info - 1) // nb) * nb
+    b = min(nb, n - k)
+    tol = 1e-12 * np.abs(a).max()
+    if k > 0:  # the blocks before it factored
+        assert np.abs(np.tril(l[:, :k]) - np.tril(lo[:, :k])).max() <= tol
+    # its diagonal block as the earlier steps left it: a - L10 L10^H on the lower triangle
+    L10 = np.tril(lo)[k:k + b, :k]
+    expect = a[k:k + b, k:k + b] - L10 @ L10.conj().T
+    assert np.abs(np.tril(l[k:k + b, k:k + b] - expect)).max() <= tol
+    assert np.array_equal(np.triu(l, 1), np.triu(a, 1))
+
+
+@pytest.mark.parametrize("n,nrhs", [(500, 1), (1500, 4)])
+def test_zpos_def_solve(pla_session, n, nrhs):
+    s = pla_session
+    a = _hpd(n, 9)
+    rng = np.random.default_rng(2)
+    xs = rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs))
+    b = a @ xs
+    A, B = pla.from_numpy(s, a), pla.from_numpy(s, b)
+    pla.call("PLA_Pos_def_solve", A, B)
+    x = pla.get_contents(B)
+    eta = np.abs(a @ x - b).max() / (np.abs(a).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+    assert eta < 64 * n * EPS
+    pla.free(A, B)

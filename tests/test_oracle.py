@@ -297,6 +297,31 @@ def test_zlu_exact_pivot_rule(ora):
     assert piv[0] == 2
 
 
+@pytest.mark.parametrize("n,nb", [(30, 8), (64, 16), (97, 32)])
+def test_zchol_blocked_vs_numpy(ora, n, nb):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a = x @ x.conj().T / n + 0.1 * np.eye(n)
+    f, info = ora.zchol_blocked(a, nb=nb)
+    assert info == 0
+    ref = np.linalg.cholesky(a)
+    assert np.abs(np.tril(f) - ref).max() <= 1e-12 * np.abs(a).max()
+    assert np.array_equal(np.triu(f, 1), np.triu(a, 1))  # the upper triangle is never written
+    assert np.all(np.diag(f).imag == 0)
+
+
+def test_zchol_failure_index(ora):
+    """The first non-positive leading minor (exact signs on a small integer case)."""
+    rng = np.random.default_rng(1)
+    x = rng.integers(-3, 4, (12, 12)) + 1j * rng.integers(-3, 4, (12, 12))
+    a = x @ x.conj().T + np.eye(12)
+    a[7, 7] -= 400.0
+    minors = [np.linalg.det(a[:k, :k]).real for k in range(1, 13)]
+    first = next(k for k, d in enumerate(minors, 1) if d <= 0)
+    _, info = ora.zchol_blocked(a, nb=4)
+    assert info == first
+
+
 def test_zlu_solve_and_stream(ora):
     n = 80
     ab = ora.uniform_z(n, n + 1, seed=2)
