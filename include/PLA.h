@@ -226,8 +226,9 @@ int PLA_Copy(PLA_Obj from, PLA_Obj to);
 /* ---- application interface (API/PLA_API.c; prototypes as in
    INCLUDE/PLA_API_prototypes.h).  Host buffers are added into objects in HBM
    (*_to_global) or pieces of objects added into host buffers (global_to_*).
-   The objects must be PLA_MATRIX or PLA_MVECTOR of MPI_DOUBLE, MPI_FLOAT or
-   MPI_INT, opened with PLA_Obj_API_open inside PLA_API_begin / PLA_API_end.
+   The objects must be PLA_MATRIX or PLA_MVECTOR of MPI_DOUBLE,
+   MPI_DOUBLE_COMPLEX, MPI_FLOAT or MPI_INT, opened with PLA_Obj_API_open inside
+   PLA_API_begin / PLA_API_end.
    A *_to_global call may return before the add is done (the buffer is
    already copied and may be reused); PLA_Obj_API_sync / _close / PLA_API_end
    complete it.  A global_to_* call returns with the buffer updated.  alpha

@@ -56,6 +56,7 @@ int elem_kind(MPI_Datatype dt) {
   if (dt == MPI_DOUBLE) return API_F64;
   if (dt == MPI_FLOAT) return API_F32;
   if (dt == MPI_INT) return API_I32;
+  if (dt == MPI_DOUBLE_COMPLEX) return API_Z64;
   return 0;
 }
 
@@ -103,7 +104,7 @@ int check_target(PLA_Obj A, const char *who) {
   if (A->objtype != PLA_MATRIX && A->objtype != PLA_MVECTOR)
     return fail(PLA_ERR_INVALID_PARAM, "%s: Illegal target object type", who);
   if (!elem_kind(A->base->datatype))
-    return fail(PLA_ERR_UNSUPPORTED, "%s: only MPI_DOUBLE, MPI_FLOAT and MPI_INT objects", who);
+    return fail(PLA_ERR_UNSUPPORTED, "%s: only MPI_DOUBLE, MPI_DOUBLE_COMPLEX, MPI_FLOAT and MPI_INT objects", who);
   return PLA_SUCCESS;
 }
 
@@ -206,6 +207,19 @@ int to_global(PLA_Obj A, int M, int N, const void *alpha, const void *buf, int l
   return PLA_SUCCESS;
 }
 
+struct zpair {
+  double re, im;
+};
+inline zpair operator*(zpair a, zpair b) {
+#pragma clang fp contract(off)
+  return zpair{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+inline zpair &operator+=(zpair &a, zpair b) {
+  a.re += b.re;
+  a.im += b.im;
+  return a;
+}
+
 template <typename T>
 void add_scaled(int M, int nc, const void *alpha, const char *packed, void *buf, int ldim, int inc, int j0) {
 #pragma clang fp contract(off)  // alpha * x rounded, then added, as on the device
@@ -267,6 +281,8 @@ int from_global(PLA_Obj A, int M, int N, const void *alpha, void *buf, int ldim,
       add_scaled<double>(M, cur.nc, alpha, h, buf, ldim, inc, cur.j0);
     else if (kind == API_F32)
       add_scaled<float>(M, cur.nc, alpha, h, buf, ldim, inc, cur.j0);
+    else if (kind == API_Z64)
+      add_scaled<zpair>(M, cur.nc, alpha, h, buf, ldim, inc, cur.j0);
     else
       add_scaled<int>(M, cur.nc, alpha, h, buf, ldim, inc, cur.j0);
     if (nxt.slot < 0) break;
@@ -279,8 +295,12 @@ const void *one_of(PLA_Obj A) {
   static const double d1 = 1.0;
   static const float f1 = 1.0f;
   static const int i1 = 1;
+  static const zpair z1 = {1.0, 0.0};
   const int k = elem_kind(A->base->datatype);
-  return k == API_F64 ? (const void *)&d1 : k == API_F32 ? (const void *)&f1 : (const void *)&i1;
+  return k == API_F64 ? (const void *)&d1
+         : k == API_F32 ? (const void *)&f1
+         : k == API_Z64 ? (const void *)&z1
+                        : (const void *)&i1;
 }
 
 // One rectangular block or a grid of sub-blocks, in either direction.  The

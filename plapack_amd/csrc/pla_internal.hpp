@@ -174,7 +174,7 @@ int k_trsm_right_leaf(hipStream_t s, bool lower, bool trans, bool unit, int nrow
 int k_potrf_leaf(hipStream_t s, int b, double *A, long lda, int offset, int *info);
 // application-interface transfers of a packed M x N block (leading dimension
 // M) to / from its place in an object (maps or offsets r0, c0)
-enum { API_F64 = 1, API_F32 = 2, API_I32 = 3 };
+enum { API_F64 = 1, API_F32 = 2, API_I32 = 3, API_Z64 = 4 };
 int k_api_scatter_add(hipStream_t s, int elem, int M, int N, const void *alpha, const void *src, void *dst, long ldd,
                       const int *rmap, const int *cmap, int r0, int c0);
 int k_api_gather(hipStream_t s, int elem, int M, int N, const void *src, long lds_, void *dst, const int *rmap,

@@ -1033,6 +1033,17 @@ __global__ __launch_bounds__(256) void copy2d_kernel(int m, int n, const double 
 // rmap[i] / column cmap[j] (or r0 + i / c0 + j without a map), so many
 // sub-blocks of a multi-axpy go in one launch.  Rows run over x (coalesced
 // within a column), columns over y.
+// MPI_DOUBLE_COMPLEX elements of the application interface: alpha x + d with
+// the complex product rounded before the add
+struct api_z {
+  double re, im;
+};
+__device__ __forceinline__ api_z operator*(api_z a, api_z b) {
+#pragma clang fp contract(off)
+  return api_z{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ api_z operator+(api_z a, api_z b) { return api_z{a.re + b.re, a.im + b.im}; }
+
 template <typename T>
 __global__ __launch_bounds__(256) void api_scatter_add_kernel(int M, int N, T alpha, const T *__restrict__ src,
                                                               T *__restrict__ dst, long ldd,
@@ -1335,6 +1346,7 @@ int k_api_scatter_add(hipStream_t s, int elem, int M, int N, const void *alpha, 
     case API_F64: return api_scatter_t<double>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
     case API_F32: return api_scatter_t<float>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
     case API_I32: return api_scatter_t<int>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
+    case API_Z64: return api_scatter_t<api_z>(s, M, N, alpha, src, dst, ldd, rmap, cmap, r0, c0);
   }
   return fail(PLA_ERR_UNSUPPORTED, "api scatter: element kind %d", elem);
 }
@@ -1346,6 +1358,7 @@ int k_api_gather(hipStream_t s, int elem, int M, int N, const void *src, long ld
     case API_F64: return api_gather_t<double>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
     case API_F32: return api_gather_t<float>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
     case API_I32: return api_gather_t<int>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
+    case API_Z64: return api_gather_t<api_z>(s, M, N, src, lds_, dst, rmap, cmap, r0, c0);
   }
   return fail(PLA_ERR_UNSUPPORTED, "api gather: element kind %d", elem);
 }

@@ -207,6 +207,28 @@ def test_float_and_int_objects(api):
     pla.free(fa, ia)
 
 
+def test_double_complex_objects(api):
+    """MPI_DOUBLE_COMPLEX: complex alpha, one rounded complex product and one add per element."""
+    s = api
+    rng = np.random.default_rng(21)
+    a0 = rng.standard_normal((90, 70)) + 1j * rng.standard_normal((90, 70))
+    A = pla.from_numpy(s, a0)
+    pla.call("PLA_Obj_API_open", A)
+    buf = np.asfortranarray(rng.standard_normal((40, 30)) + 1j * rng.standard_normal((40, 30)))
+    alpha = np.array([0.5 - 2.0j])
+    pla.call("PLA_API_axpy_matrix_to_global", 40, 30, _ptr(alpha), _ptr(buf), 40, A, 11, 22)
+    loc = np.asfortranarray(rng.standard_normal((40, 30)) + 1j * rng.standard_normal((40, 30)))
+    loc0 = loc.copy()
+    beta = np.array([-1.0 + 0.25j])
+    pla.call("PLA_API_axpy_global_to_matrix", 40, 30, _ptr(beta), A, 11, 22, _ptr(loc), 40)
+    pla.call("PLA_Obj_API_close", A)
+    ref = a0.copy()
+    ref[11:51, 22:52] += alpha[0] * buf
+    assert np.array_equal(pla.get_contents(A), ref)
+    assert np.array_equal(loc, loc0 + beta[0] * ref[11:51, 22:52])
+    pla.free(A)
+
+
 def test_host_mirror_sees_closed_adds(api):
     """A local-buffer mirror taken before the adds holds them after close."""
     s = api
