@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void zlaswp_kernel(zd *__restrict__ A, long ld
 // first maximum of |re| + |im|), the interchange across all pw panel columns,
 // a_21 scaled by the pivot's reciprocal, the rank-1 update (zgeru) of the
 // leaf's columns j+1..w-1.  piv[j] is the pivot row's offset from row j.
-constexpr int ZLEAF_W = 8;
+constexpr int ZLEAF_W = 4;
 constexpr int ZLEAF_THREADS = 1024;
 
 __global__ __launch_bounds__(ZLEAF_THREADS) void zleaf_kernel(zd *__restrict__ P, long lda, int mp, int w, int pw,
