@@ -100,16 +100,17 @@ int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
-constexpr int PLA_NUM_WS = 48;
+constexpr int PLA_NUM_WS = 49;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
-// 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-47 complex
-// (GEMM operand embedding, conjugate transposes, Cholesky info word and block backup)
+// 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-48 complex
+// (GEMM operand embedding, conjugate transposes, Cholesky info word and block backup,
+// the look-ahead panel's operand embedding)
 enum {
   WS_SRES_PART = 24, WS_SRES_MAX, WS_ONES, WS_NORM_R, WS_MDOT,
   WS_GM_V = 29, WS_GM_W, WS_GM_H, WS_GM_T, WS_GM_Z,
   WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R,
-  WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO
+  WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO, WS_ZEMBED_P
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();

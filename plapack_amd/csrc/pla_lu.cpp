@@ -185,7 +185,7 @@ static int lu_entry(PLA_Obj A, PLA_Obj pivots, bool swap_left) {
   if ((rc = dev_ptr(A, (void **)&a)) || (rc = dev_ptr(pivots, (void **)&piv))) return rc;
   if (A->base->datatype == MPI_DOUBLE_COMPLEX) {
     if (!swap_left) return fail(PLA_ERR_UNSUPPORTED, "PLA_LU_linpack: MPI_DOUBLE_COMPLEX is not supported");
-    const int nbz = ctx().nb_device > 0 ? ctx().nb_device : (ctx().nb_user > 0 ? ctx().nb_user : 256);
+    const int nbz = ctx().nb_device > 0 ? ctx().nb_device : (ctx().nb_user > 0 ? ctx().nb_user : 512);
     if ((rc = zlu_factor(ctx().stream, m, n, a, ld(A), piv, nbz))) return rc;
     if ((rc = written(A))) return rc;
     return written(pivots);

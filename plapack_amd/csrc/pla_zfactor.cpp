@@ -35,10 +35,11 @@ int split_z(int n, int q) {
 
 // C -= A B, A m x k, B k x n, C m x n (complex, column-major); skipped once
 // *abort_flag is set
+// (ws: the workspace slot of the embedded operand -- one per stream)
 int zgemm_sub(hipStream_t s, int m, int n, int k, const zc *A, long lda, const zc *B, long ldb, zc *C, long ldc,
-              const int *abort_flag = nullptr) {
+              const int *abort_flag = nullptr, int ws = WS_ZEMBED) {
   if (m <= 0 || n <= 0 || k <= 0) return PLA_SUCCESS;
-  double *H = (double *)workspace(sizeof(double) * 4 * (size_t)m * k, WS_ZEMBED);
+  double *H = (double *)workspace(sizeof(double) * 4 * (size_t)m * k, ws);
   if (!H) return PLA_ERR_NO_MEMORY;
   int rc;
   if ((rc = k_zembed(s, m, k, A, lda, H, 2L * m))) return rc;
@@ -47,7 +48,8 @@ int zgemm_sub(hipStream_t s, int m, int n, int k, const zc *A, long lda, const z
 }
 
 // B <- inv(T) B, T n x n lower or upper, unit or not, B n x nrhs
-int ztrsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, const zc *T, long ldt, zc *B, long ldb) {
+int ztrsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, const zc *T, long ldt, zc *B, long ldb,
+               int ws = WS_ZEMBED) {
   if (n <= 0 || nrhs <= 0) return PLA_SUCCESS;
   const int leaf = ztrsm_leaf_size();
   if (n <= leaf) return k_ztrsm_leaf(s, lower, unit, n, nrhs, T, ldt, B, ldb);
@@ -55,13 +57,13 @@ int ztrsm_left(hipStream_t s, bool lower, bool unit, int n, int nrhs, const zc *
   const zc *T11 = T + (long)n0 * ldt + n0;
   int rc;
   if (lower) {
-    if ((rc = ztrsm_left(s, true, unit, n0, nrhs, T, ldt, B, ldb))) return rc;
-    if ((rc = zgemm_sub(s, n1, nrhs, n0, T + n0, ldt, B, ldb, B + n0, ldb))) return rc;
-    return ztrsm_left(s, true, unit, n1, nrhs, T11, ldt, B + n0, ldb);
+    if ((rc = ztrsm_left(s, true, unit, n0, nrhs, T, ldt, B, ldb, ws))) return rc;
+    if ((rc = zgemm_sub(s, n1, nrhs, n0, T + n0, ldt, B, ldb, B + n0, ldb, nullptr, ws))) return rc;
+    return ztrsm_left(s, true, unit, n1, nrhs, T11, ldt, B + n0, ldb, ws);
   }
-  if ((rc = ztrsm_left(s, false, unit, n1, nrhs, T11, ldt, B + n0, ldb))) return rc;
-  if ((rc = zgemm_sub(s, n0, nrhs, n1, T + (long)n0 * ldt, ldt, B + n0, ldb, B, ldb))) return rc;
-  return ztrsm_left(s, false, unit, n0, nrhs, T, ldt, B, ldb);
+  if ((rc = ztrsm_left(s, false, unit, n1, nrhs, T11, ldt, B + n0, ldb, ws))) return rc;
+  if ((rc = zgemm_sub(s, n0, nrhs, n1, T + (long)n0 * ldt, ldt, B + n0, ldb, B, ldb, nullptr, ws))) return rc;
+  return ztrsm_left(s, false, unit, n0, nrhs, T, ldt, B, ldb, ws);
 }
 
 // X <- X inv(L)^H, X nrows x b, L b x b lower non-unit
@@ -119,31 +121,69 @@ int zpanel_rec(hipStream_t s, zc *panel, int mp, int pw, long lda, int *piv, int
   int rc;
   if ((rc = zpanel_rec(s, panel, mp, pw, lda, piv, c0, w0))) return rc;
   zc *Pr = P + (long)w0 * lda;
-  if ((rc = ztrsm_left(s, true, true, w0, w1, P, lda, Pr, lda))) return rc;
-  if ((rc = zgemm_sub(s, m - w0, w1, w0, P + w0, lda, Pr, lda, Pr + w0, lda))) return rc;
+  if ((rc = ztrsm_left(s, true, true, w0, w1, P, lda, Pr, lda, WS_ZEMBED_P))) return rc;
+  if ((rc = zgemm_sub(s, m - w0, w1, w0, P + w0, lda, Pr, lda, Pr + w0, lda, nullptr, WS_ZEMBED_P))) return rc;
   return zpanel_rec(s, panel, mp, pw, lda, piv, c0 + w0, w1);
 }
 
 }  // namespace
 
-int zlu_factor(hipStream_t s, int m, int n, void *av, long lda, int *piv, int nb) {
+// Blocked right-looking LU with one step of look-ahead, as lu_factor: the
+// panel of step k+1 is factored on the look-ahead stream (its GEMMs on their
+// own operand workspace) while the trailing stream finishes step k's update;
+// the next panel's columns are interchanged, solved and updated first.  The
+// interchanges left of a panel go last in their step, after the update that
+// reads the previous panel's L21.
+int zlu_factor(hipStream_t sMain, int m, int n, void *av, long lda, int *piv, int nb) {
   zc *a = (zc *)av;
   const int mn = m < n ? m : n;
   if (nb < 1) nb = 1;
-  for (int k = 0, b; k < mn; k += b) {
-    b = mn - k < nb ? mn - k : nb;
-    zc *Akk = a + (long)k * lda + k;
-    int rc;
-    if ((rc = zpanel_rec(s, Akk, m - k, b, lda, piv + k, 0, b))) return rc;
-    const int nr = n - k - b, mr = m - k - b;
-    if (k > 0 && (rc = k_zlaswp(s, a + k, lda, m - k, k, b, piv + k))) return rc;
-    if (nr <= 0) continue;
-    zc *A12 = Akk + (long)b * lda;
-    if ((rc = k_zlaswp(s, A12, lda, m - k, nr, b, piv + k))) return rc;
-    if ((rc = ztrsm_left(s, true, true, b, nr, Akk, lda, A12, lda))) return rc;
-    if (mr > 0 && (rc = zgemm_sub(s, mr, nr, b, Akk + b, lda, A12, lda, A12 + b, lda))) return rc;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
+  if (!ev_ready) {
+    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
+    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
   }
-  return PLA_SUCCESS;
+  hipStream_t sP, sM;
+  int gcap, rc;
+  if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
+  auto width_at = [&](int k) { return mn - k < nb ? mn - k : nb; };
+  auto panel = [&](int k, int b) -> int {
+    return zpanel_rec(sP, a + (long)k * lda + k, m - k, b, lda, piv + k, 0, b);
+  };
+  auto steps = [&]() -> int {
+    int r;
+    if ((r = panel(0, width_at(0)))) return r;
+    PLA_HIP(hipEventRecord(ev_panel, sP));
+    for (int k = 0, b; k < mn; k += b) {
+      b = width_at(k);
+      PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
+      zc *Akk = a + (long)k * lda + k;
+      const int kn = k + b, nr = n - kn, mr = m - kn;
+      const int bn = (kn < mn && mr > 0) ? width_at(kn) : 0;
+      const int n1 = bn > 0 ? bn : nr;
+      auto right_part = [&](int c0, int nc) -> int {
+        if (nc <= 0) return PLA_SUCCESS;
+        zc *A12 = Akk + (long)(b + c0) * lda;
+        int q;
+        if ((q = k_zlaswp(sM, A12, lda, m - k, nc, b, piv + k))) return q;
+        if ((q = ztrsm_left(sM, true, true, b, nc, Akk, lda, A12, lda))) return q;
+        return mr > 0 ? zgemm_sub(sM, mr, nc, b, Akk + b, lda, A12, lda, A12 + b, lda) : PLA_SUCCESS;
+      };
+      if ((r = right_part(0, n1))) return r;
+      if (bn > 0) {
+        PLA_HIP(hipEventRecord(ev_ready, sM));
+        PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+        if ((r = panel(kn, bn))) return r;
+        PLA_HIP(hipEventRecord(ev_panel, sP));
+      }
+      if ((r = right_part(n1, nr - n1))) return r;
+      if (k > 0 && (r = k_zlaswp(sM, a + k, lda, m - k, k, b, piv + k))) return r;
+    }
+    return PLA_SUCCESS;
+  };
+  rc = steps();
+  const int rc_end = lu_streams_end(sMain, sP, sM);
+  return rc ? rc : rc_end;
 }
 
 int zlu_solve(hipStream_t s, int n, int nrhs, const void *lu, long lda, const int *piv, void *b, long ldb) {
