@@ -43,7 +43,7 @@ struct lu_bufs {
 
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
                            long lda, int *piv, int nb, bool swap_left, const lu_bufs &bf, hipEvent_t ev_ready,
-                           hipEvent_t ev_panel, hipEvent_t ev_step);
+                           hipEvent_t ev_panel);
 
 // Blocked right-looking LU of the m x n column-major device matrix a, with
 // one step of look-ahead: panel k+1 is factored on the look-ahead stream as
@@ -70,11 +70,10 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(m, m < n ? m : n, nb) * cstride, 4);
   if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
   const lu_bufs bf{pairs, pstride, comp, cstride};
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr, ev_step = nullptr;
+  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
   if (!ev_ready) {
     PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
   }
   int rc;
   // sP: panels; sM: interchanges right of the panel, A12 solves and trailing
@@ -82,7 +81,7 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   hipStream_t sP, sM;
   int gcap;
   if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
-  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, swap_left, bf, ev_ready, ev_panel, ev_step);
+  rc = lu_factor_steps(sP, sM, sMain, gcap, m, n, a, lda, piv, nb, swap_left, bf, ev_ready, ev_panel);
   // the main stream waits for both streams, also after an error
   const int rc_end = lu_streams_end(sMain, sP, sM);
   if (rc) return rc;
@@ -93,11 +92,12 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
 // The interchanges of panel k on the columns left of it go to sL when it
 // differs from sM, off the trailing-update stream, else to the end of step k
 // on sM.  Those columns include panel k-1's L21, which step k-1's trailing
-// update reads: sL waits for the end of step k-1 on sM (ev_step) before
-// permuting them.  Step k's composite list is comp + k * cstride.
+// update reads: sL waits for step k's next-panel columns on sM (ev_ready),
+// which follow the whole of step k-1 there, before permuting them.  Step k's
+// composite list is comp + k * cstride.
 static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int gcap, int m, int n, double *a,
                            long lda, int *piv, int nb, bool swap_left, const lu_bufs &bf, hipEvent_t ev_ready,
-                           hipEvent_t ev_panel, hipEvent_t ev_step) {
+                           hipEvent_t ev_panel) {
   int *const pairs = bf.pairs, *const comp = bf.comp;
   const size_t pstride = bf.pstride;
   const int cstride = bf.cstride;
@@ -131,11 +131,6 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
     const int nr = n - k - b, mr = m - k - b;
     const int *cp = comp + (size_t)step * cstride;
-    if (split_left && swap_left) {
-      PLA_HIP(hipStreamWaitEvent(sL, ev_panel, 0));
-      if (k > 0) PLA_HIP(hipStreamWaitEvent(sL, ev_step, 0));  // step k-1 no longer reads panel k-1's L21
-      if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
-    }
     // Right of the panel, the next panel's columns go first (interchanges,
     // A12 solve, update): they gate panel k+1.  The other columns follow
     // once panel k+1 has been released.
@@ -143,10 +138,13 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
     const int bn = (kn < mn && mr > 0) ? width_at(kn) : 0;  // next panel's width
     const int n1 = bn > 0 ? bn : nr;
     double *A12 = a + (long)kn * lda + k;
-    auto right_part = [&](int c0, int nc) -> int {  // columns kn + c0 .. kn + c0 + nc
+    // columns kn + c0 .. kn + c0 + nc: interchanges, then A12 solve and update
+    auto swap_part = [&](int c0, int nc) -> int {
+      return nc > 0 ? k_panel_swap(sM, a + k, lda, 0, kn + c0, kn + c0 + nc, cp) : PLA_SUCCESS;
+    };
+    auto update_part = [&](int c0, int nc) -> int {
       if (nc <= 0) return PLA_SUCCESS;
       int r;
-      if ((r = k_panel_swap(sM, a + k, lda, 0, kn + c0, kn + c0 + nc, cp))) return r;
       double *A12c = A12 + (long)c0 * lda;
       if ((r = trsm_left(sM, true, false, true, b, nc, Akk, lda, A12c, lda, nullptr))) return r;
       if (mr <= 0) return PLA_SUCCESS;
@@ -155,17 +153,23 @@ static int lu_factor_steps(hipStream_t sP, hipStream_t sM, hipStream_t sL, int g
       timing_end(sM, 1, 2.0 * mr * (double)nc * b);
       return r;
     };
-    if ((rc = right_part(0, n1))) return rc;
+    if ((rc = swap_part(0, n1)) || (rc = update_part(0, n1))) return rc;
+    PLA_HIP(hipEventRecord(ev_ready, sM));
+    if (split_left && swap_left) {
+      // After step k's next-panel columns: by then step k-1's update no
+      // longer reads panel k-1's L21, and the swap does not compete for CUs
+      // with the short kernels that gate panel k+1.
+      PLA_HIP(hipStreamWaitEvent(sL, ev_ready, 0));
+      if ((rc = k_panel_swap(sL, a + k, lda, k, 0, 0, cp))) return rc;
+    }
     if (bn > 0) {
-      PLA_HIP(hipEventRecord(ev_ready, sM));
       PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
       if ((rc = panel(kn, bn, par ^ 1, step + 1, &nleaves[par ^ 1]))) return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
-    if ((rc = right_part(n1, nr - n1))) return rc;
+    if ((rc = swap_part(n1, nr - n1)) || (rc = update_part(n1, nr - n1))) return rc;
     // left of the panel last: nothing reads L before the factorization ends
     if (swap_left && !split_left && (rc = k_panel_swap(sM, a + k, lda, k, 0, 0, cp))) return rc;
-    if (swap_left && split_left) PLA_HIP(hipEventRecord(ev_step, sM));
   }
   return PLA_SUCCESS;
 }

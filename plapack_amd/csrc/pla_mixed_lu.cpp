@@ -90,10 +90,6 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       const int kpad = (b + 31) / 32 * 32;  // the update's inner dimension (copies are zero beyond b)
       PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
       const int *cp = comp + (size_t)step * cstride;
-      if (split_left) {
-        PLA_HIP(hipStreamWaitEvent(sMain, ev_panel, 0));
-        if ((rc = k_panel_swap_f32(sMain, W + k, ldw, k, 0, 0, cp))) return rc;
-      }
       const int nr = n - k - b, mr = nr;
       const int kn = k + b;
       const int bn = kn < n ? width_at(kn) : 0;  // next panel's width
@@ -121,8 +117,12 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
         return r;
       };
       if ((rc = right_part(0, n1))) return rc;
+      PLA_HIP(hipEventRecord(ev_ready, sM));
+      if (split_left) {  // after the next-panel columns, which gate panel k+1
+        PLA_HIP(hipStreamWaitEvent(sMain, ev_ready, 0));
+        if ((rc = k_panel_swap_f32(sMain, W + k, ldw, k, 0, 0, cp))) return rc;
+      }
       if (bn > 0) {
-        PLA_HIP(hipEventRecord(ev_ready, sM));
         PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
         if ((rc = panel(kn, bn, par ^ 1, step + 1))) return rc;
         PLA_HIP(hipEventRecord(ev_panel, sP));
