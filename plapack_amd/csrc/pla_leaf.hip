@@ -8,14 +8,18 @@
 // (Local_BLAS/PLA_Local_BLAS1.c:258) one reciprocal and a multiply, and
 // PLA_Local_ger (Local_BLAS/PLA_Local_BLAS2.c:578-656) a rank-1 update.
 //
-// Layout.  A workgroup is ONE wave of 64 lanes and uses no LDS; lane l keeps
-// rows r0 + l + 64 i (i < RPL) of the leaf, all W columns, in registers
-// (RPL * W = 64 doubles per lane).  Without LDS and with about 200 VGPRs a
-// leaf wave fits on a CU beside two workgroups of the trailing-update GEMM
-// (FP64: 2 x 68.6 KB of LDS, 2 x 4 waves of 128 VGPRs; FP16/BF16: 2 x 80 KB,
-// 88 VGPRs), so the leaf never waits for trailing-update workgroups to drain,
-// and the waves of one leaf (all of which must be resident at once: they
-// exchange pivot candidates every column) are placed as soon as it launches.
+// Layout.  Each wave of 64 lanes is one participant and uses no LDS; lane l
+// keeps rows r0 + l + 64 i (i < RPL) of the wave's share of the leaf, all W
+// columns, in registers (RPL * W = 64 doubles per lane, about 160 VGPRs).
+// Waves are launched four to a workgroup (normally one per SIMD of a CU), so a leaf of
+// G waves occupies G / 4 CUs.  Beside the FP16x3 trailing update (two
+// workgroups of 4 waves of 156 VGPRs and 80 KB of LDS per CU) a leaf
+// workgroup fits without displacing it; beside the FP64 update (256 VGPRs per
+// wave, two waves per SIMD) it takes the place of one update workgroup on
+// each of its G / 4 CUs, not on G CUs as single-wave workgroups spread over
+// the chip would.  All waves of one leaf must be resident at once (they
+// exchange pivot candidates every column); a bounded wait turns a failure to
+// become resident into an error.
 //
 // Per column j (rows are never moved; each carries its logical position,
 // initially its row number; positions < j hold the pivot rows chosen so far):
@@ -49,7 +53,8 @@
 
 namespace {
 
-constexpr int RL_MAXG = 512;     // workgroups (waves) of one leaf
+constexpr int RL_MAXG = 512;     // waves of one leaf
+constexpr int RL_WG_WAVES = 4;   // waves per workgroup (one per SIMD of a CU)
 constexpr int RL_MAXW = 32;
 constexpr int RL_ROWW = 2 * RL_MAXW;                    // words of one candidate row
 constexpr int RL_PAR = 3 * RL_MAXG + RL_MAXG * RL_ROWW;  // words of one column parity
@@ -127,10 +132,11 @@ __device__ inline bool spin_fail(unsigned &it, const rleaf_args &a) {
 }
 
 template <int W>
-__global__ __launch_bounds__(64, 2) void lu_rleaf_kernel(rleaf_args a) {
+__global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
   constexpr int RPL = 64 / W;  // rows per lane
-  constexpr int R = 64 * RPL;  // rows per workgroup
-  const int G = gridDim.x, g = blockIdx.x, lane = threadIdx.x;
+  constexpr int R = 64 * RPL;  // rows per wave
+  // G waves in workgroups of RL_WG_WAVES; wave g is one participant
+  const int G = gridDim.x * RL_WG_WAVES, g = blockIdx.x * RL_WG_WAVES + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int w = a.w, mp = a.mp;
   const long lda = a.lda;
   if (ld_dev(a.err)) {  // a timed-out exchange earlier on this stream: no interchanges
@@ -373,9 +379,11 @@ int k_lu_rleaf(hipStream_t s, const lu_leaf_call &c, int W) {
   if ((W != 8 && W != 16 && W != 32) || c.w > W || c.mp < c.w)
     return fail(PLA_ERR_INVALID_PARAM, "lu leaf: %d x %d in a width-%d leaf", c.mp, c.w, W);
   const int R = rleaf_rows(W);
-  const int G = (c.mp + R - 1) / R;
+  // whole workgroups of waves; the waves past the last row hold no rows
+  const int G = ((c.mp + R - 1) / R + RL_WG_WAVES - 1) / RL_WG_WAVES * RL_WG_WAVES;
   if (G > RL_MAXG || (c.gcap > 0 && G > c.gcap))
     return fail(PLA_ERR_DIMENSIONS, "lu leaf: %d rows need %d waves of width %d", c.mp, G, W);
+  const dim3 grid(G / RL_WG_WAVES), block(64 * RL_WG_WAVES);
   const size_t slot_bytes = (size_t)2 * RL_PAR * sizeof(unsigned long long);
   leaf_ws &ws = leaf_workspace(slot_bytes);
   if (!ws.base) return PLA_ERR_NO_MEMORY;
@@ -397,9 +405,9 @@ int k_lu_rleaf(hipStream_t s, const lu_leaf_call &c, int W) {
   a.tag0 = tag0;
   a.spin_limit = ctx().leaf_spin_limit;
   switch (W) {
-    case 8: hipLaunchKernelGGL(lu_rleaf_kernel<8>, dim3(G), dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(lu_rleaf_kernel<16>, dim3(G), dim3(64), 0, s, a); break;
-    default: hipLaunchKernelGGL(lu_rleaf_kernel<32>, dim3(G), dim3(64), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(lu_rleaf_kernel<8>, grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL(lu_rleaf_kernel<16>, grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL(lu_rleaf_kernel<32>, grid, block, 0, s, a); break;
   }
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
