@@ -877,22 +877,18 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int b, int ncols, const 
                                                         long ldt, double *__restrict__ B, long ldb,
                                                         const int *__restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ double sT[TRSM_LEAF][TRSM_LEAF + 1];  // sT[i][k] = T(i,k)
-  for (int e = threadIdx.x; e < b * b; e += 256) {
-    const int kcol = e / b, i = e - kcol * b;
-    sT[i][kcol] = T[(long)kcol * ldt + i];
-  }
-  __syncthreads();
   constexpr bool FORWARD = (LOWER != TRANS);  // op(T) lower-type
   const int lane = threadIdx.x & 63;
-  // row `lane` of M = op(T), strictly on the solved side of the diagonal
+  // row `lane` of M = op(T), strictly on the solved side of the diagonal,
+  // read straight from T (64 x 64 at most, cached): no LDS, so the leaf can
+  // run beside trailing-update workgroups that hold all of a CU's LDS
   double mrow[TRSM_LEAF];
 #pragma unroll
   for (int k = 0; k < TRSM_LEAF; k++) {
     const bool in = lane < b && k < b && (FORWARD ? k < lane : k > lane);
-    mrow[k] = in ? (TRANS ? sT[k][lane] : sT[lane][k]) : 0.0;
+    mrow[k] = in ? (TRANS ? T[(long)lane * ldt + k] : T[(long)k * ldt + lane]) : 0.0;
   }
-  const double diag = (!UNIT && lane < b) ? sT[lane][lane] : 1.0;
+  const double diag = (!UNIT && lane < b) ? T[(long)lane * ldt + lane] : 1.0;
   const int nwaves = gridDim.x * 4;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   for (int c0 = wave * TRSM_NC; c0 < ncols; c0 += nwaves * TRSM_NC) {
