@@ -789,23 +789,14 @@ __global__ __launch_bounds__(256) void panel_compose_kernel(const int *__restric
 // Full-width interchanges of one panel: for every column outside the panel
 // (ranges [0, nleft) and [c1, c2)) apply the composite (dst, src) list, one
 // column per wave: every lane loads all its sources before any lane stores.
-// LDS: the list is staged in LDS; otherwise each lane reads its entries from
-// the list itself (16 KB at most, cached) and the kernel needs no LDS, so it
-// can run beside trailing-update workgroups that hold all of a CU's LDS.
-template <typename T, bool LDS>
+// Each lane reads its (dst, src) entries from the list itself (16 KB at most,
+// cached), so the kernel uses no LDS and can run beside trailing-update and
+// panel workgroups that hold most of a CU's LDS.
+template <typename T>
 __global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long lda, int nleft, int c1, int c2,
                                                          const int *__restrict__ comp, const int *__restrict__ err) {
-  constexpr int NL = LDS ? COMP_MAX : 1;
-  __shared__ int sd[NL], ss[NL];
   if (ld_dev(err)) return;  // a timed-out exchange: the list is not trustworthy
   const int cnt = min(comp[0], COMP_MAX);
-  if (LDS) {
-    for (int e = threadIdx.x; e < cnt; e += 256) {
-      sd[e] = comp[1 + 2 * e];
-      ss[e] = comp[2 + 2 * e];
-    }
-    __syncthreads();
-  }
   const int total = nleft + (c2 - c1);
   const int lane = threadIdx.x & 63;
   constexpr int PER = COMP_MAX / 64;
@@ -816,12 +807,12 @@ __global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int p = lane + 64 * i;
-      if (p < cnt) v[i] = cp[LDS ? ss[p] : comp[2 + 2 * p]];
+      if (p < cnt) v[i] = cp[comp[2 + 2 * p]];
     }
 #pragma unroll
     for (int i = 0; i < PER; i++) {
       const int p = lane + 64 * i;
-      if (p < cnt) cp[LDS ? sd[p] : comp[1 + 2 * p]] = v[i];
+      if (p < cnt) cp[comp[1 + 2 * p]] = v[i];
     }
   }
 }
@@ -1506,14 +1497,8 @@ static int panel_swap_launch(hipStream_t s, T *A, long lda, int nleft, int c1, i
   if (blocks > cap) blocks = cap;
   int *err = leaf_error_word();
   if (!err) return PLA_ERR_NO_MEMORY;
-  // the columns left of the panel are permuted beside the trailing update:
-  // without LDS there
-  if (nleft > 0 && c2 <= c1)
-    hipLaunchKernelGGL((panel_swap_kernel<T, false>), dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1, c1, comp,
-                       (const int *)err);
-  else
-    hipLaunchKernelGGL((panel_swap_kernel<T, true>), dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1,
-                       c2 > c1 ? c2 : c1, comp, (const int *)err);
+  hipLaunchKernelGGL(panel_swap_kernel<T>, dim3(blocks), dim3(256), 0, s, A, lda, nleft, c1, c2 > c1 ? c2 : c1,
+                     comp, (const int *)err);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
