@@ -219,6 +219,11 @@ static void timing_reset() {
   }
 }
 
+void timing_release() {
+  timing_reset();
+  g_timing = false;
+}
+
 }  // namespace pla
 
 using namespace pla;

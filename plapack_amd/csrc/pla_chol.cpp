@@ -30,23 +30,31 @@ namespace pla {
 //                       A22[bn:, :bn] -= A21[bn:] A21[:bn]^T -> signal
 //                       -> lower(A22[bn:, bn:]) -= A21[bn:] A21[bn:]^T
 // look-ahead stream:    wait(signal) -> back up + factor A11 -> A21 inv(L11)^T
-int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out) {
+int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out, const chol_scratch *scr) {
   hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
-  if (nb > 512) nb = 512;
+  if (nb > CHOL_MAX_NB) nb = CHOL_MAX_NB;
   if (nb < 1) nb = 1;
   // info: first failure index, written by the panels on sP.  seen: the
   // main stream's snapshot of info taken when panel k is done and panel k+1
   // not yet started, so step k's update runs exactly when the reference's
   // would (every panel <= k succeeded), whatever panel k+1 does meanwhile.
-  int *info = (int *)workspace(2 * sizeof(int), 0);
-  int *seen = info + 1;
-  double *backup = (double *)workspace(sizeof(double) * (size_t)nb * nb, 1);
-  if (!info || !backup) return PLA_ERR_NO_MEMORY;
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
-  if (!ev_ready) {
-    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+  int *info;
+  double *backup;
+  if (scr) {  // caller-provided scratch: no allocation, no kernel timing
+    if ((size_t)nb * nb > scr->backup_elems)
+      return fail(PLA_ERR_DIMENSIONS, "Cholesky scratch holds a %zu-element block backup, nb = %d needs %zu",
+                  scr->backup_elems, nb, (size_t)nb * nb);
+    info = scr->info;
+    backup = scr->backup;
+  } else {
+    info = (int *)workspace(2 * sizeof(int), 0);
+    backup = (double *)workspace(sizeof(double) * (size_t)nb * nb, 1);
+    if (!info || !backup) return PLA_ERR_NO_MEMORY;
   }
+  int *seen = info + 1;
+  hipEvent_t ev_ready = lib_event(EV_CHOL_READY), ev_panel = lib_event(EV_CHOL_PANEL);
+  if (!ev_ready || !ev_panel) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
+  const bool timed = !scr;
   PLA_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int), sM));
   int rc;
   // factor the column block that starts at diagonal position k (on sP)
@@ -73,12 +81,12 @@ int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_ou
     double *A21 = a + (long)k * lda + k + b;
     double *A22 = a + (long)(k + b) * lda + k + b;
     const int bn = r < nb ? r : nb;
-    timing_begin(sM, 2);
+    if (timed) timing_begin(sM, 2);
     rc = k_syrk_lower(sM, 1, bn, b, -1.0, A21, lda, 1.0, A22, lda, seen);
     if (!rc && r > bn)
       rc = k_gemm(sM, 1, false, true, r - bn, bn, b, -1.0, A21 + bn, lda, A21, lda, 1.0, A22 + bn, lda,
                   seen);
-    timing_end(sM, 2, (double)bn * (bn + 1) * b + 2.0 * (r - bn) * (double)bn * b);
+    if (timed) timing_end(sM, 2, (double)bn * (bn + 1) * b + 2.0 * (r - bn) * (double)bn * b);
     if (rc) return rc;
     PLA_HIP(hipEventRecord(ev_ready, sM));
     PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
@@ -86,9 +94,9 @@ int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_ou
     PLA_HIP(hipEventRecord(ev_panel, sP));
     if (r > bn) {
       const int rr = r - bn;
-      timing_begin(sM, 2);
+      if (timed) timing_begin(sM, 2);
       rc = k_syrk_lower(sM, 1, rr, b, -1.0, A21 + bn, lda, 1.0, A22 + (long)bn * lda + bn, lda, seen);
-      timing_end(sM, 2, (double)rr * (rr + 1) * b);
+      if (timed) timing_end(sM, 2, (double)rr * (rr + 1) * b);
       if (rc) return rc;
     }
   }

@@ -22,6 +22,7 @@ namespace pla {
 
 static std::string g_last_error;
 static context g_ctx;
+static hipEvent_t g_events[EV_COUNT];
 context &ctx() { return g_ctx; }
 
 int fail(int code, const char *fmt, ...) {
@@ -53,7 +54,28 @@ int ensure_device() {
   PLA_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   PLA_HIP(hipStreamCreateWithFlags(&c.stream_la, hipStreamNonBlocking));
   PLA_HIP(hipStreamCreateWithFlags(&c.stream_tr, hipStreamNonBlocking));
+  for (auto &e : g_events)
+    if (!e) PLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return PLA_SUCCESS;
+}
+
+hipEvent_t lib_event(int id) { return id >= 0 && id < EV_COUNT ? g_events[id] : nullptr; }
+
+// Destroys every stream and event the library created (PLA_Finalize); the
+// next PLA_Init creates them again.
+static void release_streams(context &c) {
+  hipStream_t *streams[] = {&c.stream, &c.stream_la, &c.stream_tr, &c.stream_panel, &c.stream_trail};
+  for (hipStream_t *sp : streams)
+    if (*sp) (void)hipStreamSynchronize(*sp);
+  for (auto &e : g_events) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  for (hipStream_t *sp : streams) {
+    if (*sp) (void)hipStreamDestroy(*sp);
+    *sp = nullptr;
+  }
+  c.masked_cus = 0;
 }
 
 struct ws_slot {
@@ -119,8 +141,6 @@ static int make_masked_streams(context &c, int k) {
   return PLA_SUCCESS;
 }
 
-static hipEvent_t g_ev_join[3];
-
 int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, int *gcap) {
   context &c = g_ctx;
   *gcap = 0;
@@ -128,8 +148,7 @@ int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, 
     *sP = *sT = main;
     return PLA_SUCCESS;
   }
-  if (!g_ev_join[0])
-    for (auto &e : g_ev_join) PLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!g_events[EV_JOIN0]) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   int k = c.panel_cus;
   if (k > 0 && c.cu_count % 32 == 0 && k < c.cu_count) {
     // enough CUs that the first (tallest) panel's leaf rows fit in LDS
@@ -143,20 +162,20 @@ int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, 
     *sP = c.stream_la;
     *sT = c.stream_tr;
   }
-  PLA_HIP(hipEventRecord(g_ev_join[0], main));
-  PLA_HIP(hipStreamWaitEvent(*sP, g_ev_join[0], 0));
-  if (*sT != main) PLA_HIP(hipStreamWaitEvent(*sT, g_ev_join[0], 0));
+  PLA_HIP(hipEventRecord(g_events[EV_JOIN0], main));
+  PLA_HIP(hipStreamWaitEvent(*sP, g_events[EV_JOIN0], 0));
+  if (*sT != main) PLA_HIP(hipStreamWaitEvent(*sT, g_events[EV_JOIN0], 0));
   return PLA_SUCCESS;
 }
 
 int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT) {
   if (sP != main) {
-    PLA_HIP(hipEventRecord(g_ev_join[1], sP));
-    PLA_HIP(hipStreamWaitEvent(main, g_ev_join[1], 0));
+    PLA_HIP(hipEventRecord(g_events[EV_JOIN1], sP));
+    PLA_HIP(hipStreamWaitEvent(main, g_events[EV_JOIN1], 0));
   }
   if (sT != main) {
-    PLA_HIP(hipEventRecord(g_ev_join[2], sT));
-    PLA_HIP(hipStreamWaitEvent(main, g_ev_join[2], 0));
+    PLA_HIP(hipEventRecord(g_events[EV_JOIN2], sT));
+    PLA_HIP(hipStreamWaitEvent(main, g_events[EV_JOIN2], 0));
   }
   return PLA_SUCCESS;
 }
@@ -315,12 +334,9 @@ int PLA_Init(MPI_Comm comm) {
 int PLA_Finalize(void) {
   if (!g_initialized) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   pla::context &c = pla::ctx();
-  if (c.stream) (void)hipStreamSynchronize(c.stream);
-  if (c.stream_la) (void)hipStreamSynchronize(c.stream_la);
-  if (c.stream_tr) (void)hipStreamSynchronize(c.stream_tr);
-  if (c.stream_panel) (void)hipStreamSynchronize(c.stream_panel);
-  if (c.stream_trail) (void)hipStreamSynchronize(c.stream_trail);
   pla::api_release();
+  pla::timing_release();
+  pla::release_streams(c);
   pla::release_workspaces();
   g_initialized = 0;
   c.initialized = false;

@@ -84,6 +84,13 @@ struct context {
 };
 context &ctx();
 int ensure_device();                 // lazily initialise device + streams
+// Events of the library's stream orderings, created with the streams at
+// PLA_Init and destroyed at PLA_Finalize (nullptr outside that window).
+enum {
+  EV_LU_READY, EV_LU_PANEL, EV_MLU_READY, EV_MLU_PANEL, EV_ZLU_READY, EV_ZLU_PANEL, EV_CHOL_READY,
+  EV_CHOL_PANEL, EV_JOIN0, EV_JOIN1, EV_JOIN2, EV_COUNT
+};
+hipEvent_t lib_event(int id);
 
 // Streams of a blocked LU with look-ahead.  *sP receives the panel
 // factorizations, *sT the interchanges right of the panel and the trailing
@@ -287,6 +294,22 @@ int lu_panel(hipStream_t s, double *P, int mp, int w, long lda, int *piv, int *p
 // offset added).
 int potrf_block(hipStream_t s, int b, double *A, long lda, int offset, int *info);
 
+// ---- Cholesky (pla_chol.cpp) ----
+constexpr int CHOL_MAX_NB = 512;  // widest diagonal block of chol_factor
+// Scratch of one factorization: info[0] first failure index, info[1] the
+// trailing stream's snapshot of it; backup: the diagonal block being factored.
+struct chol_scratch {
+  int *info;
+  double *backup;
+  size_t backup_elems;
+};
+// Blocked lower Cholesky with look-ahead; *info_out = 0 or the 1-based index
+// of the first non-positive pivot.  scr == nullptr: scratch from the grow-only
+// workspaces, trailing updates timed when kernel timing is on; otherwise the
+// caller's scratch, no allocation and no timing.
+int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out,
+                const chol_scratch *scr = nullptr);
+
 // ---- kernel timing (hipEvents on the library stream, per role) ----
 // role 1: trailing update of PLA_LU (GEMM); role 2: trailing update of PLA_Chol (SYRK)
 bool timing_on();
@@ -333,6 +356,7 @@ int refine(hipStream_t s, const refine_ops &op, double anorm, const double *B, l
 void set_last_refine(const refine_stats &st, double shift);
 
 void timing_begin(hipStream_t s, int role);
+void timing_release();  // destroys pending timing events, timing off (PLA_Finalize)
 void timing_end(hipStream_t s, int role, double flops);
 
 }  // namespace pla

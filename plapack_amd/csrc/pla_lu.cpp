@@ -70,11 +70,8 @@ int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, in
   int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(m, m < n ? m : n, nb) * cstride, 4);
   if (!pairs || !comp) return PLA_ERR_NO_MEMORY;
   const lu_bufs bf{pairs, pstride, comp, cstride};
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
-  if (!ev_ready) {
-    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-  }
+  hipEvent_t ev_ready = lib_event(EV_LU_READY), ev_panel = lib_event(EV_LU_PANEL);
+  if (!ev_ready || !ev_panel) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   int rc;
   // sP: panels; sM: interchanges right of the panel, A12 solves and trailing
   // updates; the main stream: interchanges left of the panel

@@ -47,11 +47,8 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   // of a step may still be reading theirs while later panels are composed)
   int *comp = (int *)workspace(sizeof(int) * (size_t)lu_step_count(n, n, nb) * cstride, WS_COMP);
   if (!P64 || !U64 || !Lh || !Uh || !pairs || !comp || (split && (!Ll || !Ul))) return PLA_ERR_NO_MEMORY;
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
-  if (!ev_ready) {
-    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-  }
+  hipEvent_t ev_ready = lib_event(EV_MLU_READY), ev_panel = lib_event(EV_MLU_PANEL);
+  if (!ev_ready || !ev_panel) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   int nleaves[2] = {0, 0};
   int rc;
   // sP: panels; sM: interchanges right of the panel, U12 solves and trailing

@@ -138,11 +138,8 @@ int zlu_factor(hipStream_t sMain, int m, int n, void *av, long lda, int *piv, in
   zc *a = (zc *)av;
   const int mn = m < n ? m : n;
   if (nb < 1) nb = 1;
-  static hipEvent_t ev_ready = nullptr, ev_panel = nullptr;
-  if (!ev_ready) {
-    PLA_HIP(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
-    PLA_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-  }
+  hipEvent_t ev_ready = lib_event(EV_ZLU_READY), ev_panel = lib_event(EV_ZLU_PANEL);
+  if (!ev_ready || !ev_panel) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   hipStream_t sP, sM;
   int gcap, rc;
   if ((rc = lu_streams_begin(sMain, m, &sP, &sM, &gcap))) return rc;
