@@ -401,6 +401,19 @@ int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, i
    applied, GMRES steps in total, corrections computed by GMRES, and the
    diagonal shift of the factored matrix (0 for LU).  Any pointer may be NULL. */
 int PLA_Mixed_stats(int *outer, int *gmres_steps, int *gmres_corrections, double *shift);
+/* No counterpart in the reference.  PLA_Chol (SOLVERS/Cholesky/PLA_Chol.c:13)
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc): the same
+   kernels and results as PLA_Chol on MPI_DOUBLE, with the factorization's
+   scratch (two info words and the backup of the diagonal block being
+   factored) in one static device array of 2 MiB + 256 B per device instead
+   of library-allocated workspaces.  The block size is PLA_Chol's, capped at
+   512 as in PLA_Chol, whose 512 x 512 backup is the array's limit (a block
+   the array cannot hold is rejected with PLA_ERR_DIMENSIONS).  Returns 0 or
+   the 1-based index of the first non-positive pivot, as PLA_Chol.  The info
+   words are cleared at the start of every call; calls that overlap on one
+   device share that array and race. */
+int PLA_Chol_no_alloc(int uplo, PLA_Obj A);
 /* Message of the last error returned by any entry point. */
 const char *PLA_Last_error(void);
 /* Device pointer of the object's top-left element (HBM).  Drops the host

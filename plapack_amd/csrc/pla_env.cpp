@@ -56,7 +56,9 @@ int ensure_device() {
   PLA_HIP(hipStreamCreateWithFlags(&c.stream_tr, hipStreamNonBlocking));
   for (auto &e : g_events)
     if (!e) PLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  return PLA_SUCCESS;
+  // the no-allocation Cholesky's static scratch: its address on this device
+  void *scratch_ptr = nullptr;
+  return chol_static_scratch(c.device, &scratch_ptr);
 }
 
 hipEvent_t lib_event(int id) { return id >= 0 && id < EV_COUNT ? g_events[id] : nullptr; }

@@ -309,6 +309,9 @@ struct chol_scratch {
 // caller's scratch, no allocation and no timing.
 int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out,
                 const chol_scratch *scr = nullptr);
+// Device address of the no-allocation Cholesky's static scratch on `device`
+// (looked up once per device; PLA_Init calls it for the library's device).
+int chol_static_scratch(int device, void **ptr);
 
 // ---- kernel timing (hipEvents on the library stream, per role) ----
 // role 1: trailing update of PLA_LU (GEMM); role 2: trailing update of PLA_Chol (SYRK)

@@ -134,6 +134,7 @@ _SIGS = {
     "PLA_LU_linpack": [_obj, _obj],
     "PLA_Chol_simple": [_i, _obj],
     "PLA_Chol": [_i, _obj],
+    "PLA_Chol_no_alloc": [_i, _obj],
     "PLA_Apply_pivots_to_rows": [_obj, _obj],
     "PLA_Trsv": [_i, _i, _i, _obj, _obj],
     "PLA_Trsm": [_i, _i, _i, _i, _obj, _obj, _obj],
