@@ -28,10 +28,53 @@
 #endif
 
 #include <stdint.h>
+/* the standard headers the reference's INCLUDE/PLA.h brings in */
+#include <math.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#ifndef NULL
+#define NULL 0
+#endif
+
+/* Machine identifiers (INCLUDE/PLA_Machines.h); drivers test
+   MANUFACTURE == CRAY, so both must be defined.  The reference's build passes
+   MANUFACTURE and MACHINE_TYPE on the command line; a PC running Linux is the
+   default here. */
+#undef CRAY
+#define CRAY  10
+#define IBM   20
+#define INTEL 30
+#define SUN   40
+#define PC    50
+#define SGI   60
+#define HP    70
+#define CRAYPVP 100
+#define CRAYMPP 101
+#define PARAGON 200
+#define SP2     300
+#define SUNOS   400
+#define LINUX   500
+#define NT      600
+#ifndef MANUFACTURE
+#define MANUFACTURE PC
+#endif
+#ifndef MACHINE_TYPE
+#define MACHINE_TYPE LINUX
+#endif
 
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Element types of complex objects (INCLUDE/PLA_Misc.h:11-17). */
+typedef struct PLA_COMPLEX {
+  float real, imaginary;
+} PLA_COMPLEX;
+typedef struct PLA_DOUBLE_COMPLEX {
+  double real, imaginary;
+} PLA_DOUBLE_COMPLEX;
 
 /* ---- constants: values identical to INCLUDE/PLA_define.h:11-102 ---- */
 #define PLA_OBJTYPE_OFFSET 10
@@ -73,6 +116,15 @@ extern "C" {
 #define PLA_SHAPE_UPPER_TRAPEZOIDAL (PLA_SHAPE_OFFSET + 5)
 #define PLA_SHAPE_UP_TRAP           (PLA_SHAPE_OFFSET + 5)
 
+#define PLA_FLOW_OFFSET 60
+#define PLA_DIR_NOFLOW (PLA_FLOW_OFFSET + 1)
+#define PLA_DIR_UP     (PLA_FLOW_OFFSET + 2)
+#define PLA_DIR_DOWN   (PLA_FLOW_OFFSET + 3)
+#define PLA_DIR_LEFT   (PLA_FLOW_OFFSET + 4)
+#define PLA_DIR_RIGHT  (PLA_FLOW_OFFSET + 5)
+#define PLA_TEMP_COL   (PLA_FLOW_OFFSET + 6)
+#define PLA_TEMP_ROW   (PLA_FLOW_OFFSET + 7)
+
 #define PLA_OP_OFFSET 80
 #define PLA_OP_MAT_PAN       (PLA_OP_OFFSET + 1)
 #define PLA_OP_PAN_MAT       (PLA_OP_OFFSET + 2)
@@ -113,6 +165,18 @@ extern "C" {
 #ifndef min
 #define min(x, y) ((x) < (y) ? (x) : (y))
 #endif
+#ifndef dabs
+#define dabs(x) ((x) < 0 ? -(x) : (x))
+#endif
+
+/* INCLUDE/PLA_Versions.h:11-13 (PLA_Gemm_x versions), INCLUDE/PLA_Misc.h:43-46 */
+#define PLA_VERSION_A 100
+#define PLA_VERSION_B 200
+#define PLA_VERSION_C 300
+#define PLA_METHOD_INV     0
+#define PLA_METHOD_FACTORS 1
+#define PLA_METHOD_STABLE  2
+#define PLA_METHOD_TRSM    3
 
 /* Negative error codes (the reference aborts instead). */
 #define PLA_ERR_NOT_INITIALIZED -1001
@@ -123,6 +187,7 @@ extern "C" {
 #define PLA_ERR_DEVICE          -1006
 #define PLA_ERR_NO_MEMORY       -1007
 #define PLA_ERR_MULTI_PROCESS   -1008
+#define PLA_ERR_NOT_CONVERGED   -1009  /* refinement ended above its tolerance */
 
 /* ---- opaque handles (INCLUDE/PLA_Obj.h:82, INCLUDE/PLA_Template.h:31) ---- */
 typedef struct PLA_Obj_view_struct *PLA_Obj;
@@ -138,6 +203,10 @@ extern int PLA_CHECK_AGAINST_R12;
 /* TEMPLATE/PLA_Init.c:20.  Only a 1x1 mesh is supported: rows*cols != 1
    returns PLA_ERR_MULTI_PROCESS.  comm2d receives comm1d. */
 int PLA_Comm_1D_to_2D(MPI_Comm comm1d, int rows, int cols, MPI_Comm *comm2d);
+/* TEMPLATE/PLA_Init.c:38: the mesh whose rows / cols is closest to ratio.
+   On one rank that is 1x1 whatever the ratio: comm2d receives comm1d; a
+   communicator of more ranks returns PLA_ERR_MULTI_PROCESS. */
+int PLA_Comm_1D_to_2D_ratio(MPI_Comm comm1d, double ratio, MPI_Comm *comm2d);
 /* TEMPLATE/PLA_Init.c:85.  Also selects and initialises the HIP device. */
 int PLA_Init(MPI_Comm comm);
 /* TEMPLATE/PLA_Init.c:132 */
@@ -163,6 +232,19 @@ int PLA_Environ_nb_alg(int type, PLA_Template templ, int *nb_alg);
 /* OBJ/PLA_Diagnostics.c:402, :436 */
 int PLA_Set_error_checking(int error, int parameters, int sequential, int r12);
 int PLA_Get_error_checking(int *error, int *parameters, int *sequential, int *r12);
+/* OBJ/PLA_Diagnostics.c:108.  Called by a program to give up: prints the
+   message, file and line to stderr and ends the process with status 1 --
+   through MPI_Abort when the program has initialised MPI, else exit(1) (the
+   reference exits with status 0).  The library itself never calls it. */
+int PLA_Abort(char *message, int line, char *file);
+
+/* OBJ/PLA_Memory.c:18, :51, :68 -- host memory of the reference's drivers:
+   malloc / calloc(size * typesize) / free.  PLA_malloc and PLA_calloc return
+   NULL when the host is out of memory (the reference exits); PLA_free returns
+   NULL. */
+void *PLA_malloc(size_t size);
+void *PLA_calloc(size_t size, size_t typesize);
+void *PLA_free(void *buffer);
 
 /* ---- objects (OBJ/PLA_Obj.c) ---- */
 /* :323 */
@@ -305,6 +387,28 @@ int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_
 /* BLAS3/PLA_Gemm.c:13 */
 int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta,
              PLA_Obj C);
+/* BLAS3/PLA_Gemm_x.c:13: the reference's distributed algorithm variants
+   (PLA_VERSION_A / _B / _C) of PLA_Gemm; on one GPU every version is
+   PLA_Gemm.  An unknown version returns PLA_ERR_INVALID_PARAM. */
+int PLA_Gemm_x(int version, int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta,
+               PLA_Obj C);
+/* BLAS1/PLA_Axpy.c:13: y <- alpha x + y for x, y of one shape and datatype
+   (MPI_DOUBLE, MPI_FLOAT or MPI_DOUBLE_COMPLEX) and alpha a 1x1 object of
+   that datatype */
+int PLA_Axpy(PLA_Obj alpha, PLA_Obj x, PLA_Obj y);
+/* BLAS1/PLA_Nrm2.c:13: alpha <- ||x||_2 of a vector x (one column, or one
+   row) of MPI_DOUBLE, MPI_FLOAT or MPI_DOUBLE_COMPLEX; alpha is a 1x1 object
+   of the real datatype (MPI_DOUBLE, or MPI_FLOAT for MPI_FLOAT x) */
+int PLA_Nrm2(PLA_Obj x, PLA_Obj alpha);
+/* BLAS2/PLA_Symv.c:13, BLAS2/PLA_Hemv.c:13: y <- alpha A x + beta y with A
+   symmetric (MPI_DOUBLE, MPI_FLOAT) or Hermitian (MPI_DOUBLE_COMPLEX; the
+   imaginary part of the diagonal is taken as zero), only the uplo triangle of
+   A read; x and y vectors (one column) */
+int PLA_Symv(int uplo, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y);
+int PLA_Hemv(int uplo, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y);
+/* UTIL/PLA_Shift.c:15: add *value (MPI_DOUBLE or MPI_FLOAT, converted to
+   obj's datatype) to the diagonal of obj (MPI_DOUBLE or MPI_FLOAT) */
+int PLA_Shift(PLA_Obj obj, MPI_Datatype datatype, void *value);
 /* BLAS3/PLA_Syrk.c:13: C <- beta C on all of C (PLA_Syrk_panpan.c:22), then
    the uplo triangle += alpha op(A) op(A)^T */
 int PLA_Syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Obj C);
@@ -331,7 +435,9 @@ int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B);
      max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
    is <= tol or max_iter refinement steps were taken.  *iterations and
    *backward_error (either may be NULL) report the outcome.  Returns
-   PLA_SUCCESS also when tol was not reached: check *backward_error. */
+   PLA_ERR_NOT_CONVERGED when tol was not reached (or the backward error is
+   not a number); X, *iterations and *backward_error then hold the last
+   iterate and its figures. */
 #define PLA_MIXED_FP16   1
 #define PLA_MIXED_BF16   2
 #define PLA_MIXED_FP16X3 3
@@ -393,7 +499,8 @@ int PLA_Obj_attach_buffer(PLA_Obj obj, void *buffer, int local_ldim, int user_bu
    X (n x nrhs, nrhs <= 16) is then refined in FP64 against A: classic
    refinement, switching to GMRES-IR preconditioned by the factors when a
    step fails to halve the normwise backward error, until it is <= tol or
-   max_iter corrections were applied.  PLA_LU_mixed refines the same way. */
+   max_iter corrections were applied.  PLA_LU_mixed refines the same way,
+   and returns PLA_ERR_NOT_CONVERGED the same way. */
 #define PLA_MIXED_FP8    5
 int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, int max_iter,
                    int *iterations, double *backward_error);

@@ -336,6 +336,7 @@ int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x) {
   if ((rc = trsv_sf(ctx().stream, uplo == PLA_LOWER_TRIANGULAR, trans != PLA_NO_TRANSPOSE, diag == PLA_UNIT_DIAG,
                     n, 1, a, ld(A), b, ld(x) > n ? ld(x) : n, nullptr)))
     return rc;
+  if ((rc = leaf_check_error(ctx().stream, "PLA_Trsv"))) return rc;
   return written(x);
 }
 
@@ -359,6 +360,15 @@ int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Ob
   if ((rc = k_gemm(ctx().stream, 0, ta, tb, m, n, ak, al, a, ld(A), b, ld(B), be, c, ld(C), nullptr)))
     return rc;
   return written(C);
+}
+
+// BLAS3/PLA_Gemm_x.c:13: the versions select how the reference distributes
+// the product over a mesh (BLAS3/PLA_Gemm_x.c dispatches on them); on one GPU
+// they all compute PLA_Gemm's product.
+int PLA_Gemm_x(int version, int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta, PLA_Obj C) {
+  if (version != PLA_VERSION_A && version != PLA_VERSION_B && version != PLA_VERSION_C)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Gemm_x: unknown version %d", version);
+  return PLA_Gemm(transa, transb, alpha, A, B, beta, C);
 }
 
 // C <- beta C, then the uplo triangle of C += alpha op(A) op(A)^T

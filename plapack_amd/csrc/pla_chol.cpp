@@ -189,6 +189,7 @@ int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B) {
   if (B->width <= trsv_max_rhs()) {
     if ((rc = trsv_sf(s, true, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
     if ((rc = trsv_sf(s, true, true, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = leaf_check_error(s, "PLA_Pos_def_solve: triangular solve"))) return rc;
   } else {
     if ((rc = trsm_left(s, true, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
     if ((rc = trsm_left(s, true, true, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;

@@ -229,7 +229,7 @@ int *leaf_error_word() {
   return w.err;
 }
 
-int leaf_check_error(hipStream_t s) {
+int leaf_check_error(hipStream_t s, const char *what) {
   if (!g_leaf.base) return PLA_SUCCESS;
   int h = 0;
   PLA_HIP(hipMemcpyAsync(&h, g_leaf.err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -237,7 +237,7 @@ int leaf_check_error(hipStream_t s) {
   if (h) {
     PLA_HIP(hipMemsetAsync(g_leaf.err, 0, sizeof(int), s));
     PLA_HIP(hipStreamSynchronize(s));
-    return fail(PLA_ERR_DEVICE, "panel factorization: a cross-workgroup exchange timed out");
+    return fail(PLA_ERR_DEVICE, "%s: a cross-workgroup exchange timed out", what);
   }
   return PLA_SUCCESS;
 }
@@ -317,6 +317,14 @@ int PLA_Comm_1D_to_2D(MPI_Comm comm1d, int rows, int cols, MPI_Comm *comm2d) {
     return fail(PLA_ERR_MULTI_PROCESS, "PLA_Comm_1D_to_2D: communicator has %d ranks", size);
   *comm2d = comm1d;
   return PLA_SUCCESS;
+}
+
+// TEMPLATE/PLA_Init.c:38-83 picks the divisor of the rank count closest to
+// sqrt(ratio * ranks) as the row count; with one rank that is 1 x 1.
+int PLA_Comm_1D_to_2D_ratio(MPI_Comm comm1d, double ratio, MPI_Comm *comm2d) {
+  (void)ratio;
+  if (!comm2d) return fail(PLA_ERR_INVALID_PARAM, "PLA_Comm_1D_to_2D_ratio: comm2d is NULL");
+  return PLA_Comm_1D_to_2D(comm1d, 1, 1, comm2d);
 }
 
 int PLA_Init(MPI_Comm comm) {

@@ -77,6 +77,7 @@ struct context {
   int nb_tail = 0;          // LU block size once at most nb_switch rows remain (0: none)
   int nb_switch = 0;
   unsigned leaf_spin_limit = 400000u;  // bounded wait of a panel-leaf exchange (polls)
+  unsigned trsv_spin_limit = 2000000u; // bounded wait of a triangular-solve block (polls)
   // CU-masked stream pair of the LU look-ahead: the panel stream runs on
   // masked_cus CUs (masked_cus / 8 per XCD), the trailing stream on the rest
   hipStream_t stream_panel = nullptr, stream_trail = nullptr;
@@ -107,17 +108,18 @@ int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
-constexpr int PLA_NUM_WS = 49;
+constexpr int PLA_NUM_WS = 51;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
 // 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-48 complex
 // (GEMM operand embedding, conjugate transposes, Cholesky info word and block backup,
-// the look-ahead panel's operand embedding)
+// the look-ahead panel's operand embedding), 49-50 level-1/2 helpers (pla_util.hip)
 enum {
   WS_SRES_PART = 24, WS_SRES_MAX, WS_ONES, WS_NORM_R, WS_MDOT,
   WS_GM_V = 29, WS_GM_W, WS_GM_H, WS_GM_T, WS_GM_Z,
   WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R,
-  WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO, WS_ZEMBED_P
+  WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO, WS_ZEMBED_P,
+  WS_UTIL = 49, WS_UTIL_NORM
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
@@ -156,8 +158,9 @@ struct leaf_ws {
 leaf_ws &leaf_workspace(size_t slot_bytes);
 // First of `count` fresh exchange tags for a leaf launch on stream s.
 int leaf_tags(hipStream_t s, unsigned count, unsigned *tag0);
-// Returns PLA_ERR_DEVICE if any panel exchange timed out since the last call.
-int leaf_check_error(hipStream_t s);
+// Waits for stream s; returns PLA_ERR_DEVICE (message naming `what`) and
+// clears the word if any bounded wait gave up since the last check.
+int leaf_check_error(hipStream_t s, const char *what = "panel factorization");
 // The device error word shared by the bounded waits of every kernel.
 int *leaf_error_word();
 
@@ -357,6 +360,9 @@ struct refine_stats {
 int refine(hipStream_t s, const refine_ops &op, double anorm, const double *B, long ldb, double *X, long ldx,
            double tol, int max_iter, bool gmres_fallback, refine_stats *st);
 void set_last_refine(const refine_stats &st, double shift);
+// PLA_SUCCESS when st.eta <= tol, else PLA_ERR_NOT_CONVERGED with a message
+// (a NaN backward error also fails the comparison)
+int refine_outcome(const char *who, const refine_stats &st, double tol);
 
 void timing_begin(hipStream_t s, int role);
 void timing_release();  // destroys pending timing events, timing off (PLA_Finalize)

@@ -230,6 +230,7 @@ int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
   if (B->width <= trsv_max_rhs()) {
     if ((rc = trsv_sf(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
     if ((rc = trsv_sf(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
+    if ((rc = leaf_check_error(s, "PLA_General_solve: triangular solve"))) return rc;
   } else {
     if ((rc = trsm_left(s, true, false, true, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;
     if ((rc = trsm_left(s, false, false, false, n, B->width, a, ld(A), b, ld(B), nullptr))) return rc;

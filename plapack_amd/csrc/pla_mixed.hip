@@ -771,10 +771,10 @@ static int trsv_launch(hipStream_t s, bool lower, bool trans, int n, int nrhs, c
   const int nblk = (n + TB - 1) / TB;
 #define PLA_TSF(LO, TR)                                                                                      \
   hipLaunchKernelGGL((trsv_sf_kernel<T, LO, TR, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, \
-                     ldx, xs, tag, 2000000u, err)
+                     ldx, xs, tag, ctx().trsv_spin_limit, err)
 #define PLA_TNT(LO)                                                                                          \
   hipLaunchKernelGGL((trsv_nt_kernel<T, LO, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, ldx, \
-                     xs, tag, 2000000u, err)
+                     xs, tag, ctx().trsv_spin_limit, err)
   if (lower && !trans) PLA_TNT(true);
   else if (lower && trans) PLA_TSF(true, true);
   else if (!lower && !trans) PLA_TNT(false);

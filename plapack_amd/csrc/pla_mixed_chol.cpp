@@ -133,7 +133,8 @@ int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, i
   set_last_refine(st, shift);
   if (iterations) *iterations = st.outer;
   if (backward_error) *backward_error = st.eta;
-  return written(X);
+  if ((rc = written(X))) return rc;
+  return refine_outcome("PLA_Chol_mixed", st, tol);
 }
 
 int PLA_Mixed_stats(int *outer, int *gmres_steps, int *gmres_corrections, double *shift) {

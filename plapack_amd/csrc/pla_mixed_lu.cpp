@@ -222,8 +222,8 @@ int PLA_LU_mixed(PLA_Obj A, PLA_Obj pivots, PLA_Obj B, PLA_Obj X, int precision,
   set_last_refine(st, 0.0);
   if (iterations) *iterations = st.outer;
   if (backward_error) *backward_error = st.eta;
-  if ((rc = written(X))) return rc;
-  return written(pivots);
+  if ((rc = written(X)) || (rc = written(pivots))) return rc;
+  return refine_outcome("PLA_LU_mixed", st, tol);
 }
 
 }  // extern "C"

@@ -197,4 +197,12 @@ int refine(hipStream_t s, const refine_ops &op, double anorm, const double *B, l
   return PLA_SUCCESS;
 }
 
+int refine_outcome(const char *who, const refine_stats &st, double tol) {
+  if (st.eta <= tol) return PLA_SUCCESS;
+  return fail(PLA_ERR_NOT_CONVERGED,
+              "%s: refinement stopped at backward error %.3e above the tolerance %.3e after %d corrections (%d GMRES "
+              "steps); X holds the last iterate",
+              who, st.eta, tol, st.outer, st.inner);
+}
+
 }  // namespace pla

@@ -38,4 +38,10 @@ int pla_th_set_leaf_spin_limit(unsigned limit) {
   return PLA_SUCCESS;
 }
 
+// Poll limit of the triangular-solve block waits (0 restores the default).
+int pla_th_set_trsv_spin_limit(unsigned limit) {
+  ctx().trsv_spin_limit = limit ? limit : 2000000u;
+  return PLA_SUCCESS;
+}
+
 }  // extern "C"

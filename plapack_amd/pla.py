@@ -52,6 +52,7 @@ PLA_ERR_DIMENSIONS = -1005
 PLA_ERR_DEVICE = -1006
 PLA_ERR_NO_MEMORY = -1007
 PLA_ERR_MULTI_PROCESS = -1008
+PLA_ERR_NOT_CONVERGED = -1009
 PLA_MODE_CLOSED = 71
 PLA_MODE_OPEN = 72
 PLA_STATE_API_ACTIVE = 73
@@ -86,6 +87,14 @@ _d = ctypes.c_double
 # name -> argtypes (all return int unless listed in _RESTYPES)
 _SIGS = {
     "PLA_Comm_1D_to_2D": [_i, _i, _i, _pi],
+    "PLA_Comm_1D_to_2D_ratio": [_i, _d, _pi],
+    "PLA_Abort": [ctypes.c_char_p, _i, ctypes.c_char_p],
+    "PLA_Gemm_x": [_i, _i, _i, _obj, _obj, _obj, _obj, _obj],
+    "PLA_Axpy": [_obj, _obj, _obj],
+    "PLA_Nrm2": [_obj, _obj],
+    "PLA_Symv": [_i, _obj, _obj, _obj, _obj, _obj],
+    "PLA_Hemv": [_i, _obj, _obj, _obj, _obj, _obj],
+    "PLA_Shift": [_obj, _i, _p],
     "PLA_Init": [_i],
     "PLA_Finalize": [],
     "PLA_Initialized": [_pi],

@@ -307,6 +307,51 @@ def test_lu_exchange_timeout_reports_error(pla_session, ora, kind):
     assert np.array_equal(piv, piv_o)
 
 
+@pytest.mark.parametrize("entry", ["PLA_Trsv", "PLA_General_solve", "PLA_Pos_def_solve"])
+def test_triangular_solve_timeout_reports_error(pla_session, ora, entry):
+    """A triangular-solve block wait that gives up (forced from the host with
+    a one-poll limit) makes the entry point return PLA_ERR_DEVICE instead of a
+    silently wrong solution; the error word is cleared, so the next solve and
+    the next LU are correct."""
+    hooks = ctypes.CDLL(os.path.join(os.path.dirname(pla.LIB_PATH), "libpla_testhooks.so"))
+    hooks.pla_th_set_trsv_spin_limit.argtypes = [ctypes.c_uint]
+    s = pla_session
+    n = 2000
+    a = ora.uniform(n, n, 6)
+    spd = a @ a.T / n + np.eye(n)
+    b = np.random.default_rng(6).uniform(-1, 1, (n, 1))
+
+    def run():
+        if entry == "PLA_Trsv":
+            A, X = pla.from_numpy(s, np.tril(a) + n * np.eye(n)), pla.from_numpy(s, b)
+            args = (pla.PLA_LOWER_TRIANGULAR, pla.PLA_NO_TRANSPOSE, pla.PLA_NONUNIT_DIAG, A, X)
+            objs, ref = (A, X), np.linalg.solve(np.tril(a) + n * np.eye(n), b)
+        elif entry == "PLA_General_solve":
+            A, X, P = pla.from_numpy(s, a), pla.from_numpy(s, b), s.mvector(n, dtype=pla.MPI_INT)
+            args, objs, ref = (A, P, X), (A, X, P), np.linalg.solve(a, b)
+        else:
+            A, X = pla.from_numpy(s, spd), pla.from_numpy(s, b)
+            args, objs, ref = (A, X), (A, X), np.linalg.solve(spd, b)
+        try:
+            pla.call(entry, *args)
+            return pla.get_contents(X), ref
+        finally:
+            pla.free(*objs)
+
+    hooks.pla_th_set_trsv_spin_limit(1)
+    try:
+        with pytest.raises(pla.PLAError) as e:
+            run()
+        assert e.value.code == pla.PLA_ERR_DEVICE and "timed out" in str(e.value)
+    finally:
+        hooks.pla_th_set_trsv_spin_limit(0)
+    x, ref = run()
+    assert np.abs(x - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
+    lu, piv = _lu_gpu(s, n=1000, seed=3, nb=128)
+    _, piv_o = ora.lu_blocked(ora.uniform(1000, 1000, 3), nb=128)
+    assert np.array_equal(piv, piv_o)
+
+
 @pytest.mark.parametrize("m,n,nb,tail,switch", [(3000, 3000, 512, 128, 1500), (2500, 1100, 1024, 256, 2000),
                                                 (2000, 2000, 256, 64, 100000)])
 def test_lu_block_size_schedule(pla_session, ora, m, n, nb, tail, switch):
