@@ -573,9 +573,16 @@ int PLA_Get_device_nb_for(int n, int factorization, int *nb);
    stream (the reference keeps per-operation timers in PLA_TIMINGS,
    OBJ/PLA_Timings.c:17-125).  role 1: PLA_LU's A22 -= A21 A12 GEMM;
    role 2: PLA_Chol's lower(A22) -= A21 A21^T SYRK; role 3: PLA_LU_mixed's
-   low-precision trailing GEMM; role 4: PLA_Chol_mixed's FP8 SYRK.  PLA_Kernel_timing(on)
-   clears the counters; the query waits for the recorded events and returns
-   the launch count, summed kernel milliseconds and summed algorithmic flops. */
+   low-precision trailing GEMM; role 4: PLA_Chol_mixed's FP8 SYRK.  Roles 5-7
+   are HBM-bound and count algorithmic bytes instead of flops: role 5 the
+   FP64 residual r = b - A x of the refinements (general or symmetric);
+   role 6 the sync-free triangular solves (PLA_Trsv, the solves of
+   PLA_General_solve / PLA_Pos_def_solve and of the refinements); role 7
+   the row interchanges of the LU factorizations, whose unit is the number of
+   columns swapped (the rows moved per step are known only on the device).
+   PLA_Kernel_timing(on) clears the counters; the query waits for the
+   recorded events and returns the launch count, summed kernel milliseconds
+   and the summed work. */
 int PLA_Kernel_timing(int on);
 int PLA_Kernel_timing_query(int role, long *launches, double *total_ms, double *total_flops);
 

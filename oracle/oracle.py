@@ -115,6 +115,115 @@ def lu_blocked(a, nb=128):
     return a, piv
 
 
+_hblas = None
+
+
+def host_blas():
+    """The host BLAS/LAPACK the CPU baseline calls: the OpenBLAS that scipy
+    ships (its Fortran entry points carry a scipy_ prefix), loaded with
+    ctypes.  Returns (library, threads)."""
+    global _hblas
+    if _hblas is None:
+        import glob
+
+        import scipy
+        libs = glob.glob(os.path.join(os.path.dirname(scipy.__file__), "..", "scipy.libs", "libscipy_openblas*.so"))
+        if not libs:
+            raise OSError("scipy's OpenBLAS library was not found")
+        L = ctypes.CDLL(libs[0])
+        L.scipy_openblas_get_num_threads.restype = ctypes.c_int
+        _hblas = L
+    return _hblas, _hblas.scipy_openblas_get_num_threads()
+
+
+def _ri(v):
+    return ctypes.byref(ctypes.c_int(v))
+
+
+def _rd(v):
+    return ctypes.byref(ctypes.c_double(v))
+
+
+def _rc(c):
+    return ctypes.c_char_p(c.encode())
+
+
+def _at(a, i, j):
+    """Address of a[i, j] of a column-major array."""
+    return ctypes.c_void_p(a.ctypes.data + 8 * (i + j * a.shape[0]))
+
+
+def lu_blocked_blas(a, nb=128):
+    """Blocked right-looking LU with partial pivoting structured as
+    PLA_LU_right (SOLVERS/LU/PLA_LU.c:37-212) on a 1x1 template, with the
+    flops in a host BLAS/LAPACK as the reference puts them in its Fortran
+    BLAS (INCLUDE/PLA_BLAS.h:32-36).  Per step of width b at column k:
+      dgetrf on the m-k x b panel       (PLA_LU_right_mv, :144; LAPACK's
+                                         partial pivoting picks the first
+                                         maximum |a|, as idamax does)
+      dlaswp left and right of it       (PLA_Apply_pivots_to_rows on all
+                                         columns, :152)
+      dtrsm  A12 <- inv(L11) A12        (:174-182)
+      dgemm  A22 <- A22 - A21 A12       (:185-186)
+    all in place on the column-major array (leading dimension m), in the
+    host BLAS's threads.  Returns (factors, pivots) in the reference's
+    encoding."""
+    L, _ = host_blas()
+    a = _f(a).copy(order="F")
+    m, n = a.shape
+    mn = min(m, n)
+    piv = np.zeros(mn, dtype=np.int32)
+    ip = np.zeros(max(nb, 1), dtype=np.int32)
+    info = ctypes.c_int(0)
+    for k in range(0, mn, nb):
+        b = min(nb, mn - k)
+        L.scipy_dgetrf_(_ri(m - k), _ri(b), _at(a, k, k), _ri(m), _ptr(ip), ctypes.byref(info))
+        piv[k:k + b] = ip[:b] - 1 - np.arange(b)          # 1-based panel rows -> offsets from row j
+        if k:
+            L.scipy_dlaswp_(_ri(k), _at(a, k, 0), _ri(m), _ri(1), _ri(b), _ptr(ip), _ri(1))
+        if k + b < n:
+            L.scipy_dlaswp_(_ri(n - k - b), _at(a, k, k + b), _ri(m), _ri(1), _ri(b), _ptr(ip), _ri(1))
+            L.scipy_dtrsm_(_rc("L"), _rc("L"), _rc("N"), _rc("U"), _ri(b), _ri(n - k - b), _rd(1.0), _at(a, k, k),
+                           _ri(m), _at(a, k, k + b), _ri(m))
+            if k + b < m:
+                L.scipy_dgemm_(_rc("N"), _rc("N"), _ri(m - k - b), _ri(n - k - b), _ri(b), _rd(-1.0),
+                               _at(a, k + b, k), _ri(m), _at(a, k, k + b), _ri(m), _rd(1.0), _at(a, k + b, k + b),
+                               _ri(m))
+    return a, piv
+
+
+def chol_blocked_blas(a, nb=128):
+    """Blocked right-looking lower Cholesky structured as PLA_Chol_lower_large
+    (SOLVERS/Cholesky/PLA_Chol.c:83-234), flops in the host BLAS/LAPACK:
+    dpotrf on the diagonal block (:178), dtrsm A21 <- A21 inv(L11)^T
+    (:189-195), and the lower trailing update (:207-210) as dsyrk on each
+    diagonal block plus dgemm below it, so the upper triangle is never
+    written.  Returns (factors, info), info the 1-based index of the first
+    non-positive pivot (0 if none)."""
+    L, _ = host_blas()
+    a = _f(a).copy(order="F")
+    n = a.shape[0]
+    info = ctypes.c_int(0)
+    for k in range(0, n, nb):
+        b = min(nb, n - k)
+        L.scipy_dpotrf_(_rc("L"), _ri(b), _at(a, k, k), _ri(n), ctypes.byref(info))
+        if info.value > 0:
+            return a, k + info.value
+        r = n - k - b
+        if r <= 0:
+            continue
+        L.scipy_dtrsm_(_rc("R"), _rc("L"), _rc("T"), _rc("N"), _ri(r), _ri(b), _rd(1.0), _at(a, k, k), _ri(n),
+                       _at(a, k + b, k), _ri(n))
+        for j in range(k + b, n, nb):
+            w = min(nb, n - j)
+            L.scipy_dsyrk_(_rc("L"), _rc("N"), _ri(w), _ri(b), _rd(-1.0), _at(a, j, k), _ri(n), _rd(1.0),
+                           _at(a, j, j), _ri(n))
+            if j + w < n:
+                L.scipy_dgemm_(_rc("N"), _rc("T"), _ri(n - j - w), _ri(w), _ri(b), _rd(-1.0), _at(a, j + w, k),
+                               _ri(n), _at(a, j, k), _ri(n), _rd(1.0), _at(a, j + w, j), _ri(n))
+    return a, 0
+
+
 def lu_linpack(a, nb=128):
     """PLA_LU_linpack (ora_lu_linpack): returns (factors, pivots)."""
     a = _f(a).copy(order="F")

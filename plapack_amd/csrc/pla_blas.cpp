@@ -175,12 +175,12 @@ struct role_timer {
   long done_launches = 0;
 };
 static bool g_timing = false;
-static role_timer g_timers[5];
+static role_timer g_timers[8];
 
 bool timing_on() { return g_timing; }
 
 void timing_begin(hipStream_t s, int role) {
-  if (!g_timing || role < 1 || role > 4) return;
+  if (!g_timing || role < 1 || role > 7) return;
   hipEvent_t e;
   if (hipEventCreate(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
@@ -188,7 +188,7 @@ void timing_begin(hipStream_t s, int role) {
 }
 
 void timing_end(hipStream_t s, int role, double flops) {
-  if (!g_timing || role < 1 || role > 4) return;
+  if (!g_timing || role < 1 || role > 7) return;
   role_timer &t = g_timers[role];
   if (t.ev.size() % 2 != 1) return;
   hipEvent_t e;
@@ -245,7 +245,7 @@ int PLA_Kernel_timing(int on) {
 }
 
 int PLA_Kernel_timing_query(int role, long *launches, double *total_ms, double *total_flops) {
-  if (role < 1 || role > 4) return fail(PLA_ERR_INVALID_PARAM, "PLA_Kernel_timing_query: role %d", role);
+  if (role < 1 || role > 7) return fail(PLA_ERR_INVALID_PARAM, "PLA_Kernel_timing_query: role %d", role);
   role_timer &t = g_timers[role];
   timing_collect(t);
   if (launches) *launches = t.done_launches;

@@ -222,12 +222,15 @@ int sym_residual(hipStream_t s, int n, int nrhs, const double *A, long lda, cons
     hipLaunchKernelGGL((sresid_n_kernel<NR, AB>), gn, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, pn); \
     hipLaunchKernelGGL((sresid_t_kernel<NR, AB>), gt, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, pt); \
   } while (0)
+  timing_begin(s, 5);
   if (abs_mode) PLA_SR(1, true);
   else if (nrhs <= 1) PLA_SR(1, false);
   else if (nrhs <= 4) PLA_SR(4, false);
   else if (nrhs <= 8) PLA_SR(8, false);
   else PLA_SR(16, false);
 #undef PLA_SR
+  // algorithmic bytes: the lower triangle once (both kernels read it), X, the partials
+  timing_end(s, 5, 4.0 * n * (n + 1.0) + 8.0 * nrhs * (double)n * (1 + 2 * nch));
   hipLaunchKernelGGL(sresid_final_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nrhs, nch, pn, pt, B,
                      ldb, R, ldr, mx);
   PLA_HIP(hipGetLastError());

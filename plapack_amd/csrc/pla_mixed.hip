@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
       }
       if (__syncthreads_and(ok)) break;
       ++it;
-      const int bail = (it > spin_limit) || (tid == 0 && (it & 63) == 0 &&
+      const int bail = (it >= spin_limit) || (tid == 0 && (it & 63) == 0 &&
                                                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
       if (__syncthreads_or(bail)) {
         failed = true;
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
       }
       if (__syncthreads_and(ok)) break;
       ++it;
-      const int bail = (it > spin_limit) || (tid == 0 && (it & 63) == 0 &&
+      const int bail = (it >= spin_limit) || (tid == 0 && (it & 63) == 0 &&
                                                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
       if (__syncthreads_or(bail)) {
         failed = true;
@@ -816,9 +816,14 @@ static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n,
     g_trsv_tag = 1;
   }
   const unsigned tag = g_trsv_tag++;
-  if (nrhs <= 1) return trsv_launch<T, 1>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
-  if (nrhs <= 4) return trsv_launch<T, 4>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
-  return trsv_launch<T, MAXRHS>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  int rc;
+  timing_begin(s, 6);
+  if (nrhs <= 1) rc = trsv_launch<T, 1>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else if (nrhs <= 4) rc = trsv_launch<T, 4>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else rc = trsv_launch<T, MAXRHS>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  // algorithmic bytes: the triangle once, the inverted diagonal blocks once, X read and written
+  timing_end(s, 6, sizeof(T) * 0.5 * n * (n + 1.0) + 8.0 * TB * (double)n + 16.0 * nrhs * (double)n);
+  return rc;
 }
 
 int trsv_sf(hipStream_t s, bool lower, bool trans, bool unit, int n, int nrhs, const double *A, long lda, double *X,
@@ -881,12 +886,15 @@ int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const do
   if (!part || !mx) return PLA_ERR_NO_MEMORY;
   PLA_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long) * MAXRHS, s));
   const dim3 rg((n + 255) / 256, nch);
+  timing_begin(s, 5);
   if (nrhs <= 1)
     hipLaunchKernelGGL(resid_partial_kernel<1>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
   else if (nrhs <= 4)
     hipLaunchKernelGGL(resid_partial_kernel<4>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
   else
     hipLaunchKernelGGL(resid_partial_kernel<MAXRHS>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
+  // algorithmic bytes: A once, X once per column chunk (cached), the partials written once
+  timing_end(s, 5, 8.0 * n * (double)n + 8.0 * nrhs * (double)n * (1 + nch));
   hipLaunchKernelGGL(resid_final_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nrhs, nch, part, B, ldb,
                      R, ldr, mx);
   PLA_HIP(hipGetLastError());

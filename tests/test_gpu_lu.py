@@ -310,13 +310,15 @@ def test_lu_exchange_timeout_reports_error(pla_session, ora, kind):
 @pytest.mark.parametrize("entry", ["PLA_Trsv", "PLA_General_solve", "PLA_Pos_def_solve"])
 def test_triangular_solve_timeout_reports_error(pla_session, ora, entry):
     """A triangular-solve block wait that gives up (forced from the host with
-    a one-poll limit) makes the entry point return PLA_ERR_DEVICE instead of a
-    silently wrong solution; the error word is cleared, so the next solve and
-    the next LU are correct."""
+    a one-poll limit: a block gives up at the first poll that finds a
+    preceding block's solution unpublished, which along a chain of 128 blocks
+    happens to the later ones) makes the entry point return PLA_ERR_DEVICE
+    instead of a silently wrong solution; the error word is cleared, so the
+    next solve and the next LU are correct."""
     hooks = ctypes.CDLL(os.path.join(os.path.dirname(pla.LIB_PATH), "libpla_testhooks.so"))
     hooks.pla_th_set_trsv_spin_limit.argtypes = [ctypes.c_uint]
     s = pla_session
-    n = 2000
+    n = 8192
     a = ora.uniform(n, n, 6)
     spd = a @ a.T / n + np.eye(n)
     b = np.random.default_rng(6).uniform(-1, 1, (n, 1))

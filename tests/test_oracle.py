@@ -424,3 +424,29 @@ def test_fp8_tile_quantization_scales(ora):
     deq = ora.e4m3_decode(q[:, :200]) * np.exp2(np.repeat(ex, [256, 256, 88]))[:, None]
     assert np.abs(deq - p).max() <= 2.0 ** -4 * np.abs(p).max()     # unit roundoff 2^-4
     assert (q[:, 200:] == 0).all()
+
+
+def test_blas_restatements_match_the_scalar_oracle(ora):
+    """The CPU baseline's restatements over the host BLAS (lu_blocked_blas,
+    chol_blocked_blas) follow the same algorithms as the scalar oracle: the
+    same pivots (and the golden n=2048 vector), factors within rounding, the
+    same Cholesky failure index, the upper triangle never written."""
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lu_piv_n2048_seed0.npy"))
+    lu_b, piv_b = ora.lu_blocked_blas(ora.uniform(2048, 2048, 0), nb=128)
+    assert np.array_equal(piv_b, golden)
+    for m, n, nb in ((300, 300, 64), (700, 500, 128), (500, 700, 128), (1000, 1000, 96)):
+        a = ora.uniform(m, n, m + n)
+        lu_s, piv_s = ora.lu_blocked(a, nb=nb)
+        lu_b, piv_b = ora.lu_blocked_blas(a, nb=nb)
+        assert np.array_equal(piv_s, piv_b)
+        assert np.abs(lu_s - lu_b).max() <= 1e-10
+    rng = np.random.default_rng(4)
+    b = rng.uniform(-1, 1, (600, 600))
+    spd = b @ b.T / 600 + 1e-2 * np.eye(600)
+    l_s, i_s = ora.chol_blocked(spd, nb=128)
+    l_b, i_b = ora.chol_blocked_blas(spd, nb=128)
+    assert i_s == i_b == 0
+    assert np.abs(np.tril(l_s) - np.tril(l_b)).max() <= 1e-12
+    assert np.array_equal(np.triu(l_b, 1), np.triu(spd, 1))
+    spd[333, 333] = -1.0
+    assert ora.chol_blocked(spd, nb=128)[1] == ora.chol_blocked_blas(spd, nb=128)[1] == 334
