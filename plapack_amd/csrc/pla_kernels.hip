@@ -323,6 +323,36 @@ __device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, i
   gemm_store_c<TRI>(m, n, 1.0, C, ldc, i0, j0, acc);
 }
 
+// Tile b of the lower triangle (diagonal included) of a tiles x tiles grid,
+// in bands of `group` tile rows: inside a band the columns left of it (all
+// the band's rows), then the band's own triangle column by column.  False
+// once b is past the last tile.
+__device__ inline bool tri_tile(int b, int tiles, int group, int *tm, int *tn) {
+  for (int r0 = 0; r0 < tiles; r0 += group) {
+    const int h = min(group, tiles - r0);
+    const int full = r0 * h;
+    const int cnt = full + h * (h + 1) / 2;
+    if (b < cnt) {
+      if (b < full) {
+        *tn = b / h;
+        *tm = r0 + b % h;
+        return true;
+      }
+      b -= full;
+      for (int c = 0; c < h; c++) {
+        if (b < h - c) {
+          *tn = r0 + c;
+          *tm = r0 + c + b;
+          return true;
+        }
+        b -= h - c;
+      }
+    }
+    b -= cnt;
+  }
+  return false;
+}
+
 // fast: alpha is +-1 and every operand lane offset fits 32 bits (host check)
 template <bool TA, bool TB, int TRI, int ROLE>
 __global__ __launch_bounds__(256, 2) void dgemm_kernel(
@@ -330,21 +360,31 @@ __global__ __launch_bounds__(256, 2) void dgemm_kernel(
     const double *__restrict__ B, long ldb, double beta, double *__restrict__ C, long ldc,
     int tiles_m, int tiles_n, const int *__restrict__ abort_flag, int fast) {
   if (abort_flag && *abort_flag) return;
-  const int ntiles = tiles_m * tiles_n;
   // Consecutive workgroup ids are dispatched round-robin over the 8 XCDs;
   // give each XCD a contiguous range of tiles so its L2 sees neighbours.
   const int nblk = gridDim.x;  // a multiple of 8
   int bid = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-  if (bid >= ntiles) return;
   constexpr int GROUP = 8;  // tile rows per group: A rows are reused inside a group
-  const int group = bid / (GROUP * tiles_n);
-  const int first_m = group * GROUP;
-  const int gsz = min(tiles_m - first_m, GROUP);
-  const int within = bid - group * GROUP * tiles_n;
-  const int tm = first_m + within % gsz;
-  const int tn = within / gsz;
+  int tm, tn;
+  if (TRI == 0) {
+    if (bid >= tiles_m * tiles_n) return;
+    const int group = bid / (GROUP * tiles_n);
+    const int first_m = group * GROUP;
+    const int gsz = min(tiles_m - first_m, GROUP);
+    const int within = bid - group * GROUP * tiles_n;
+    tm = first_m + within % gsz;
+    tn = within / gsz;
+  } else {
+    // only the triangle's tiles are launched (a square tile grid), so every
+    // XCD's contiguous range holds the same number of them
+    if (!tri_tile(bid, tiles_m, GROUP, &tm, &tn)) return;
+    if (TRI == 2) {
+      const int t = tm;
+      tm = tn;
+      tn = t;
+    }
+  }
   const int i0 = tm * GBM, j0 = tn * GBN;
-  if ((TRI == 1 && i0 + GBM <= j0) || (TRI == 2 && j0 + GBN <= i0)) return;
   __shared__ gemm_smem sm;
   if (fast && i0 + GBM <= m && j0 + GBN <= n)
     gemm_tile_fast<TA, TB, TRI>(sm, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, i0, j0);
@@ -1171,7 +1211,10 @@ static void launch_gemm(hipStream_t s, int m, int n, int k, double alpha, const 
                         const double *B, long ldb, double beta, double *C, long ldc,
                         const int *abort_flag) {
   const int tm = (m + GBM - 1) / GBM, tn = (n + GBN - 1) / GBN;
-  const int nblk = ((tm * tn + 7) / 8) * 8;
+  // triangular updates (square C: m == n, GBM == GBN) launch only the
+  // triangle's tm (tm + 1) / 2 tiles
+  const long nt = TRI ? (long)tm * (tm + 1) / 2 : (long)tm * tn;
+  const int nblk = (int)(((nt + 7) / 8) * 8);
   // the interior path addresses a lane's operand pairs as a 32-bit byte offset
   // of at most (31 ld + 128) doubles from a uniform base
   const long lim = (1L << 32) / 8 - 128;
