@@ -326,6 +326,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1, help="plain FP64 steps timed beside mixed (0: skip)")
     ap.add_argument("--chol-n", type=int, default=32768, help="PLA_Chol size (0: skip)")
     ap.add_argument("--chol-steps", type=int, default=2)
+    ap.add_argument("--chol-panel-cus", type=int, default=-1,
+                    help="CUs reserved for the Cholesky panel (-1: the library's default)")
     ap.add_argument("--fp8-n", type=int, default=49152, help="FP8 PLA_Chol_mixed size (0: skip)")
     ap.add_argument("--fp8-deltas", type=str, default="4e-2,4e-4")
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU baseline LU size (0: skip)")
@@ -500,6 +502,8 @@ def main():
                 out["speedup_vs_plain"] = round(value / plain["value"], 3)
         pla.free(A0, ones, b, x, r, P, one, zero, mone, nrm)
         pla.call("PLA_Set_device_nb", 0)
+        if args.chol_panel_cus >= 0:
+            pla.call("PLA_Set_device_param", pla.PLA_PARAM_CHOL_PANEL_CUS, args.chol_panel_cus)
         if args.chol_n > 0 or args.fp8_n > 0:
             deltas = [float(v) for v in args.fp8_deltas.split(",") if v]
             out["cholesky"] = run_cholesky(pla, s, args.chol_n, args.fp8_n, deltas, args.chol_steps)

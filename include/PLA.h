@@ -560,6 +560,11 @@ int PLA_Set_device_nb(int nb);
 #define PLA_PARAM_LEAF_KIND 5
 #define PLA_PARAM_NB_TAIL 6
 #define PLA_PARAM_NB_SWITCH 7
+/* PLA_PARAM_CHOL_PANEL_CUS: CUs reserved for the Cholesky panel (diagonal
+   block and the solve below it) while the trailing update runs on the others
+   (a multiple of 32 below the device's CU count; 0 shares every CU).  The
+   CU-masked stream pair is created when the parameter is set. */
+#define PLA_PARAM_CHOL_PANEL_CUS 8
 int PLA_Set_device_param(int param, int value);
 int PLA_Get_device_nb(int n, int *nb);
 /* Block size the device uses for an n x n factorization of the given kind:

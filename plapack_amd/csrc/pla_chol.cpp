@@ -30,8 +30,7 @@ namespace pla {
 //                       A22[bn:, :bn] -= A21[bn:] A21[:bn]^T -> signal
 //                       -> lower(A22[bn:, bn:]) -= A21[bn:] A21[bn:]^T
 // look-ahead stream:    wait(signal) -> back up + factor A11 -> A21 inv(L11)^T
-int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out, const chol_scratch *scr) {
-  hipStream_t sP = ctx().lookahead ? ctx().stream_la : sM;
+int chol_factor(hipStream_t sMain, int n, double *a, long lda, int nb, int *info_out, const chol_scratch *scr) {
   if (nb > CHOL_MAX_NB) nb = CHOL_MAX_NB;
   if (nb < 1) nb = 1;
   // info: first failure index, written by the panels on sP.  seen: the
@@ -55,8 +54,12 @@ int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_ou
   hipEvent_t ev_ready = lib_event(EV_CHOL_READY), ev_panel = lib_event(EV_CHOL_PANEL);
   if (!ev_ready || !ev_panel) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   const bool timed = !scr;
-  PLA_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int), sM));
+  PLA_HIP(hipMemsetAsync(info, 0, 2 * sizeof(int), sMain));
   int rc;
+  // sP: the panels; sM: the trailing updates (CU-masked apart when
+  // PLA_PARAM_CHOL_PANEL_CUS reserves CUs for the panels)
+  hipStream_t sP, sM;
+  if ((rc = chol_streams_begin(sMain, &sP, &sM))) return rc;
   // factor the column block that starts at diagonal position k (on sP)
   auto panel = [&](int k) -> int {
     const int b = n - k < nb ? n - k : nb;
@@ -68,6 +71,8 @@ int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_ou
     if (r > 0 && (rc2 = trsm_rltn(sP, r, b, Akk, lda, Akk + b, lda, info))) return rc2;
     return PLA_SUCCESS;
   };
+  auto steps = [&]() -> int {
+  int rc;
   PLA_HIP(hipEventRecord(ev_ready, sM));
   PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
   if ((rc = panel(0))) return rc;
@@ -100,6 +105,14 @@ int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_ou
       if (rc) return rc;
     }
   }
+  return PLA_SUCCESS;
+  };
+  rc = steps();
+  // the caller's stream waits for both streams, also after an error
+  const int rc_end = lu_streams_end(sMain, sP, sM);
+  if (rc) return rc;
+  if (rc_end) return rc_end;
+  sM = sMain;
   int h_info = 0;
   PLA_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, sM));
   PLA_HIP(hipStreamSynchronize(sM));

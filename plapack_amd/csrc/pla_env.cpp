@@ -56,6 +56,10 @@ int ensure_device() {
   PLA_HIP(hipStreamCreateWithFlags(&c.stream_tr, hipStreamNonBlocking));
   for (auto &e : g_events)
     if (!e) PLA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (c.chol_panel_cus > 0 && c.chol_panel_cus < c.cu_count) {
+    const int rc = masked_streams(c.chol_panel_cus);
+    if (rc) return rc;
+  }
   // the no-allocation Cholesky's static scratch: its address on this device
   void *scratch_ptr = nullptr;
   return chol_static_scratch(c.device, &scratch_ptr);
@@ -163,6 +167,28 @@ int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, 
   } else {
     *sP = c.stream_la;
     *sT = c.stream_tr;
+  }
+  PLA_HIP(hipEventRecord(g_events[EV_JOIN0], main));
+  PLA_HIP(hipStreamWaitEvent(*sP, g_events[EV_JOIN0], 0));
+  if (*sT != main) PLA_HIP(hipStreamWaitEvent(*sT, g_events[EV_JOIN0], 0));
+  return PLA_SUCCESS;
+}
+
+int masked_streams(int k) { return make_masked_streams(g_ctx, k); }
+
+int chol_streams_begin(hipStream_t main, hipStream_t *sP, hipStream_t *sT) {
+  context &c = g_ctx;
+  if (!c.lookahead) {
+    *sP = *sT = main;
+    return PLA_SUCCESS;
+  }
+  if (!g_events[EV_JOIN0]) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
+  if (c.chol_panel_cus > 0 && c.masked_cus == c.chol_panel_cus && c.stream_panel) {
+    *sP = c.stream_panel;
+    *sT = c.stream_trail;
+  } else {
+    *sP = c.stream_la;
+    *sT = main;
   }
   PLA_HIP(hipEventRecord(g_events[EV_JOIN0], main));
   PLA_HIP(hipStreamWaitEvent(*sP, g_events[EV_JOIN0], 0));
@@ -475,6 +501,13 @@ int PLA_Set_device_param(int param, int value) {
       if (value < 0 || value % 32 != 0 || (value > 0 && value >= c.cu_count))
         return fail(PLA_ERR_INVALID_PARAM, "panel CUs must be a multiple of 32 below %d", c.cu_count);
       c.panel_cus = value;
+      return PLA_SUCCESS;
+    case PLA_PARAM_CHOL_PANEL_CUS:
+      if (value < 0 || value % 32 != 0 || (value > 0 && value >= c.cu_count))
+        return fail(PLA_ERR_INVALID_PARAM, "panel CUs must be a multiple of 32 below %d", c.cu_count);
+      c.chol_panel_cus = value;
+      // the masked pair is made here, so the factorization itself creates nothing
+      if (value > 0 && c.initialized) return pla::masked_streams(value);
       return PLA_SUCCESS;
     case PLA_PARAM_LEAF_KIND:
       if (value != 0 && value != 1) return fail(PLA_ERR_INVALID_PARAM, "leaf kind must be 0 or 1");

@@ -72,6 +72,7 @@ struct context {
   int lookahead = 1;        // factor panel k+1 beside step k's trailing update
   int leaf_rows = 256;      // target rows per workgroup of the panel leaf
   int panel_cus = 0;        // CUs reserved for the LU panel (0: panel shares every CU)
+  int chol_panel_cus = 32;  // CUs reserved for the Cholesky panel (0: panel shares every CU)
   int leaf_width = 0;       // panel-LU leaf width (0: the widest that fits)
   int leaf_kind = 0;        // panel-LU leaf: 0 rows in registers, 1 rows in LDS
   int nb_tail = 0;          // LU block size once at most nb_switch rows remain (0: none)
@@ -105,6 +106,13 @@ hipEvent_t lib_event(int id);
 // for both.
 int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, int *gcap);
 int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
+// The same for the Cholesky look-ahead: *sP the panel factorizations, *sT the
+// trailing updates.  With PLA_PARAM_CHOL_PANEL_CUS set they are the CU-masked
+// pair (created when the parameter is set, never here); otherwise *sP is the
+// look-ahead stream and *sT is `main`.  End with lu_streams_end.
+int chol_streams_begin(hipStream_t main, hipStream_t *sP, hipStream_t *sT);
+// Creates (or re-creates) the CU-masked stream pair with k panel CUs.
+int masked_streams(int k);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.

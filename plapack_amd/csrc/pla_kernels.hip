@@ -969,12 +969,27 @@ __global__ __launch_bounds__(256) void trsm_right_leaf_kernel(int nrows, int b, 
                                                               const int *__restrict__ abort_flag) {
   if (abort_flag && *abort_flag) return;
   __shared__ double sM[TRSM_LEAF][TRSM_LEAF + 1];
-  for (int e = threadIdx.x; e < b * b; e += 256) {
-    const int col = e / b, row = e - col * b;  // T(row, col), read down the columns
-    if (trans)
-      sM[col][row] = T[(long)col * ldt + row];
-    else
-      sM[row][col] = T[(long)col * ldt + row];
+  {
+    // the b x b triangle block: all of a thread's loads in flight at once
+    constexpr int PER = TRSM_LEAF * TRSM_LEAF / 256;
+    double tv[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + 256 * q;
+      const int col = e / b, row = e - col * b;  // T(row, col), read down the columns
+      tv[q] = e < b * b ? T[(long)col * ldt + row] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + 256 * q;
+      const int col = e / b, row = e - col * b;
+      if (e < b * b) {
+        if (trans)
+          sM[col][row] = tv[q];
+        else
+          sM[row][col] = tv[q];
+      }
+    }
   }
   __syncthreads();
   const int r = blockIdx.x * 256 + threadIdx.x;
@@ -1014,8 +1029,15 @@ __global__ __launch_bounds__(64) void potrf_leaf_kernel(int b, double *__restric
   __shared__ double S[TRSM_LEAF][TRSM_LEAF + 1];  // S[i][j] = A(i,j), lower part used
   __shared__ int fail;
   const int i = threadIdx.x;
-  for (int j = 0; j < b; j++)
-    if (i < b && i >= j) S[i][j] = Ablk[(long)j * lda + i];
+  {
+    // every load of the row in flight at once, then into LDS
+    double row[TRSM_LEAF];
+#pragma unroll
+    for (int j = 0; j < TRSM_LEAF; j++) row[j] = (j < b && i < b && i >= j) ? Ablk[(long)j * lda + i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < TRSM_LEAF; j++)
+      if (j < b && i < b && i >= j) S[i][j] = row[j];
+  }
   if (i == 0) fail = 0;
   __syncthreads();
   for (int kk = 0; kk < b; kk++) {

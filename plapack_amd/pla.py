@@ -70,6 +70,7 @@ PLA_PARAM_LEAF_WIDTH = 4
 PLA_PARAM_LEAF_KIND = 5
 PLA_PARAM_NB_TAIL = 6
 PLA_PARAM_NB_SWITCH = 7
+PLA_PARAM_CHOL_PANEL_CUS = 8
 PLA_NB_LU = 0
 PLA_NB_CHOL = 1
 PLA_NB_LU_MIXED = 2
