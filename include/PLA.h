@@ -14,9 +14,16 @@
  * Entry points marked EXTENSION have no counterpart in the reference.
  *
  * Error behaviour: functions return PLA_SUCCESS (0) or a negative error
- * code; PLA_Last_error() gives the message.  The library never calls
- * exit() (the reference's PLA_Abort does, OBJ/PLA_Diagnostics.c:108-129).
+ * code; PLA_Last_error() gives the message.  No entry point ends the
+ * process except PLA_Abort, which a caller invokes itself (the reference's
+ * routines abort on bad input, OBJ/PLA_Diagnostics.c:108-129).
  * Not thread-safe, like the reference.
+ *
+ * Datatypes: every routine computes in MPI_DOUBLE or MPI_DOUBLE_COMPLEX.
+ * MPI_FLOAT and MPI_COMPLEX objects are accepted wherever the reference
+ * accepts them: they are widened into FP64 temporaries, the FP64 routine
+ * runs, and the results are rounded once back into the objects (so they
+ * are at least as accurate as single-precision arithmetic would give).
  */
 #ifndef PLA_MI355X_H
 #define PLA_MI355X_H
@@ -353,7 +360,8 @@ int PLA_API_multi_axpy_global_to_matrix(int nsub_row, int nsub_col, int *size_ro
                                         int local_ldim);
 
 /* ---- factor and solve path ---- */
-/* SOLVERS/LU/PLA_LU.c:13.  A: m x n MPI_DOUBLE or MPI_DOUBLE_COMPLEX matrix
+/* SOLVERS/LU/PLA_LU.c:13.  A: m x n MPI_DOUBLE, MPI_DOUBLE_COMPLEX, MPI_FLOAT
+   or MPI_COMPLEX matrix
    (complex pivots: the first maximum of |re| + |im|, izamax's rule),
    overwritten with L (unit, below the diagonal) and U; pivots: MPI_INT mvector of length
    min(m,n) receiving, for each row j, the 0-based offset of its pivot row
@@ -365,7 +373,7 @@ int PLA_LU(PLA_Obj A, PLA_Obj pivots);
    order of their own step (LINPACK's dgefa layout); on [A b] (n x (n+1)) the
    last column ends as inv(L) P b, ready for PLA_Trsv with U */
 int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots);
-/* SOLVERS/Cholesky/PLA_Chol.c:13.  MPI_DOUBLE (A = L L^T) or
+/* SOLVERS/Cholesky/PLA_Chol.c:13.  MPI_DOUBLE or MPI_FLOAT (A = L L^T), or
    MPI_DOUBLE_COMPLEX (A = L L^H, zero imaginary diagonal).  Lower only (the
    reference aborts on upper).  Returns 0, or the 1-based global index k+i+1 of the first
    non-positive pivot (PLA_Chol.c:181-185); the diagonal block that failed
@@ -380,11 +388,14 @@ int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots);
 /* BLAS2/PLA_Trsv.c:15 */
 int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x);
 /* BLAS3/PLA_Trsm.c:13: B <- alpha op(A)^-1 B (PLA_SIDE_LEFT) or
-   alpha B op(A)^-1 (PLA_SIDE_RIGHT), every uplo / trans / diag */
+   alpha B op(A)^-1 (PLA_SIDE_RIGHT), every uplo / trans / diag; real or
+   complex (trans also PLA_CONJUGATE_TRANSPOSE and PLA_CONJUGATE) */
 int PLA_Trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, PLA_Obj B);
 /* BLAS2/PLA_Gemv.c:13 */
 int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y);
-/* BLAS3/PLA_Gemm.c:13 */
+/* BLAS3/PLA_Gemm.c:13: real or complex; for complex operands transa/transb
+   also take PLA_CONJUGATE_TRANSPOSE and PLA_CONJUGATE (conj without
+   transpose) */
 int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta,
              PLA_Obj C);
 /* BLAS3/PLA_Gemm_x.c:13: the reference's distributed algorithm variants
@@ -410,7 +421,8 @@ int PLA_Hemv(int uplo, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Ob
    obj's datatype) to the diagonal of obj (MPI_DOUBLE or MPI_FLOAT) */
 int PLA_Shift(PLA_Obj obj, MPI_Datatype datatype, void *value);
 /* BLAS3/PLA_Syrk.c:13: C <- beta C on all of C (PLA_Syrk_panpan.c:22), then
-   the uplo triangle += alpha op(A) op(A)^T */
+   the uplo triangle += alpha op(A) op(A)^T (op N or T; complex objects give
+   the complex-symmetric update, no conjugation) */
 int PLA_Syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Obj C);
 /* BLAS2/PLA_Matrix_infinity_norm.c:13, BLAS2/PLA_Matrix_one_norm.c:13 --
    alpha (1x1 MPI_DOUBLE) <- max absolute row / column sum of A */
@@ -562,8 +574,9 @@ int PLA_Set_device_nb(int nb);
 #define PLA_PARAM_NB_SWITCH 7
 /* PLA_PARAM_CHOL_PANEL_CUS: CUs reserved for the Cholesky panel (diagonal
    block and the solve below it) while the trailing update runs on the others
-   (a multiple of 32 below the device's CU count; 0 shares every CU).  The
-   CU-masked stream pair is created when the parameter is set. */
+   (a multiple of 32 below the device's CU count; 0 shares every CU; 32 by
+   default, one CU per shader engine).  The CU-masked stream pair is created
+   at PLA_Init and when the parameter is set. */
 #define PLA_PARAM_CHOL_PANEL_CUS 8
 int PLA_Set_device_param(int param, int value);
 int PLA_Get_device_nb(int n, int *nb);

@@ -255,6 +255,7 @@ int PLA_Kernel_timing_query(int role, long *launches, double *total_ms, double *
 }
 
 int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots) {
+  if (is_single(A)) return on_promoted({&A}, {true}, [&] { return PLA_Apply_pivots_to_rows(A, pivots); });
   const bool cplx = valid(A) && A->base->datatype == MPI_DOUBLE_COMPLEX;
   int rc = cplx ? PLA_SUCCESS : check_double(A, "PLA_Apply_pivots_to_rows");
   if (rc) return rc;
@@ -285,6 +286,10 @@ int PLA_Apply_pivots_to_rows(PLA_Obj A, PLA_Obj pivots) {
 }
 
 int PLA_Trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, PLA_Obj B) {
+  if (is_single(A) || is_single(B))
+    return on_promoted({&alpha, &A, &B}, {false, false, true},
+                       [&] { return PLA_Trsm(side, uplo, trans, diag, alpha, A, B); });
+  if (valid(A) && A->base->datatype == MPI_DOUBLE_COMPLEX) return zblas_trsm(side, uplo, trans, diag, alpha, A, B);
   int rc;
   if ((rc = check_double(A, "PLA_Trsm")) || (rc = check_double(B, "PLA_Trsm"))) return rc;
   if (uplo != PLA_LOWER_TRIANGULAR && uplo != PLA_UPPER_TRIANGULAR)
@@ -318,6 +323,14 @@ int PLA_Trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, 
 }
 
 int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x) {
+  if (is_single(A) || is_single(x))
+    return on_promoted({&A, &x}, {false, true}, [&] { return PLA_Trsv(uplo, trans, diag, A, x); });
+  // complex: the one-column case of the complex triangular solve
+  if (valid(A) && A->base->datatype == MPI_DOUBLE_COMPLEX) {
+    if (!valid(x) || x->width != 1)
+      return fail(PLA_ERR_DIMENSIONS, "PLA_Trsv: x must be one column");
+    return zblas_trsm(PLA_SIDE_LEFT, uplo, trans, diag, nullptr, A, x);
+  }
   int rc;
   if ((rc = check_double(A, "PLA_Trsv")) || (rc = check_double(x, "PLA_Trsv"))) return rc;
   const int n = A->length;
@@ -341,6 +354,10 @@ int PLA_Trsv(int uplo, int trans, int diag, PLA_Obj A, PLA_Obj x) {
 }
 
 int PLA_Gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta, PLA_Obj C) {
+  if (is_single(A) || is_single(B) || is_single(C))
+    return on_promoted({&alpha, &A, &B, &beta, &C}, {false, false, false, false, true},
+                       [&] { return PLA_Gemm(transa, transb, alpha, A, B, beta, C); });
+  if (valid(C) && C->base->datatype == MPI_DOUBLE_COMPLEX) return zblas_gemm(transa, transb, alpha, A, B, beta, C);
   int rc;
   if ((rc = check_double(A, "PLA_Gemm")) || (rc = check_double(B, "PLA_Gemm")) ||
       (rc = check_double(C, "PLA_Gemm")))
@@ -377,6 +394,10 @@ int PLA_Gemm_x(int version, int transa, int transb, PLA_Obj alpha, PLA_Obj A, PL
 // PLA_NO_TRANS / PLA_CONJ: op(A) = A, n x k; PLA_TRANS / PLA_CONJ_TRANS:
 // op(A) = A^T, A k x n (real data: conjugation is the identity).
 int PLA_Syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Obj C) {
+  if (is_single(A) || is_single(C))
+    return on_promoted({&alpha, &A, &beta, &C}, {false, false, false, true},
+                       [&] { return PLA_Syrk(uplo, trans, alpha, A, beta, C); });
+  if (valid(C) && C->base->datatype == MPI_DOUBLE_COMPLEX) return zblas_syrk(uplo, trans, alpha, A, beta, C);
   int rc;
   if ((rc = check_double(A, "PLA_Syrk")) || (rc = check_double(C, "PLA_Syrk"))) return rc;
   if (uplo != PLA_LOWER_TRIANGULAR && uplo != PLA_UPPER_TRIANGULAR)
@@ -405,6 +426,8 @@ int PLA_Syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Ob
 }
 
 static int matrix_norm(PLA_Obj A, PLA_Obj alpha, bool inf_norm, const char *who) {
+  if (is_single(A) || is_single(alpha))
+    return on_promoted({&A, &alpha}, {false, true}, [&] { return matrix_norm(A, alpha, inf_norm, who); });
   int rc;
   if ((rc = check_double(A, who)) || (rc = check_double(alpha, who))) return rc;
   double *a, v = 0.0;
@@ -425,6 +448,14 @@ int PLA_Matrix_one_norm(PLA_Obj A, PLA_Obj alpha) {
 }
 
 int PLA_Gemv(int transa, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y) {
+  if (is_single(A) || is_single(x) || is_single(y))
+    return on_promoted({&alpha, &A, &x, &beta, &y}, {false, false, false, false, true},
+                       [&] { return PLA_Gemv(transa, alpha, A, x, beta, y); });
+  if (valid(A) && A->base->datatype == MPI_DOUBLE_COMPLEX) {
+    if (!valid(x) || !valid(y) || x->width != 1 || y->width != 1)
+      return fail(PLA_ERR_DIMENSIONS, "PLA_Gemv: x and y must be one column");
+    return zblas_gemm(transa, PLA_NO_TRANSPOSE, alpha, A, x, beta, y);
+  }
   int rc;
   if ((rc = check_double(A, "PLA_Gemv")) || (rc = check_double(x, "PLA_Gemv")) ||
       (rc = check_double(y, "PLA_Gemv")))

@@ -132,6 +132,7 @@ int chol_factor(hipStream_t sMain, int n, double *a, long lda, int nb, int *info
 extern "C" {
 
 int PLA_Chol(int uplo, PLA_Obj A) {
+  if (is_single(A)) return on_promoted({&A}, {true}, [&] { return PLA_Chol(uplo, A); });
   // SOLVERS/Cholesky/PLA_Chol_enter_exit.c:43-47: only the lower triangle
   if (uplo != PLA_LOWER_TRIANGULAR)
     return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol: only uplo = PLA_LOWER_TRIANGULAR is implemented");
@@ -164,6 +165,7 @@ int PLA_Chol(int uplo, PLA_Obj A) {
 // reference's simple variant returns 0 and leaves the block it failed on
 // factored as far as it got).
 int PLA_Chol_simple(int nb_alg, PLA_Obj A) {
+  if (is_single(A)) return on_promoted({&A}, {true}, [&] { return PLA_Chol_simple(nb_alg, A); });
   if (!valid(A)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Chol_simple: invalid A");
   if (A->base->datatype != MPI_DOUBLE && A->base->datatype != MPI_DOUBLE_COMPLEX)
     return fail(PLA_ERR_UNSUPPORTED, "PLA_Chol_simple: only MPI_DOUBLE and MPI_DOUBLE_COMPLEX matrices are supported");
@@ -185,6 +187,8 @@ int PLA_Chol_simple(int nb_alg, PLA_Obj A) {
 // Returns the PLA_Chol failure index (and leaves B untouched) if A is not
 // positive definite.
 int PLA_Pos_def_solve(PLA_Obj A, PLA_Obj B) {
+  if (is_single(A) || is_single(B))
+    return on_promoted({&A, &B}, {true, true}, [&] { return PLA_Pos_def_solve(A, B); });
   int rc = PLA_Chol(PLA_LOWER_TRIANGULAR, A);
   if (rc) return rc;
   if (!valid(B) || B->base->datatype != A->base->datatype)

@@ -17,6 +17,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <functional>
+#include <initializer_list>
 #include <string>
 
 struct pla_base {
@@ -46,6 +47,10 @@ struct PLA_Template_struct {
   MPI_Comm comm_all, comm_row, comm_col;
   int nb, zero_or_one, me, myrow, mycol, nprows, npcols, times_used;
 };
+
+// A new zero-filled object with like's object type, shape and owners and the
+// given datatype (pla_obj.cpp).
+extern "C" int pla_obj_create_like(PLA_Obj like, MPI_Datatype datatype, PLA_Obj *new_obj);
 
 namespace pla {
 
@@ -116,7 +121,7 @@ int masked_streams(int k);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
-constexpr int PLA_NUM_WS = 51;
+constexpr int PLA_NUM_WS = 54;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
 // 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-48 complex
@@ -127,7 +132,8 @@ enum {
   WS_GM_V = 29, WS_GM_W, WS_GM_H, WS_GM_T, WS_GM_Z,
   WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R,
   WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO, WS_ZEMBED_P,
-  WS_UTIL = 49, WS_UTIL_NORM
+  WS_UTIL = 49, WS_UTIL_NORM,
+  WS_ZOPA = 51, WS_ZOPB, WS_ZTRI  // complex BLAS3 operands (pla_zblas.cpp)
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
@@ -217,6 +223,43 @@ int zchol_factor(hipStream_t s, int n, void *a, long lda, int nb, int *info_out)
 int zchol_solve(hipStream_t s, int n, int nrhs, const void *l, long lda, void *b, long ldb);
 int zlu_factor(hipStream_t s, int m, int n, void *a, long lda, int *piv, int nb);
 int zlu_solve(hipStream_t s, int n, int nrhs, const void *lu, long lda, const int *piv, void *b, long ldb);
+// complex C -= A B (A m x k, B k x n) on the real embedding, embedded operand
+// in workspace slot ws
+int zgemm_minus(hipStream_t s, int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
+                long ldc, int ws);
+// complex B <- inv(T) B, T n x n lower or upper (not transposed)
+int ztrsm_left_n(hipStream_t s, bool lower, bool unit, int n, int nrhs, const void *T, long ldt, void *B, long ldb,
+                 int ws);
+// ---- datatype conversions and complex operand kernels (pla_convert.hip) ----
+// dst = src converted (MPI_FLOAT <-> MPI_DOUBLE, MPI_COMPLEX <-> MPI_DOUBLE_COMPLEX), m x n elements
+int k_convert2d(hipStream_t s, int m, int n, const void *src, long lds_, MPI_Datatype sdt, void *dst, long ldd,
+                MPI_Datatype ddt);
+// complex B (mo x no) = alpha op(A): op 0 = A, 1 = A^T, 2 = A^H, 3 = conj(A)
+int k_zop(hipStream_t s, int op, int mo, int no, double are, double aim, const void *A, long lda, void *B, long ldb);
+// complex C = beta C (beta = 0 clears C)
+int k_zscale2d(hipStream_t s, int m, int n, double bre, double bim, void *C, long ldc);
+// the uplo triangle (diagonal included) of the complex n x n C += P
+int k_ztri_add(hipStream_t s, int n, bool lower, const void *P, long ldp, void *C, long ldc);
+// ---- single-precision objects (pla_single.cpp) ----
+// True if obj is MPI_FLOAT or MPI_COMPLEX.
+bool is_single(PLA_Obj obj);
+// A double-precision copy of obj's view (MPI_FLOAT -> MPI_DOUBLE, MPI_COMPLEX
+// -> MPI_DOUBLE_COMPLEX, same object type and shape) and the rounding of such a
+// copy back into the original view.
+int single_promote(PLA_Obj obj, PLA_Obj *wide);
+int single_demote(PLA_Obj wide, PLA_Obj obj);
+// Complex scalar of a 1 x 1 object (real types give im = 0).
+int scalar_value_z(PLA_Obj o, double *re, double *im);
+// Runs call() with every single-precision object among *objs replaced by a
+// double-precision copy (the pointers are swapped for the call and restored);
+// the copies whose `out` flag is set are rounded back when call() returns a
+// non-negative value.  Null pointers and other datatypes are left alone.
+int on_promoted(std::initializer_list<PLA_Obj *> objs, std::initializer_list<bool> out,
+                const std::function<int()> &call);
+// ---- MPI_DOUBLE_COMPLEX BLAS3 (pla_zblas.cpp) ----
+int zblas_gemm(int transa, int transb, PLA_Obj alpha, PLA_Obj A, PLA_Obj B, PLA_Obj beta, PLA_Obj C);
+int zblas_trsm(int side, int uplo, int trans, int diag, PLA_Obj alpha, PLA_Obj A, PLA_Obj B);
+int zblas_syrk(int uplo, int trans, PLA_Obj alpha, PLA_Obj A, PLA_Obj beta, PLA_Obj C);
 void api_forget(PLA_Obj obj);  // drop a freed object from the open-object list
 void api_release();            // free the staging buffers (PLA_Finalize)
 int k_copy2d(hipStream_t s, int m, int n, const double *S, long lds_, double *D, long ldd,

@@ -197,17 +197,25 @@ static int lu_entry(PLA_Obj A, PLA_Obj pivots, bool swap_left) {
   return written(pivots);
 }
 
-int PLA_LU(PLA_Obj A, PLA_Obj pivots) { return lu_entry(A, pivots, true); }
+int PLA_LU(PLA_Obj A, PLA_Obj pivots) {
+  if (is_single(A)) return on_promoted({&A}, {true}, [&] { return PLA_LU(A, pivots); });
+  return lu_entry(A, pivots, true);
+}
 
 // SOLVERS/LINPACK/PLA_LU_linpack.c:13: the same U, pivots and trailing
 // columns as PLA_LU (on the n x (n+1) appended system [A b] the last column
 // ends as inv(L) P b); the multipliers of each panel stay in the row order of
 // their own step.
-int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots) { return lu_entry(A, pivots, false); }
+int PLA_LU_linpack(PLA_Obj A, PLA_Obj pivots) {
+  if (is_single(A)) return on_promoted({&A}, {true}, [&] { return PLA_LU_linpack(A, pivots); });
+  return lu_entry(A, pivots, false);
+}
 
 // Factor A (PLA_LU), then solve A X = B in place: interchanges on B, then the
 // unit-lower and upper triangular solves (PLA_General.c:80-111).
 int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
+  if (is_single(A) || is_single(B))
+    return on_promoted({&A, &B}, {true, true}, [&] { return PLA_General_solve(A, ipiv, B); });
   int rc = PLA_LU(A, ipiv);
   if (rc) return rc;
   if (!valid(B) || B->base->datatype != A->base->datatype)

@@ -208,6 +208,12 @@ using namespace pla;
 
 extern "C" {
 
+int pla_obj_create_like(PLA_Obj like, MPI_Datatype datatype, PLA_Obj *new_obj) {
+  if (!valid(like)) return fail(PLA_ERR_INVALID_OBJECT, "invalid object");
+  return new_object(like->objtype, datatype, like->length, like->width, like->base->templ, like->owner_row,
+                    like->owner_col, new_obj);
+}
+
 int PLA_Matrix_create(MPI_Datatype datatype, int global_length, int global_width,
                       PLA_Template templ, int global_align_row, int global_align_col,
                       PLA_Obj *new_obj) {
@@ -389,9 +395,12 @@ int PLA_Obj_set_local_contents(int trans, int rows_in_buf, int cols_in_buf, void
 
 int PLA_Obj_fill_drand48(PLA_Obj obj, long seed, int64_t skip) {
   if (!valid(obj)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Obj_fill_drand48: invalid obj");
+  // MPI_FLOAT / MPI_COMPLEX: the same draws, each rounded to single as the
+  // assignment buff_A[...] = drand48() * 2.0 - 1.0 does (TEST/create_problem.c:35-43)
+  if (is_single(obj)) return on_promoted({&obj}, {true}, [&] { return PLA_Obj_fill_drand48(obj, seed, skip); });
   const bool cplx = obj->base->datatype == MPI_DOUBLE_COMPLEX;
   if (obj->base->datatype != MPI_DOUBLE && !cplx)
-    return fail(PLA_ERR_UNSUPPORTED, "PLA_Obj_fill_drand48: only MPI_DOUBLE and MPI_DOUBLE_COMPLEX objects");
+    return fail(PLA_ERR_UNSUPPORTED, "PLA_Obj_fill_drand48: only MPI_DOUBLE, MPI_FLOAT and complex objects");
   if (skip < 0) return fail(PLA_ERR_INVALID_PARAM, "PLA_Obj_fill_drand48: negative skip");
   void *p;
   int rc = dev_ptr(obj, &p);

@@ -255,6 +255,8 @@ using namespace pla;
 extern "C" {
 
 int PLA_Axpy(PLA_Obj alpha, PLA_Obj x, PLA_Obj y) {
+  if (valid(y) && y->base->datatype == MPI_COMPLEX)
+    return on_promoted({&alpha, &x, &y}, {false, false, true}, [&] { return PLA_Axpy(alpha, x, y); });
   if (!valid(x) || !valid(y)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Axpy: invalid object");
   const MPI_Datatype dt = y->base->datatype;
   if (!real_or_complex(dt))
@@ -285,6 +287,8 @@ int PLA_Axpy(PLA_Obj alpha, PLA_Obj x, PLA_Obj y) {
 }
 
 int PLA_Nrm2(PLA_Obj x, PLA_Obj alpha) {
+  if (valid(x) && x->base->datatype == MPI_COMPLEX)
+    return on_promoted({&x, &alpha}, {false, true}, [&] { return PLA_Nrm2(x, alpha); });
   if (!valid(x) || !valid(alpha)) return fail(PLA_ERR_INVALID_OBJECT, "PLA_Nrm2: invalid object");
   const MPI_Datatype dt = x->base->datatype;
   if (!real_or_complex(dt))
@@ -329,6 +333,9 @@ int PLA_Symv(int uplo, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Ob
 }
 
 int PLA_Hemv(int uplo, PLA_Obj alpha, PLA_Obj A, PLA_Obj x, PLA_Obj beta, PLA_Obj y) {
+  if (valid(A) && A->base->datatype == MPI_COMPLEX)
+    return on_promoted({&alpha, &A, &x, &beta, &y}, {false, false, false, false, true},
+                       [&] { return PLA_Hemv(uplo, alpha, A, x, beta, y); });
   return symv_common("PLA_Hemv", true, uplo, alpha, A, x, beta, y);
 }
 

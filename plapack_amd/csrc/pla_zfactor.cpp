@@ -128,6 +128,16 @@ int zpanel_rec(hipStream_t s, zc *panel, int mp, int pw, long lda, int *piv, int
 
 }  // namespace
 
+int zgemm_minus(hipStream_t s, int m, int n, int k, const void *A, long lda, const void *B, long ldb, void *C,
+                long ldc, int ws) {
+  return zgemm_sub(s, m, n, k, (const zc *)A, lda, (const zc *)B, ldb, (zc *)C, ldc, nullptr, ws);
+}
+
+int ztrsm_left_n(hipStream_t s, bool lower, bool unit, int n, int nrhs, const void *T, long ldt, void *B, long ldb,
+                 int ws) {
+  return ztrsm_left(s, lower, unit, n, nrhs, (const zc *)T, ldt, (zc *)B, ldb, ws);
+}
+
 // Blocked right-looking LU with one step of look-ahead, as lu_factor: the
 // panel of step k+1 is factored on the look-ahead stream (its GEMMs on their
 // own operand workspace) while the trailing stream finishes step k's update;

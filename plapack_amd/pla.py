@@ -24,6 +24,7 @@ MPI_INT = 0x4C000405
 MPI_FLOAT = 0x4C00040A
 MPI_DOUBLE = 0x4C00080B
 MPI_DOUBLE_COMPLEX = 1275072546
+MPI_COMPLEX = 1275070494
 
 # constants (INCLUDE/PLA_define.h:11-102)
 PLA_MVECTOR = 12
@@ -305,7 +306,7 @@ def datatype(o):
 
 def _np_dtype(dt):
     return {MPI_DOUBLE: np.float64, MPI_INT: np.int32, MPI_FLOAT: np.float32,
-            MPI_DOUBLE_COMPLEX: np.complex128}[dt]
+            MPI_DOUBLE_COMPLEX: np.complex128, MPI_COMPLEX: np.complex64}[dt]
 
 
 def set_contents(o, arr):
