@@ -277,6 +277,14 @@ def run_cholesky(pla, s, n_plain, n_fp8, deltas, steps):
         B, X = s.matrix(n, nrhs), s.matrix(n, nrhs)
         pla.call("PLA_Obj_fill_drand48", B, 1, 0)
         res = {"n": n, "rhs": nrhs, "tile": 256, "runs": []}
+        # warmup (untimed): the library's grow-only workspaces are allocated
+        # by the first call
+        if deltas:
+            pla.call("PLA_Copy", Abase, A)
+            v = np.array([deltas[0]])
+            pla.call("PLA_Shift", A, pla.MPI_DOUBLE, v.ctypes.data_as(ctypes.c_void_p))
+            it, be = ctypes.c_int(), ctypes.c_double()
+            pla.call("PLA_Chol_mixed", A, B, X, pla.PLA_MIXED_FP8, 1e-14, 50, ctypes.byref(it), ctypes.byref(be))
         for delta in deltas:
             pla.call("PLA_Copy", Abase, A)
             v = np.array([delta])

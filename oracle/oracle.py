@@ -441,22 +441,27 @@ def e4m3_decode(b):
 
 
 def fp8_quantize_tiles(p, tile=256, width=256):
-    """Quantization of an R x b panel (b <= width) in tiles of `tile` rows:
-    per tile the smallest e with max|tile| <= 448 * 2^e, values rounded to
-    FP32 then to E4M3.  Returns (bytes R x width, row-major; exponents)."""
+    """Quantization of an R x b panel (b <= width, width a multiple of
+    `tile`) in tile x tile blocks: per block the smallest e with
+    max|block| <= 448 * 2^e, clamped to [-127, 127] (the range of an E8M0
+    scale), values rounded to FP32 then to E4M3.  Returns (bytes R x width,
+    row-major; exponents in row-tile-major order, width / tile per row tile)."""
     p = np.asarray(p, dtype=np.float64)
     R, b = p.shape
     q = np.zeros((R, width), dtype=np.uint8)
     exps = []
     for t0 in range(0, R, tile):
-        blk = p[t0:t0 + tile]
-        amax = np.abs(blk).max() if blk.size else 0.0
-        if amax > 0:
-            f, e2 = np.frexp(amax / E4M3_MAX)
-            ex = int(e2 - 1 if f == 0.5 else e2)
-        else:
-            ex = 0
-        exps.append(ex)
-        scaled = (blk * 2.0 ** -ex).astype(np.float32).astype(np.float64)
-        q[t0:t0 + tile, :b] = e4m3_encode(e4m3_round(scaled))
+        for c0 in range(0, width, tile):
+            blk = p[t0:t0 + tile, c0:min(b, c0 + tile)]
+            amax = np.abs(blk).max() if blk.size else 0.0
+            if amax > 0:
+                f, e2 = np.frexp(amax / E4M3_MAX)
+                ex = int(e2 - 1 if f == 0.5 else e2)
+            else:
+                ex = 0
+            ex = min(127, max(-127, ex))
+            exps.append(ex)
+            if blk.size:
+                scaled = (blk * 2.0 ** -ex).astype(np.float32).astype(np.float64)
+                q[t0:t0 + tile, c0:c0 + blk.shape[1]] = e4m3_encode(e4m3_round(scaled))
     return q, np.array(exps, dtype=np.int32)

@@ -176,14 +176,14 @@ int lu_streams_begin(hipStream_t main, int m, hipStream_t *sP, hipStream_t *sT, 
 
 int masked_streams(int k) { return make_masked_streams(g_ctx, k); }
 
-int chol_streams_begin(hipStream_t main, hipStream_t *sP, hipStream_t *sT) {
+int chol_streams_begin(hipStream_t main, hipStream_t *sP, hipStream_t *sT, bool allow_mask) {
   context &c = g_ctx;
   if (!c.lookahead) {
     *sP = *sT = main;
     return PLA_SUCCESS;
   }
   if (!g_events[EV_JOIN0]) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
-  if (c.chol_panel_cus > 0 && c.masked_cus == c.chol_panel_cus && c.stream_panel) {
+  if (allow_mask && c.chol_panel_cus > 0 && c.masked_cus == c.chol_panel_cus && c.stream_panel) {
     *sP = c.stream_panel;
     *sT = c.stream_trail;
   } else {

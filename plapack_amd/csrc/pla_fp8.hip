@@ -1,11 +1,12 @@
-// pla_fp8.hip -- FP8 (OCP E4M3) panel quantization and the FP8 MFMA SYRK
-// trailing update of the mixed-precision Cholesky.
+// pla_fp8.hip -- FP8 (OCP E4M3) panel quantization and the FP8 MFMA trailing
+// update of the mixed-precision Cholesky.
 //
 // No counterpart in the reference: PLA_Chol_lower_large's FP64 SYRK
 // (SOLVERS/Cholesky/PLA_Chol.c:207-210, BLAS3/PLA_Syrk_perform_local_part.c)
 // becomes lower(C) -= L21 L21^T with L21 stored as E4M3 bytes and one
-// power-of-two scale per 256 x 256 tile; products are exact in FP32 and
-// accumulate in FP32 on gfx950's 16x16x128 FP8 MFMA.
+// power-of-two scale per 256 x 256 tile.  The scales enter gfx950's scaled
+// 16x16x128 FP8 MFMA as its E8M0 operands, so products of tiles with
+// different scales are exact in FP32 and accumulate in FP32 in one pass.
 #include "pla_internal.hpp"
 
 namespace {
@@ -13,22 +14,29 @@ namespace {
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int QT = 256;       // rows per scale tile (and bytes per quantized row)
+constexpr int QT = 256;        // rows and columns of one scale tile
+constexpr int KT_MAX = 2;      // scale tiles along k: panels of up to 512 columns
 constexpr int E4M3_MAX = 448;  // largest finite E4M3 value
+constexpr int EXP_LIM = 127;   // an E8M0 scale spans 2^-127 .. 2^127
 
-// One workgroup per 256-row tile of the panel P (R x b FP64, column-major):
-// exps[t] = the smallest e with max|tile| <= 448 * 2^e, and row r of Lq
-// (QT bytes, row-major) = E4M3(P[r][k] * 2^-e) for k < b, zero for k >= b.
-// The value is rounded to FP32 first, then to E4M3 (round to nearest even).
+// One workgroup per 256 x 256 tile of the R x b FP64 panel P (column-major;
+// blockIdx.x: row tile t, blockIdx.y: k tile kt, KT = gridDim.y):
+// exps[t * KT + kt] = the smallest e with max|tile| <= 448 * 2^e, clamped to
+// [-127, 127]; Lq row r (kq bytes, row-major) holds E4M3(P[r][k] * 2^-e) in
+// bytes k of the tile, zero for k >= b.  The value is rounded to FP32 first,
+// then to E4M3 (round to nearest even).
 __global__ __launch_bounds__(256) void fp8_quantize_kernel(int R, int b, const double *__restrict__ P, long ldp,
-                                                           unsigned char *__restrict__ Lq, int *__restrict__ exps) {
+                                                           unsigned char *__restrict__ Lq, int kq,
+                                                           int *__restrict__ exps) {
   __shared__ double red[4];
-  const int t = blockIdx.x, r0 = t * QT, rows = min(QT, R - r0);
+  const int t = blockIdx.x, kt = blockIdx.y, KT = gridDim.y;
+  const int r0 = t * QT, rows = min(QT, R - r0);
+  const int c0 = kt * QT, cols = max(0, min(QT, b - c0));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double m = 0.0;
-  for (int e = tid; e < rows * b; e += 256) {
+  for (int e = tid; e < rows * cols; e += 256) {
     const int j = e / rows, i = e - j * rows;
-    m = fmax(m, fabs(P[(long)j * ldp + r0 + i]));
+    m = fmax(m, fabs(P[(long)(c0 + j) * ldp + r0 + i]));
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
@@ -40,16 +48,17 @@ __global__ __launch_bounds__(256) void fp8_quantize_kernel(int R, int b, const d
     int e2;
     const double f = frexp(amax / E4M3_MAX, &e2);  // amax / 448 = f 2^e2, f in [0.5, 1)
     ex = (f == 0.5) ? e2 - 1 : e2;
+    ex = min(EXP_LIM, max(-EXP_LIM, ex));
   }
   const double sc = ldexp(1.0, -ex);
-  if (tid == 0) exps[t] = ex;
+  if (tid == 0) exps[t * KT + kt] = ex;
   if (tid < rows) {
     const int r = r0 + tid;
-    int *dst = (int *)(Lq + (size_t)r * QT);
+    int *dst = (int *)(Lq + (size_t)r * kq + c0);
     for (int k = 0; k < QT; k += 4) {
       float v[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) v[u] = (k + u < b) ? (float)(P[(long)(k + u) * ldp + r] * sc) : 0.0f;
+      for (int u = 0; u < 4; u++) v[u] = (k + u < cols) ? (float)(P[(long)(c0 + k + u) * ldp + r] * sc) : 0.0f;
       int w = 0;
       w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
       w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
@@ -58,57 +67,102 @@ __global__ __launch_bounds__(256) void fp8_quantize_kernel(int R, int b, const d
   }
 }
 
-// lower(C) -= 2^(e_i + e_j) Lq_i Lq_j^T for the R x R lower triangle of C
-// (FP32, column-major), Lq R x 256 E4M3 bytes (row-major), exps per 256 rows.
-// 128 x 128 tiles (4 waves of 64 x 64), the full k extent staged in LDS.
-// MFMA operands: lane l holds row l%16, k bytes [32 (l/16), 32 (l/16) + 32)
-// of the 128-wide k block; result lane l, register r holds row 4(l/16) + r,
-// column l%16.  The column-side rows are fed first, so lane l ends up with
-// C(i0 + l%16, j0 + 4(l/16) + r): 16 consecutive rows of one column.
-constexpr int ST = 128;             // output tile
-constexpr int QLDS = QT + 16;       // LDS row stride (bytes)
+// C(i, j) -= sum_k 2^(er(i, k) + ec(j, k)) Lr(i, k) Lc(j, k) over the M x N
+// FP32 block C (column-major): TRI, the lower triangle of an M x M block
+// whose row and column operands are the same rows; else every element.
+// Lr, Lc: rows of kq E4M3 bytes (row-major), the first row of each at a
+// 256-row scale-tile boundary; er, ec: their exponents, KT per row tile.
+//
+// 128 x 128 output tiles, 4 waves of 64 x 64 (4 x 4 MFMA blocks).  k runs in
+// chunks of 128 bytes (one MFMA k block) through double-buffered LDS: the
+// next chunk's global loads are in flight while the current one is
+// multiplied, then stored to the other buffer, one barrier per chunk.  A
+// chunk lies inside one 256-column scale tile and a 128-row output tile
+// inside one 256-row scale tile, so each MFMA takes two wave-uniform E8M0
+// scales.  MFMA operands: lane l holds row l%16, bytes [32 (l/16),
+// 32 (l/16) + 32) of the chunk; result lane l, register r holds row
+// 4(l/16) + r, column l%16 of the first operand's rows x the second's.  The
+// column side goes first, so lane l ends up with C(i0 + l%16,
+// j0 + 4(l/16) + r): 16 consecutive rows of one column per 16 lanes.
+constexpr int ST = 128;         // output tile
+constexpr int KC = 128;         // k bytes per chunk
+constexpr int LROW = KC + 16;   // LDS row stride (bytes)
 
-__global__ __launch_bounds__(256, 2) void fp8_syrk_kernel(int R, const unsigned char *__restrict__ Lq,
-                                                         const int *__restrict__ exps, float *__restrict__ C,
-                                                         long ldc, const int *__restrict__ abort_flag) {
-  if (abort_flag && *abort_flag) return;
-  // lower tile (ti, tj), ti >= tj, from the linear block index
-  const long bid = blockIdx.x;
-  int ti = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
-  while ((long)(ti + 1) * (ti + 2) / 2 <= bid) ti++;
-  while ((long)ti * (ti + 1) / 2 > bid) ti--;
-  const int tj = (int)(bid - (long)ti * (ti + 1) / 2);
-  const int i0 = ti * ST, j0 = tj * ST;
-  __shared__ __align__(16) unsigned char As[ST * QLDS];  // row side
-  __shared__ __align__(16) unsigned char Bs[ST * QLDS];  // column side
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int e = tid; e < ST * (QT / 16); e += 256) {
-    const int r = e / (QT / 16), c16 = e - r * (QT / 16);
-    int4 va = {0, 0, 0, 0}, vb = {0, 0, 0, 0};
-    if (i0 + r < R) va = *(const int4 *)(Lq + (size_t)(i0 + r) * QT + c16 * 16);
-    if (j0 + r < R) vb = *(const int4 *)(Lq + (size_t)(j0 + r) * QT + c16 * 16);
-    *(int4 *)(As + r * QLDS + c16 * 16) = va;
-    *(int4 *)(Bs + r * QLDS + c16 * 16) = vb;
+struct fp8_upd_args {
+  int M, N, nks, kq, KT, tiles_m;
+  const unsigned char *Lr, *Lc;
+  const int *er, *ec;
+  float *C;
+  long ldc;
+  const int *abort_flag;
+};
+
+template <bool TRI>
+__global__ __launch_bounds__(256, 2) void fp8_update_kernel(fp8_upd_args a) {
+  if (a.abort_flag && *a.abort_flag) return;
+  int ti, tj;
+  if (TRI) {  // lower tile (ti, tj), ti >= tj, from the linear block index
+    const long bid = blockIdx.x;
+    ti = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while ((long)(ti + 1) * (ti + 2) / 2 <= bid) ti++;
+    while ((long)ti * (ti + 1) / 2 > bid) ti--;
+    tj = (int)(bid - (long)ti * (ti + 1) / 2);
+  } else {
+    ti = blockIdx.x % a.tiles_m;
+    tj = blockIdx.x / a.tiles_m;
   }
-  __syncthreads();
+  const int i0 = ti * ST, j0 = tj * ST;
+  __shared__ __align__(16) unsigned char As[2][ST * LROW];  // row side
+  __shared__ __align__(16) unsigned char Bs[2][ST * LROW];  // column side
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int *erow = a.er + (i0 / QT) * a.KT, *ecol = a.ec + (j0 / QT) * a.KT;
+
+  // 128 rows x 128 bytes per operand and chunk: 4 int4 per thread, 8 threads a row
+  int4 ra[4], rb[4];
+  auto gload = [&](int ks) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u, r = e >> 3, c = e & 7;
+      const int4 z = {0, 0, 0, 0};
+      ra[u] = (i0 + r < a.M) ? *(const int4 *)(a.Lr + (size_t)(i0 + r) * a.kq + ks * KC + c * 16) : z;
+      rb[u] = (j0 + r < a.N) ? *(const int4 *)(a.Lc + (size_t)(j0 + r) * a.kq + ks * KC + c * 16) : z;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u, r = e >> 3, c = e & 7;
+      *(int4 *)(As[buf] + r * LROW + c * 16) = ra[u];
+      *(int4 *)(Bs[buf] + r * LROW + c * 16) = rb[u];
+    }
+  };
+
   const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
   v4f acc[4][4];
 #pragma unroll
   for (int bi = 0; bi < 4; bi++)
 #pragma unroll
     for (int bj = 0; bj < 4; bj++) acc[bi][bj] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int ks = 0; ks < QT / 128; ks++) {
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int ks = 0; ks < a.nks; ks++) {
+    const bool more = ks + 1 < a.nks;
+    if (more) gload(ks + 1);
+    const int kt = (ks * KC) / QT;
+    const int scr = 127 + erow[kt], scc = 127 + ecol[kt];
+    const unsigned char *as = As[ks & 1], *bs = Bs[ks & 1];
     v8i af[4], bf[4];
 #pragma unroll
     for (int bi = 0; bi < 4; bi++) {
-      const unsigned char *p = As + (wm * 64 + bi * 16 + l16) * QLDS + ks * 128 + lq * 32;
+      const unsigned char *p = as + (wm * 64 + bi * 16 + l16) * LROW + lq * 32;
       const int4 x0 = *(const int4 *)p, x1 = *(const int4 *)(p + 16);
       af[bi] = v8i{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
     }
 #pragma unroll
     for (int bj = 0; bj < 4; bj++) {
-      const unsigned char *p = Bs + (wn * 64 + bj * 16 + l16) * QLDS + ks * 128 + lq * 32;
+      const unsigned char *p = bs + (wn * 64 + bj * 16 + l16) * LROW + lq * 32;
       const int4 x0 = *(const int4 *)p, x1 = *(const int4 *)(p + 16);
       bf[bj] = v8i{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
     }
@@ -116,26 +170,38 @@ __global__ __launch_bounds__(256, 2) void fp8_syrk_kernel(int R, const unsigned 
     for (int bi = 0; bi < 4; bi++)
 #pragma unroll
       for (int bj = 0; bj < 4; bj++)
-        // formats 0/0: E4M3 x E4M3; E8M0 scales 127 = 2^0 (tile scales are applied below)
+        // formats 0/0: E4M3 x E4M3; E8M0 scales 127 + e (2^e)
         acc[bi][bj] =
-            __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(bf[bj], af[bi], acc[bi][bj], 0, 0, 0, 127, 0, 127);
+            __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(bf[bj], af[bi], acc[bi][bj], 0, 0, 0, scc, 0, scr);
+    if (more) sstore((ks + 1) & 1);
+    __syncthreads();
   }
+  // C in two halves of 32 elements per lane: each half's loads are all issued
+  // before its stores, so they are in flight together
+  float *__restrict__ C = a.C;
 #pragma unroll
-  for (int bi = 0; bi < 4; bi++) {
-    const int i = i0 + wm * 64 + bi * 16 + l16;
-    if (i >= R) continue;
-    const float si = ldexpf(1.0f, exps[i / QT]);
+  for (int h = 0; h < 2; h++) {
+    float cv[2][4][4];
 #pragma unroll
-    for (int bj = 0; bj < 4; bj++)
+    for (int b2 = 0; b2 < 2; b2++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int j = j0 + wn * 64 + bj * 16 + lq * 4 + r;
-        if (j <= i) {
-          const float sc = si * ldexpf(1.0f, exps[j / QT]);
-          float *cp = C + (long)j * ldc + i;
-          *cp = fmaf(-sc, acc[bi][bj][r], *cp);
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = i0 + wm * 64 + (2 * h + b2) * 16 + l16, j = j0 + wn * 64 + bj * 16 + lq * 4 + r;
+          const bool in = i < a.M && j < a.N && (!TRI || j <= i);
+          cv[b2][bj][r] = in ? C[(long)j * a.ldc + i] : 0.0f;
         }
-      }
+#pragma unroll
+    for (int b2 = 0; b2 < 2; b2++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = i0 + wm * 64 + (2 * h + b2) * 16 + l16, j = j0 + wn * 64 + bj * 16 + lq * 4 + r;
+          if (i < a.M && j < a.N && (!TRI || j <= i))
+            C[(long)j * a.ldc + i] = cv[b2][bj][r] - acc[2 * h + b2][bj][r];
+        }
   }
 }
 
@@ -151,6 +217,56 @@ __global__ __launch_bounds__(256) void cvt_lower_shift_kernel(int n, const doubl
   }
 }
 
+// D(j, i) = S(i, j) for i > j, and D(i, j) = S(i, j) for i >= j when D is
+// not S: the full symmetric matrix whose lower triangle S holds (D == S: the
+// strict upper triangle of S receives the transpose of the strict lower one).
+// One workgroup per 64 x 64 tile (ti, tj), ti >= tj, through LDS (padded
+// rows: both passes are conflict-free).
+template <typename T>
+__global__ __launch_bounds__(256) void mirror_lower_kernel(int n, const T *S, long lds, T *D, long ldd) {
+  __shared__ T t[64][65];
+  const long bid = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+  while ((long)(ti + 1) * (ti + 2) / 2 <= bid) ti++;
+  while ((long)ti * (ti + 1) / 2 > bid) ti--;
+  const int tj = (int)(bid - (long)ti * (ti + 1) / 2);
+  const int i0 = ti * 64, j0 = tj * 64;
+  const bool copy = (S != D);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int c = e >> 6, r = e & 63, i = i0 + r, j = j0 + c;
+    if (i < n && i >= j) {
+      const T v = S[(long)j * lds + i];
+      t[c][r] = v;
+      if (copy) D[(long)j * ldd + i] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63, i = i0 + r, j = j0 + c;
+    if (i < n && i > j) D[(long)i * ldd + j] = t[c][r];
+  }
+}
+
+// *out = max_i A(i, i), one workgroup.
+__global__ __launch_bounds__(256) void max_diag_kernel(int n, const double *__restrict__ A, long lda,
+                                                       double *__restrict__ out) {
+  __shared__ double red[4];
+  double m = -HUGE_VAL;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, A[(long)i * lda + i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void eye_kernel(int n, double *__restrict__ T, long ldt) {
+  const int j = blockIdx.y;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) T[(long)j * ldt + i] = i == j ? 1.0 : 0.0;
+}
+
+long tri_blocks(long t) { return t * (t + 1) / 2; }
+
 }  // namespace
 
 namespace pla {
@@ -165,22 +281,80 @@ int k_cvt_lower_shift(hipStream_t s, int n, const double *A, long lda, double sh
 }
 
 int fp8_tile_rows() { return QT; }
+int fp8_row_bytes(int b) { return QT * ((b + QT - 1) / QT); }
 
 int k_fp8_quantize(hipStream_t s, int R, int b, const double *P, long ldp, unsigned char *Lq, int *exps) {
   if (R <= 0) return PLA_SUCCESS;
-  if (b > QT) return fail(PLA_ERR_INVALID_PARAM, "fp8 quantize: panel width %d > %d", b, QT);
-  hipLaunchKernelGGL(fp8_quantize_kernel, dim3((R + QT - 1) / QT), dim3(256), 0, s, R, b, P, ldp, Lq, exps);
+  if (b < 1 || b > KT_MAX * QT) return fail(PLA_ERR_INVALID_PARAM, "fp8 quantize: panel width %d", b);
+  const int kq = fp8_row_bytes(b);
+  hipLaunchKernelGGL(fp8_quantize_kernel, dim3((R + QT - 1) / QT, kq / QT), dim3(256), 0, s, R, b, P, ldp, Lq, kq,
+                     exps);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_fp8_update(hipStream_t s, bool tri, int M, int N, int b, const unsigned char *Lq, const int *exps, int ro,
+                 int co, float *C, long ldc, const int *abort_flag) {
+  if (M <= 0 || N <= 0) return PLA_SUCCESS;
+  if (b < 1 || b > KT_MAX * QT || ro % QT || co % QT || (tri && (M != N || ro != co)))
+    return fail(PLA_ERR_INVALID_PARAM, "fp8 update: b %d, rows from %d, columns from %d", b, ro, co);
+  const int kq = fp8_row_bytes(b), KT = kq / QT;
+  fp8_upd_args a;
+  a.M = M;
+  a.N = N;
+  a.nks = (b + KC - 1) / KC;
+  a.kq = kq;
+  a.KT = KT;
+  a.tiles_m = (M + ST - 1) / ST;
+  a.Lr = Lq + (size_t)ro * kq;
+  a.Lc = Lq + (size_t)co * kq;
+  a.er = exps + (ro / QT) * KT;
+  a.ec = exps + (co / QT) * KT;
+  a.C = C;
+  a.ldc = ldc;
+  a.abort_flag = abort_flag;
+  const long tn = (N + ST - 1) / ST;
+  const long nblk = tri ? tri_blocks(a.tiles_m) : a.tiles_m * tn;
+  if (nblk > (1L << 31) - 1) return fail(PLA_ERR_DIMENSIONS, "fp8 update: %d x %d", M, N);
+  if (tri) hipLaunchKernelGGL(fp8_update_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(fp8_update_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, s, a);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }
 
 int k_fp8_syrk_lower(hipStream_t s, int R, const unsigned char *Lq, const int *exps, float *C, long ldc,
                      const int *abort_flag) {
-  if (R <= 0) return PLA_SUCCESS;
-  const long t = (R + ST - 1) / ST;
-  const long nblk = t * (t + 1) / 2;
-  if (nblk > (1L << 31) - 1) return fail(PLA_ERR_DIMENSIONS, "fp8 syrk: %d rows", R);
-  hipLaunchKernelGGL(fp8_syrk_kernel, dim3((unsigned)nblk), dim3(256), 0, s, R, Lq, exps, C, ldc, abort_flag);
+  return k_fp8_update(s, true, R, R, QT, Lq, exps, 0, 0, C, ldc, abort_flag);
+}
+
+int k_mirror_lower_f32(hipStream_t s, int n, float *W, long ldw) {
+  if (n <= 1) return PLA_SUCCESS;
+  const long nblk = tri_blocks((n + 63) / 64);
+  if (nblk > (1L << 31) - 1) return fail(PLA_ERR_DIMENSIONS, "mirror: n = %d", n);
+  hipLaunchKernelGGL(mirror_lower_kernel<float>, dim3((unsigned)nblk), dim3(256), 0, s, n, W, ldw, W, ldw);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_sym_full(hipStream_t s, int n, const double *A, long lda, double *S, long lds) {
+  if (n <= 0) return PLA_SUCCESS;
+  const long nblk = tri_blocks((n + 63) / 64);
+  if (nblk > (1L << 31) - 1) return fail(PLA_ERR_DIMENSIONS, "sym_full: n = %d", n);
+  hipLaunchKernelGGL(mirror_lower_kernel<double>, dim3((unsigned)nblk), dim3(256), 0, s, n, A, lda, S, lds);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_eye(hipStream_t s, int n, double *T, long ldt) {
+  if (n <= 0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(eye_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, n, T, ldt);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
+int k_max_diag(hipStream_t s, int n, const double *A, long lda, double *out) {
+  if (n <= 0) return fail(PLA_ERR_DIMENSIONS, "max_diag: n = %d", n);
+  hipLaunchKernelGGL(max_diag_kernel, dim3(1), dim3(256), 0, s, n, A, lda, out);
   PLA_HIP(hipGetLastError());
   return PLA_SUCCESS;
 }

@@ -84,6 +84,13 @@ struct context {
   int nb_switch = 0;
   unsigned leaf_spin_limit = 400000u;  // bounded wait of a panel-leaf exchange (polls)
   unsigned trsv_spin_limit = 2000000u; // bounded wait of a triangular-solve block (polls)
+  // FP8 Cholesky (PLA_Chol_mixed): block size (256 or 512), whether its panel
+  // runs on the CU-masked Cholesky stream pair, and the first diagonal shift
+  // tried after an unshifted factorization fails, as 2^fp8_shift_exp times
+  // the largest diagonal element
+  int fp8_nb = 512;
+  int fp8_masked = 0;
+  int fp8_shift_exp = -4;
   // CU-masked stream pair of the LU look-ahead: the panel stream runs on
   // masked_cus CUs (masked_cus / 8 per XCD), the trailing stream on the rest
   hipStream_t stream_panel = nullptr, stream_trail = nullptr;
@@ -115,13 +122,13 @@ int lu_streams_end(hipStream_t main, hipStream_t sP, hipStream_t sT);
 // trailing updates.  With PLA_PARAM_CHOL_PANEL_CUS set they are the CU-masked
 // pair (created when the parameter is set, never here); otherwise *sP is the
 // look-ahead stream and *sT is `main`.  End with lu_streams_end.
-int chol_streams_begin(hipStream_t main, hipStream_t *sP, hipStream_t *sT);
+int chol_streams_begin(hipStream_t main, hipStream_t *sP, hipStream_t *sT, bool allow_mask = true);
 // Creates (or re-creates) the CU-masked stream pair with k panel CUs.
 int masked_streams(int k);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
-constexpr int PLA_NUM_WS = 54;
+constexpr int PLA_NUM_WS = 57;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
 // 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-48 complex
@@ -133,7 +140,9 @@ enum {
   WS_CH_W = 34, WS_CH_P64, WS_CH_LQ, WS_CH_EXP, WS_CH_INFO, WS_CH_DINVL, WS_CH_DINVLT, WS_CH_D, WS_CH_R,
   WS_API_0 = 43, WS_API_1, WS_ZEMBED, WS_ZCT, WS_ZINFO, WS_ZEMBED_P,
   WS_UTIL = 49, WS_UTIL_NORM,
-  WS_ZOPA = 51, WS_ZOPB, WS_ZTRI  // complex BLAS3 operands (pla_zblas.cpp)
+  WS_ZOPA = 51, WS_ZOPB, WS_ZTRI,  // complex BLAS3 operands (pla_zblas.cpp)
+  WS_CH_FULL = 54,                  // full symmetric FP64 copy of A (PLA_Chol_mixed refinement)
+  WS_CH_LINV, WS_CH_OUT             // inverse of L11 and the solved panel (PLA_Chol_mixed)
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
@@ -383,9 +392,19 @@ int k_scale_cols(hipStream_t s, int n, int nrhs, double *W, long ldw, const doub
 int k_inv_diag_f32_op(hipStream_t s, bool lower, bool trans, bool unit, int n, const float *A, long lda,
                       double *Dinv);
 int fp8_tile_rows();
+int fp8_row_bytes(int b);  // bytes of one quantized row of a b-column panel
+// exps: one exponent per 256 x 256 tile, row-tile major (fp8_row_bytes(b) / 256 per row tile)
 int k_fp8_quantize(hipStream_t s, int R, int b, const double *P, long ldp, unsigned char *Lq, int *exps);
+// C -= Lq[ro..] Lq[co..]^T over k < b (tri: lower triangle of an M x M block, ro == co)
+int k_fp8_update(hipStream_t s, bool tri, int M, int N, int b, const unsigned char *Lq, const int *exps, int ro,
+                 int co, float *C, long ldc, const int *abort_flag);
 int k_fp8_syrk_lower(hipStream_t s, int R, const unsigned char *Lq, const int *exps, float *C, long ldc,
                      const int *abort_flag);
+int k_mirror_lower_f32(hipStream_t s, int n, float *W, long ldw);
+// S (n x n) <- the full symmetric matrix whose lower triangle A holds
+int k_sym_full(hipStream_t s, int n, const double *A, long lda, double *S, long lds);
+int k_max_diag(hipStream_t s, int n, const double *A, long lda, double *out);
+int k_eye(hipStream_t s, int n, double *T, long ldt);  // T <- I (n x n)
 int k_cvt_lower_shift(hipStream_t s, int n, const double *A, long lda, double shift, float *W, long ldw);
 
 // Refinement of X for A X = B with a low-precision factorization M:

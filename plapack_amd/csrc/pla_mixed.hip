@@ -333,8 +333,45 @@ __device__ inline unsigned long long ld_dev64(const unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Wait until block `w` of this launch has published (its ready word equals
+// want): one thread polls the single word, so waiting blocks do not stream
+// the solution words through the memory system while the block is being
+// computed.  Returns false, uniformly, when the bounded wait gives up.
+__device__ inline bool wait_ready(const unsigned long long *__restrict__ w, unsigned long long want,
+                                  unsigned spin_limit, int *__restrict__ err, int *s_flag) {
+  if (threadIdx.x == 0) {
+    unsigned it = 0;
+    int f = 0;
+    while (ld_dev64(w) != want) {
+      ++it;
+      if (it >= spin_limit || ((it & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+        f = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    *s_flag = f;
+  }
+  __syncthreads();
+  return *s_flag == 0;
+}
+
+// After the solution words of block blk are stored by every thread: release
+// them, then the ready word.
+__device__ inline void publish_ready(unsigned long long *__restrict__ w, unsigned long long want) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) st_dev64(w, want);
+}
+
 // Solve M X = X in place (n x nrhs, nrhs <= NR <= 16).  xs: exchange words
-// [nblk][TB][MAXRHS][2]; tag unique per launch.
+// [nblk][TB][MAXRHS][2], then one ready word per block; tag unique per launch.
+// A block's solution words carry the tag; readers re-read them until every
+// tag matches.  From NR = 4 on a ready word (tag | 1) also follows the words
+// behind a fence, and readers wait for it before reading the words, so the
+// blocks that wait do not stream the words through the memory system (at
+// n = 49152: 8 columns 8.0 -> 5.5 ms, 16 columns 15.3 -> 10.1 ms; with 1 or
+// 2 columns the fence and the extra round trip cost more than they save).
 //
 // Thread t owns row t%64 of its block and the columns c = t/64 (mod 4) of
 // every tile; its partial sums stay in registers across tiles and the four
@@ -351,7 +388,10 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
   __shared__ double tile[TB][TB + 1];
   __shared__ double xb[TB][NR];
   __shared__ double red[4][TB][NR];
+  __shared__ int s_flag;
+  constexpr bool READY = NR >= 4;  // ready words: measured faster from 4 columns on
   const int nblk = (n + TB - 1) / TB;
+  unsigned long long *const rdy = xs + (size_t)nblk * TB * MAXRHS * 2;
   const int b = blockIdx.x;
   const int blk = LOWEFF ? b : nblk - 1 - b;
   const int r0 = blk * TB, bs = min(TB, n - r0);
@@ -394,6 +434,10 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
       tile[i][c] = (double)treg[k];
     }
     if (s + 1 < b) load_tile(s + 1);  // next tile in flight during this step
+    if (READY && !wait_ready(rdy + iblk, tag | 1, spin_limit, err, &s_flag)) {
+      failed = true;
+      break;
+    }
     unsigned it = 0;
     for (;;) {
       int ok = 1;
@@ -445,6 +489,7 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
     st_dev64(dst + ((size_t)i * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
     st_dev64(dst + ((size_t)i * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
   }
+  if (READY) publish_ready(rdy + blk, tag | 1);
 }
 
 // Untransposed solves (M = A, column-major): the chain of dependent blocks is
@@ -463,7 +508,10 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
   constexpr int RPT = (TB * NR + 255) / 256;  // right-hand-side entries per thread
   __shared__ double xb[2][TB][NR];
   __shared__ double red[4][TB][NR];
+  __shared__ int s_flag;
+  constexpr bool READY = NR >= 4;  // ready words: measured faster from 4 columns on
   const int nblk = (n + TB - 1) / TB;
+  unsigned long long *const rdy = xs + (size_t)nblk * TB * MAXRHS * 2;
   const int b = blockIdx.x;
   const int blk = LOWER ? b : nblk - 1 - b;
   const int r0 = blk * TB, bs = min(TB, n - r0);
@@ -506,6 +554,10 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
     const int cs = min(TB, n - iblk * TB);
     const unsigned long long *src = xs + (size_t)iblk * TB * MAXRHS * 2;
     double (*x)[NR] = xb[s & 1];
+    if (READY && !wait_ready(rdy + iblk, tag | 1, spin_limit, err, &s_flag)) {
+      failed = true;
+      break;
+    }
     unsigned it = 0;
     for (;;) {
       int ok = 1;
@@ -573,6 +625,7 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
     st_dev64(dst + ((size_t)i * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
     st_dev64(dst + ((size_t)i * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
   }
+  if (READY) publish_ready(rdy + blk, tag | 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -791,8 +844,9 @@ static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n,
   if (n <= 0 || nrhs <= 0) return PLA_SUCCESS;
   if (nrhs > MAXRHS) return fail(PLA_ERR_INVALID_PARAM, "trsv: %d right-hand sides (max %d)", nrhs, MAXRHS);
   const int nblk = (n + TB - 1) / TB;
-  unsigned long long *xs =
-      (unsigned long long *)workspace(sizeof(unsigned long long) * nblk * TB * MAXRHS * 2, 5, true);
+  // solution words, then one ready word per block
+  const size_t xs_words = (size_t)nblk * TB * MAXRHS * 2 + nblk;
+  unsigned long long *xs = (unsigned long long *)workspace(sizeof(unsigned long long) * xs_words, 5, true);
   if (!xs) return PLA_ERR_NO_MEMORY;
   int *err = leaf_error_word();
   if (!Dinv) {
@@ -812,14 +866,16 @@ static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n,
     PLA_HIP(hipGetLastError());
   }
   if (g_trsv_tag > 0xF0000000u) {
-    PLA_HIP(hipMemsetAsync(xs, 0, sizeof(unsigned long long) * nblk * TB * MAXRHS * 2, s));
+    PLA_HIP(hipMemsetAsync(xs, 0, sizeof(unsigned long long) * xs_words, s));
     g_trsv_tag = 1;
   }
   const unsigned tag = g_trsv_tag++;
   int rc;
   timing_begin(s, 6);
   if (nrhs <= 1) rc = trsv_launch<T, 1>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else if (nrhs <= 2) rc = trsv_launch<T, 2>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
   else if (nrhs <= 4) rc = trsv_launch<T, 4>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else if (nrhs <= 8) rc = trsv_launch<T, 8>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
   else rc = trsv_launch<T, MAXRHS>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
   // algorithmic bytes: the triangle once, the inverted diagonal blocks once, X read and written
   timing_end(s, 6, sizeof(T) * 0.5 * n * (n + 1.0) + 8.0 * TB * (double)n + 16.0 * nrhs * (double)n);
@@ -891,6 +947,8 @@ int residual(hipStream_t s, int n, int nrhs, const double *A, long lda, const do
     hipLaunchKernelGGL(resid_partial_kernel<1>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
   else if (nrhs <= 4)
     hipLaunchKernelGGL(resid_partial_kernel<4>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
+  else if (nrhs <= 8)
+    hipLaunchKernelGGL(resid_partial_kernel<8>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
   else
     hipLaunchKernelGGL(resid_partial_kernel<MAXRHS>, rg, dim3(256), 0, s, n, nrhs, A, lda, X, ldx, part);
   // algorithmic bytes: A once, X once per column chunk (cached), the partials written once

@@ -7,9 +7,9 @@
 // panel below it, update the trailing lower triangle.  Here the step-k
 // panel is factored in FP64 (potrf_block + trsm_rltn on an FP64 copy of the
 // FP32 column block), stored back in FP32 for the solves, and quantized to
-// E4M3 for the SYRK.  If the FP8-perturbed matrix is not numerically positive
-// definite, the factorization is repeated for A + sigma I with a growing
-// shift; refinement then runs against the unshifted A.
+// E4M3 for the update.  If the FP8-perturbed matrix is not numerically
+// positive definite, the factorization is repeated for A + sigma I with a
+// growing shift; refinement then runs against the unshifted A.
 #include "pla_internal.hpp"
 
 #include <algorithm>
@@ -25,33 +25,96 @@ double g_last_shift = 0.0;
 
 // FP8-updated blocked Cholesky of W (FP32, lower).  *info: 0 or the 1-based
 // index of the first non-positive pivot.
-int chol_factor_fp8(hipStream_t s, int n, float *W, long ldw, int *info_out) {
-  const int nb = fp8_tile_rows();
+//
+// One step of look-ahead, as in chol_factor: the column block of step k+1
+// is factored on the panel stream as soon as step k's update of those
+// columns is done, while the update stream applies the rest of step k's
+// update.  The quantized panel and its exponents are double buffered by step
+// parity (step k's update may still read them while panel k+1 is quantized).
+//
+// update stream, step k: wait(panel k) -> lower(C[:bn, :bn]), C[bn:, :bn]
+//                        -> signal -> lower(C[bn:, bn:])
+// panel stream:          wait(signal) -> FP64 copy -> potrf -> trsm -> FP32
+//                        copy back -> quantize -> signal(panel k+1)
+int chol_factor_fp8(hipStream_t sMain, int n, float *W, long ldw, int *info_out) {
+  const int nb = ctx().fp8_nb == 256 ? 256 : 512;
+  const int kq = fp8_row_bytes(nb), QT = fp8_tile_rows();
+  const size_t lq_stride = (size_t)n * kq + 256;
+  const size_t ex_stride = (size_t)(n / QT + 2) * (kq / QT);
   double *P64 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_CH_P64);
-  unsigned char *Lq = (unsigned char *)workspace((size_t)n * nb + 256, WS_CH_LQ);
-  int *exps = (int *)workspace(sizeof(int) * ((size_t)n / nb + 2), WS_CH_EXP);
+  double *Linv = (double *)workspace(sizeof(double) * (size_t)nb * nb, WS_CH_LINV);
+  double *X21 = (double *)workspace(sizeof(double) * (size_t)n * nb, WS_CH_OUT);
+  unsigned char *Lq = (unsigned char *)workspace(2 * lq_stride, WS_CH_LQ);
+  int *exps = (int *)workspace(sizeof(int) * 2 * ex_stride, WS_CH_EXP);
   int *info = (int *)workspace(sizeof(int) * 4, WS_CH_INFO);
-  if (!P64 || !Lq || !exps || !info) return PLA_ERR_NO_MEMORY;
-  PLA_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
+  if (!P64 || !Linv || !X21 || !Lq || !exps || !info) return PLA_ERR_NO_MEMORY;
+  hipEvent_t ev_ready = lib_event(EV_CHOL_READY), ev_panel = lib_event(EV_CHOL_PANEL);
+  if (!ev_ready || !ev_panel) return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
+  PLA_HIP(hipMemsetAsync(info, 0, sizeof(int), sMain));
+  hipStream_t sP, sM;
   int rc;
-  for (int k = 0; k < n; k += nb) {
+  if ((rc = chol_streams_begin(sMain, &sP, &sM, ctx().fp8_masked != 0))) return rc;
+  // the column block at diagonal position k, quantized into parity par.
+  // L21 = A21 inv(L11)^T is one GEMM against the explicitly inverted L11
+  // (inv(L11) = L11 \ I): the triangular solve's chain of 64-column leaves
+  // would sit on the critical path of every step, and the factor is only a
+  // preconditioner, whose rounding the refinement removes.
+  auto panel = [&](int k, int par) -> int {
     const int b = std::min(nb, n - k), R = n - k - b;
     const long ldp = n - k;
     float *Wk = W + (long)k * ldw + k;
-    if ((rc = k_cvt_f32_f64(s, n - k, b, Wk, ldw, P64, ldp))) return rc;
-    if ((rc = potrf_block(s, b, P64, ldp, k, info))) return rc;
-    if (R > 0 && (rc = trsm_rltn(s, R, b, P64, ldp, P64 + b, ldp, info))) return rc;
-    if ((rc = k_cvt_f64_f32(s, n - k, b, P64, ldp, Wk, ldw))) return rc;
-    if (R > 0) {
-      if ((rc = k_fp8_quantize(s, R, b, P64 + b, ldp, Lq, exps))) return rc;
-      timing_begin(s, 4);
-      rc = k_fp8_syrk_lower(s, R, Lq, exps, W + (long)(k + b) * ldw + k + b, ldw, info);
-      timing_end(s, 4, (double)R * (R + 1) * b);
-      if (rc) return rc;
+    int r2;
+    if ((r2 = k_cvt_f32_f64(sP, n - k, b, Wk, ldw, P64, ldp))) return r2;
+    if ((r2 = potrf_block(sP, b, P64, ldp, k, info))) return r2;
+    if ((r2 = k_cvt_f64_f32(sP, b, b, P64, ldp, Wk, ldw))) return r2;
+    if (R <= 0) return PLA_SUCCESS;
+    if ((r2 = k_eye(sP, b, Linv, b))) return r2;
+    if ((r2 = trsm_left(sP, true, false, false, b, b, P64, ldp, Linv, b, info))) return r2;
+    if ((r2 = k_gemm(sP, 0, false, true, R, b, b, 1.0, P64 + b, ldp, Linv, b, 0.0, X21, R, info))) return r2;
+    if ((r2 = k_cvt_f64_f32(sP, R, b, X21, R, Wk + b, ldw))) return r2;
+    return k_fp8_quantize(sP, R, b, X21, R, Lq + par * lq_stride, exps + par * ex_stride);
+  };
+  // update of the trailing block's columns c0 .. c0 + nc (lower part) by panel k
+  auto update = [&](int k, int par, int c0, int nc, bool to_bottom) -> int {
+    const int b = std::min(nb, n - k), R = n - k - b;
+    const unsigned char *L = Lq + par * lq_stride;
+    const int *E = exps + par * ex_stride;
+    float *C = W + (long)(k + b + c0) * ldw + k + b + c0;
+    int r2;
+    timing_begin(sM, 4);
+    r2 = k_fp8_update(sM, true, nc, nc, b, L, E, c0, c0, C, ldw, info);
+    if (!r2 && to_bottom && R > c0 + nc)
+      r2 = k_fp8_update(sM, false, R - c0 - nc, nc, b, L, E, c0 + nc, c0, C + nc, ldw, info);
+    const double rows = to_bottom ? (double)(R - c0) : (double)nc;
+    timing_end(sM, 4, (double)nc * (nc + 1) * b + 2.0 * (rows - nc) * nc * b);
+    return r2;
+  };
+  auto steps = [&]() -> int {
+    int r2;
+    PLA_HIP(hipEventRecord(ev_ready, sM));
+    PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+    if ((r2 = panel(0, 0))) return r2;
+    PLA_HIP(hipEventRecord(ev_panel, sP));
+    for (int k = 0, step = 0; k < n; k += nb, step++) {
+      const int b = std::min(nb, n - k), R = n - k - b, par = step & 1;
+      PLA_HIP(hipStreamWaitEvent(sM, ev_panel, 0));
+      if (R <= 0) continue;
+      const int bn = std::min(nb, R);  // the next panel's columns first
+      if ((r2 = update(k, par, 0, bn, true))) return r2;
+      PLA_HIP(hipEventRecord(ev_ready, sM));
+      PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
+      if ((r2 = panel(k + b, par ^ 1))) return r2;
+      PLA_HIP(hipEventRecord(ev_panel, sP));
+      if (R > bn && (r2 = update(k, par, bn, R - bn, false))) return r2;
     }
-  }
-  PLA_HIP(hipMemcpyAsync(info_out, info, sizeof(int), hipMemcpyDeviceToHost, s));
-  PLA_HIP(hipStreamSynchronize(s));
+    return PLA_SUCCESS;
+  };
+  rc = steps();
+  const int rc_end = lu_streams_end(sMain, sP, sM);
+  if (rc) return rc;
+  if (rc_end) return rc_end;
+  PLA_HIP(hipMemcpyAsync(info_out, info, sizeof(int), hipMemcpyDeviceToHost, sMain));
+  PLA_HIP(hipStreamSynchronize(sMain));
   return PLA_SUCCESS;
 }
 
@@ -91,41 +154,58 @@ int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, i
   if (!W) return PLA_ERR_NO_MEMORY;
   double anorm = 0.0;
   if ((rc = sym_inf_norm(s, n, a, lda, &anorm))) return rc;
-  // factor A + shift I, growing the shift until the FP8-updated factorization
-  // completes (E4M3 unit roundoff 2^-4)
+  double *dmax_d = (double *)workspace(sizeof(double), WS_CH_D);
+  if (!dmax_d) return PLA_ERR_NO_MEMORY;
+  double dmax = 0.0;
+  if ((rc = k_max_diag(s, n, a, lda, dmax_d))) return rc;
+  PLA_HIP(hipMemcpyAsync(&dmax, dmax_d, sizeof(double), hipMemcpyDeviceToHost, s));
+  PLA_HIP(hipStreamSynchronize(s));
+  // Factor A + shift I, doubling the shift until the FP8-updated
+  // factorization completes.  The quantization error of L21 (relative
+  // 2^-4 per element) perturbs the Schur complements by about
+  // 2^-4 max_i A(i, i) in norm, so that is where the shifts start after an
+  // unshifted attempt fails; a smaller shift gives a better preconditioner.
   double shift = 0.0;
   int info = 0;
   for (;;) {
     if ((rc = k_cvt_lower_shift(s, n, a, lda, shift, W, ldw))) return rc;
     if ((rc = chol_factor_fp8(s, n, W, ldw, &info))) return rc;
     if (info == 0) break;
-    // a shift beyond ||A|| / 4 is far outside the FP8 perturbation of an SPD
-    // matrix (unit roundoff 2^-4): A itself is not positive definite
-    const double next = shift == 0.0 ? std::ldexp(anorm, -8) : 4.0 * shift;
-    if (!(anorm > 0.0) || next > 0.25 * anorm)
+    // a shift beyond max A(i, i) is far outside the FP8 perturbation of an
+    // SPD matrix: A itself is not positive definite
+    const double next = shift == 0.0 ? std::ldexp(dmax, ctx().fp8_shift_exp) : 2.0 * shift;
+    if (!(dmax > 0.0) || next > dmax)
       return fail(PLA_ERR_INVALID_PARAM, "PLA_Chol_mixed: A is not positive definite (pivot %d, shift %g)", info,
                   shift);
     shift = next;
   }
+  // L^T into the strict upper triangle of W: both solves run untransposed
+  if ((rc = k_mirror_lower_f32(s, n, W, ldw))) return rc;
   const int nblk = (n + trsv_block_size() - 1) / trsv_block_size();
   const size_t dsz = sizeof(double) * (size_t)nblk * trsv_block_size() * trsv_block_size();
   double *DinvL = (double *)workspace(dsz, WS_CH_DINVL);
-  double *DinvLT = (double *)workspace(dsz, WS_CH_DINVLT);
-  if (!DinvL || !DinvLT) return PLA_ERR_NO_MEMORY;
+  double *DinvU = (double *)workspace(dsz, WS_CH_DINVLT);
+  if (!DinvL || !DinvU) return PLA_ERR_NO_MEMORY;
   if ((rc = k_inv_diag_f32_op(s, true, false, false, n, W, ldw, DinvL))) return rc;
-  if ((rc = k_inv_diag_f32_op(s, true, true, false, n, W, ldw, DinvLT))) return rc;
+  if ((rc = k_inv_diag_f32_op(s, false, false, false, n, W, ldw, DinvU))) return rc;
+  // The refinement's residuals read A as a full symmetric FP64 matrix, built
+  // once from its lower triangle: one pass over n^2 contiguous elements per
+  // residual instead of the lower triangle read twice, by rows and by columns.
+  double *S = (double *)workspace(sizeof(double) * (size_t)n * n, WS_CH_FULL);
+  if (!S) return PLA_ERR_NO_MEMORY;
+  if ((rc = k_sym_full(s, n, a, lda, S, n))) return rc;
   refine_ops op;
   op.n = n;
   op.nrhs = nrhs;
   op.residual = [&](const double *Xv, long ldxv, const double *Bv, long ldbv, double *R, long ldr, double *rmax) {
-    return sym_residual(s, n, nrhs, a, lda, Xv, ldxv, Bv, ldbv, R, ldr, rmax);
+    return residual(s, n, nrhs, S, n, Xv, ldxv, Bv, ldbv, R, ldr, rmax);
   };
   op.precond = [&](const double *Y, long ldy, double *Z, long ldz) -> int {
     PLA_HIP(hipMemcpy2DAsync(Z, sizeof(double) * ldz, Y, sizeof(double) * ldy, sizeof(double) * n, nrhs,
                              hipMemcpyDeviceToDevice, s));
     int r2;
     if ((r2 = trsv_sf_f32(s, true, false, false, n, nrhs, W, ldw, Z, ldz, DinvL))) return r2;
-    return trsv_sf_f32(s, true, true, false, n, nrhs, W, ldw, Z, ldz, DinvLT);
+    return trsv_sf_f32(s, false, false, false, n, nrhs, W, ldw, Z, ldz, DinvU);
   };
   refine_stats st;
   if ((rc = refine(s, op, anorm, b, ldb, x, ldx, tol, max_iter, true, &st))) return rc;

@@ -22,6 +22,34 @@ int pla_th_fp8_syrk(int R, const unsigned char *Lq, const int *exps, float *C, l
   return PLA_SUCCESS;
 }
 
+// C -= Lq[ro..] Lq[co..]^T over the first b bytes of each quantized row
+// (tri: lower triangle of an M x M block with ro == co).
+int pla_th_fp8_update(int tri, int M, int N, int b, const unsigned char *Lq, const int *exps, int ro, int co,
+                      float *C, long ldc) {
+  int rc = k_fp8_update(ctx().stream, tri != 0, M, N, b, Lq, exps, ro, co, C, ldc, nullptr);
+  if (rc) return rc;
+  PLA_HIP(hipStreamSynchronize(ctx().stream));
+  return PLA_SUCCESS;
+}
+
+// W's strict upper triangle <- the transpose of its strict lower one.
+int pla_th_mirror_lower(int n, float *W, long ldw) {
+  int rc = k_mirror_lower_f32(ctx().stream, n, W, ldw);
+  if (rc) return rc;
+  PLA_HIP(hipStreamSynchronize(ctx().stream));
+  return PLA_SUCCESS;
+}
+
+// FP8 Cholesky options: block size (256 or 512; 0 keeps the current one),
+// panel on the CU-masked stream pair (0/1; -1 keeps), the first shift after a
+// failed unshifted attempt as 2^shift_exp max A(i, i) (a value >= 0 keeps).
+int pla_th_set_fp8_options(int nb, int masked, int shift_exp) {
+  if (nb == 256 || nb == 512) ctx().fp8_nb = nb;
+  if (masked >= 0) ctx().fp8_masked = masked;
+  if (shift_exp < 0) ctx().fp8_shift_exp = shift_exp;
+  return PLA_SUCCESS;
+}
+
 int pla_th_sym_residual(int n, int nrhs, const double *A, long lda, const double *X, long ldx, const double *B,
                         long ldb, double *R, long ldr, double *rmax) {
   return sym_residual(ctx().stream, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr, rmax);

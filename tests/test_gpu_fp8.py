@@ -35,6 +35,9 @@ def th(pla_session):
     L.pla_th_fp8_syrk.argtypes = [i, p, p, p, l]
     L.pla_th_sym_residual.argtypes = [i, i, p, l, p, l, p, l, p, l, p]
     L.pla_th_sym_inf_norm.argtypes = [i, p, l, ctypes.POINTER(d)]
+    L.pla_th_fp8_update.argtypes = [i, i, i, i, p, p, i, i, p, l]
+    L.pla_th_mirror_lower.argtypes = [i, p, l]
+    L.pla_th_set_fp8_options.argtypes = [i, i, i]
     return L
 
 
@@ -60,15 +63,17 @@ def _tiles_panel(rng, R, b):
 def test_fp8_quantize_matches_oracle(pla_session, ora, th):
     s = pla_session
     rng = np.random.default_rng(1)
-    for R, b in [(600, 200), (256, 256), (37, 5)]:
+    for R, b in [(600, 200), (256, 256), (37, 5), (700, 400), (300, 512)]:
         p = _tiles_panel(rng, R, b)
+        if b > 256:
+            p[:, 256:] *= 1e-3  # the second k tile gets its own exponents
         P = _Dev(s, p, pla.MPI_DOUBLE)
-        nt = (R + 255) // 256
-        Q = _Dev(s, np.zeros((R * 64, 1), np.int32), pla.MPI_INT)
-        E = _Dev(s, np.zeros((nt, 1), np.int32), pla.MPI_INT)
+        nt, kq = (R + 255) // 256, 256 * ((b + 255) // 256)
+        Q = _Dev(s, np.zeros((R * kq // 4, 1), np.int32), pla.MPI_INT)
+        E = _Dev(s, np.zeros((nt * kq // 256, 1), np.int32), pla.MPI_INT)
         assert th.pla_th_fp8_quantize(R, b, P.ptr, R, Q.ptr, E.ptr) == 0
-        got = Q.get()[:, 0].view(np.uint8).reshape(R, 256)
-        q_ref, ex_ref = ora.fp8_quantize_tiles(p)
+        got = Q.get()[:, 0].view(np.uint8).reshape(R, kq)
+        q_ref, ex_ref = ora.fp8_quantize_tiles(p, width=kq)
         assert np.array_equal(E.get()[:, 0], ex_ref)
         assert np.array_equal(got, q_ref), (R, b, np.argwhere(got != q_ref)[:5])
         pla.free(P.obj, Q.obj, E.obj)
@@ -96,6 +101,58 @@ def test_fp8_syrk_exact(pla_session, ora, th):
     assert np.array_equal(got[lo].astype(np.float64), full[lo])
     assert np.array_equal(got[up], c0[up])
     pla.free(C.obj, Q.obj, E.obj)
+
+
+@pytest.mark.parametrize("tri", [True, False])
+def test_fp8_update_two_k_tiles_exact(pla_session, ora, th, tri):
+    """C -= 2^(e(i,k) + e(j,k)) Lq_i Lq_j^T over a 400-column panel (two k
+    tiles, 512-byte rows) whose exponents differ per row tile and per k
+    tile; the triangle and the rectangle launches at 256-row offsets.  The
+    products and sums are exact in FP32, so the result is bit-exact."""
+    s = pla_session
+    rng = np.random.default_rng(5 + tri)
+    R, b, kq = 900, 400, 512
+    vals = np.array([0.0, 0.5, 1.0, 1.5, 2.0, 3.0, -0.5, -1.0, -2.0, -3.0])
+    lq = np.zeros((R, kq))
+    lq[:, :b] = rng.choice(vals, (R, b))
+    exps = rng.integers(-3, 4, (R + 255) // 256 * 2).astype(np.int32)
+    Q = _Dev(s, ora.e4m3_encode(lq).reshape(-1).view(np.int32).reshape(-1, 1), pla.MPI_INT)
+    E = _Dev(s, exps.reshape(-1, 1), pla.MPI_INT)
+    sc = np.exp2(exps.reshape(-1, 2).astype(np.float64))[np.arange(R) // 256]  # R x 2
+    full = np.zeros((R, R))
+    for kt in range(2):
+        blk = lq[:, 256 * kt:256 * (kt + 1)] * sc[:, kt:kt + 1]
+        full += blk @ blk.T
+    if tri:
+        ro = co = 256
+        M = N = R - 256
+    else:
+        ro, co, M, N = 512, 256, R - 512, 256
+    c0 = rng.integers(-1000, 1000, (M, N)).astype(np.float32)
+    C = _Dev(s, c0, pla.MPI_FLOAT)
+    assert th.pla_th_fp8_update(int(tri), M, N, b, Q.ptr, E.ptr, ro, co, C.ptr, M) == 0
+    got = C.get().astype(np.float64)
+    want = c0.astype(np.float64) - full[ro:ro + M, co:co + N]
+    if tri:
+        lo, up = np.tril_indices(M), np.triu_indices(M, 1)
+        assert np.array_equal(got[lo], want[lo])
+        assert np.array_equal(got[up], c0[up].astype(np.float64))
+    else:
+        assert np.array_equal(got, want)
+    pla.free(C.obj, Q.obj, E.obj)
+
+
+def test_mirror_lower(pla_session, th):
+    s = pla_session
+    rng = np.random.default_rng(6)
+    for n in (1, 63, 64, 200):
+        w = rng.standard_normal((n, n)).astype(np.float32)
+        W = _Dev(s, w, pla.MPI_FLOAT)
+        assert th.pla_th_mirror_lower(n, W.ptr, n) == 0
+        got = W.get()
+        want = np.tril(w) + np.tril(w, -1).T
+        assert np.array_equal(got, want)
+        pla.free(W.obj)
 
 
 def test_sym_residual_lower_only(pla_session, th):
@@ -168,6 +225,22 @@ def test_chol_fp8_reaches_fp64_backward_error(pla_session, n, delta, nrhs):
     x_ref = np.linalg.solve(a, b)
     cond = np.linalg.cond(a, 1)
     assert np.abs(x - x_ref).max() <= 50 * cond * 1.1e-16 * np.abs(x_ref).max()
+
+
+@pytest.mark.parametrize("nb,masked", [(256, 1), (512, 1), (256, 0)])
+def test_chol_fp8_block_sizes_and_streams(pla_session, th, nb, masked):
+    """The look-ahead FP8 factorization with either block size, its panel on
+    the CU-masked stream pair or beside the update, with a ragged last block."""
+    rng = np.random.default_rng(nb + masked)
+    n = 1700
+    a = _spd(rng, n, 4e-2)
+    b = rng.uniform(-1, 1, (n, 3))
+    th.pla_th_set_fp8_options(nb, masked, 0)
+    try:
+        x, it, be, _ = _chol_mixed(pla_session, a, b)
+    finally:
+        th.pla_th_set_fp8_options(512, 0, 0)
+    assert be <= 1e-14 and _eta(a, x, b) <= 2e-14, (it, be)
 
 
 def test_chol_fp8_not_spd_raises(pla_session):

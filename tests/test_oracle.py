@@ -426,6 +426,29 @@ def test_fp8_tile_quantization_scales(ora):
     assert (q[:, 200:] == 0).all()
 
 
+def test_fp8_tile_quantization_two_k_tiles_and_clamp(ora):
+    """256 x 256 blocks along k too (a 400-column panel in 512-byte rows);
+    each block's exponent follows its own maximum; an exponent below -127 (the
+    E8M0 range) is clamped, and the block's values then round towards zero."""
+    rng = np.random.default_rng(1)
+    p = rng.uniform(-1, 1, (300, 400))
+    p[:, 256:] *= 1e3
+    p[256:, :256] *= 1e-60
+    q, ex = ora.fp8_quantize_tiles(p, width=512)
+    assert q.shape == (300, 512) and ex.shape == (4,)
+    want = [int(np.frexp(np.abs(blk).max() / 448)[1]) for blk in
+            (p[:256, :256], p[:256, 256:], p[256:, :256], p[256:, 256:])]
+    want = [min(127, max(-127, e)) for e in want]
+    assert ex.tolist() == want and ex[2] == -127
+    for t, (rs, cs) in enumerate(((slice(0, 256), slice(0, 256)), (slice(0, 256), slice(256, 400)),
+                                  (slice(256, 300), slice(256, 400)))):
+        e = ex[[0, 1, 3][t]]
+        deq = ora.e4m3_decode(q[rs, cs]) * 2.0 ** e
+        assert np.abs(deq - p[rs, cs]).max() <= 2.0 ** -4 * np.abs(p[rs, cs]).max()
+    assert np.abs(ora.e4m3_decode(q[256:, :256])).max() * 2.0 ** -127 <= 2.0 ** -136
+    assert (q[:, 400:] == 0).all()
+
+
 def test_blas_restatements_match_the_scalar_oracle(ora):
     """The CPU baseline's restatements over the host BLAS (lu_blocked_blas,
     chol_blocked_blas) follow the same algorithms as the scalar oracle: the
