@@ -15,7 +15,7 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int QT = 256;        // rows and columns of one scale tile
-constexpr int KT_MAX = 2;      // scale tiles along k: panels of up to 512 columns
+constexpr int KT_MAX = 4;      // scale tiles along k: panels of up to 1024 columns
 constexpr int E4M3_MAX = 448;  // largest finite E4M3 value
 constexpr int EXP_LIM = 127;   // an E8M0 scale spans 2^-127 .. 2^127
 

@@ -37,7 +37,7 @@ double g_last_shift = 0.0;
 // panel stream:          wait(signal) -> FP64 copy -> potrf -> trsm -> FP32
 //                        copy back -> quantize -> signal(panel k+1)
 int chol_factor_fp8(hipStream_t sMain, int n, float *W, long ldw, int *info_out) {
-  const int nb = ctx().fp8_nb == 256 ? 256 : 512;
+  const int nb = (ctx().fp8_nb == 256 || ctx().fp8_nb == 1024) ? ctx().fp8_nb : 512;
   const int kq = fp8_row_bytes(nb), QT = fp8_tile_rows();
   const size_t lq_stride = (size_t)n * kq + 256;
   const size_t ex_stride = (size_t)(n / QT + 2) * (kq / QT);
@@ -162,9 +162,11 @@ int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, i
   PLA_HIP(hipStreamSynchronize(s));
   // Factor A + shift I, doubling the shift until the FP8-updated
   // factorization completes.  The quantization error of L21 (relative
-  // 2^-4 per element) perturbs the Schur complements by about
-  // 2^-4 max_i A(i, i) in norm, so that is where the shifts start after an
-  // unshifted attempt fails; a smaller shift gives a better preconditioner.
+  // 2^-4 per element) perturbs the Schur complements by a small multiple of
+  // 2^-4 max_i A(i, i) in norm at most; after an unshifted attempt fails the
+  // shifts start at 2^-6 max_i A(i, i) (measured on B B^T / n + delta I,
+  // n = 49152, delta = 4e-4: enough, and at 2^-4 GMRES-IR needed 85 instead
+  // of 54 steps); a smaller shift gives a better preconditioner.
   double shift = 0.0;
   int info = 0;
   for (;;) {
@@ -197,6 +199,10 @@ int PLA_Chol_mixed(PLA_Obj A, PLA_Obj B, PLA_Obj X, int precision, double tol, i
   refine_ops op;
   op.n = n;
   op.nrhs = nrhs;
+  // a shifted FP8 factor is a weaker preconditioner: a longer GMRES cycle
+  // (its orthogonalization costs little beside the two solves per step)
+  op.restart = 100;
+  op.gmres_first = ctx().fp8_gmres_first != 0;
   op.residual = [&](const double *Xv, long ldxv, const double *Bv, long ldbv, double *R, long ldr, double *rmax) {
     return residual(s, n, nrhs, S, n, Xv, ldxv, Bv, ldbv, R, ldr, rmax);
   };

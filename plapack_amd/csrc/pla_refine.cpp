@@ -9,6 +9,7 @@
 // right-hand side) is kept on the host.
 #include "pla_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -157,7 +158,7 @@ int refine(hipStream_t s, const refine_ops &op, double anorm, const double *B, l
   if ((rc = col_amax(s, n, nrhs, B, ldb, bmax.data()))) return rc;
   if ((rc = op.precond(B, ldb, X, ldx))) return rc;
   refine_stats out;
-  bool use_gmres = false;
+  bool use_gmres = gmres_fallback && op.gmres_first;
   double prev = INFINITY;
   for (;;) {
     if ((rc = op.residual(X, ldx, B, ldb, Rb, n, rmax.data()))) return rc;
@@ -182,8 +183,12 @@ int refine(hipStream_t s, const refine_ops &op, double anorm, const double *B, l
     out.eta = eta;
     if (eta <= tol || out.outer >= max_iter) break;
     if (use_gmres) {
+      // the cycle only needs to take eta below tol: its relative tolerance
+      // follows from the current eta, with a factor 100 of margin for the
+      // gap between the preconditioned residual GMRES sees and eta
+      const double tau = std::min(1e-2, std::max(op.gmres_tol, 1e-2 * tol / eta));
       int steps = 0;
-      if ((rc = gmres(s, op, Rb, n, Db, op.restart, op.gmres_tol, &steps))) return rc;
+      if ((rc = gmres(s, op, Rb, n, Db, op.restart, tau, &steps))) return rc;
       out.inner += steps;
       out.gmres_outer++;
     } else {

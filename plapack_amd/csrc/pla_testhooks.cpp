@@ -40,13 +40,20 @@ int pla_th_mirror_lower(int n, float *W, long ldw) {
   return PLA_SUCCESS;
 }
 
-// FP8 Cholesky options: block size (256 or 512; 0 keeps the current one),
+// FP8 Cholesky options: block size (256, 512 or 1024; 0 keeps the current one),
 // panel on the CU-masked stream pair (0/1; -1 keeps), the first shift after a
 // failed unshifted attempt as 2^shift_exp max A(i, i) (a value >= 0 keeps).
 int pla_th_set_fp8_options(int nb, int masked, int shift_exp) {
-  if (nb == 256 || nb == 512) ctx().fp8_nb = nb;
+  if (nb == 256 || nb == 512 || nb == 1024) ctx().fp8_nb = nb;
   if (masked >= 0) ctx().fp8_masked = masked;
   if (shift_exp < 0) ctx().fp8_shift_exp = shift_exp;
+  return PLA_SUCCESS;
+}
+
+// FP8 Cholesky refinement: GMRES-IR from the first correction (1) or only
+// after classic refinement stalls (0).
+int pla_th_set_fp8_gmres_first(int on) {
+  ctx().fp8_gmres_first = on;
   return PLA_SUCCESS;
 }
 
