@@ -232,6 +232,7 @@ leaf_ws &leaf_workspace(size_t slot_bytes) {
   g_leaf.slot_bytes = slot_bytes;
   g_leaf.slots = (unsigned long long *)p;
   g_leaf.err = (int *)((char *)p + slot_bytes);
+  g_leaf.state = (unsigned long long *)((char *)p + slot_bytes + 64);
   return g_leaf;
 }
 
@@ -243,6 +244,7 @@ int leaf_tags(hipStream_t s, unsigned count, unsigned *tag0) {
   if (!g_leaf.base) return fail(PLA_ERR_INVALID_PARAM, "panel exchange workspace not allocated");
   if (g_leaf_tag > 0xF0000000u) {  // tags only grow; restart far below wrap-around
     PLA_HIP(hipMemsetAsync(g_leaf.slots, 0, g_leaf.slot_bytes, s));
+    PLA_HIP(hipMemsetAsync(g_leaf.state, 0, sizeof(unsigned long long), s));
     g_leaf_tag = 1;
   }
   *tag0 = g_leaf_tag;

@@ -129,7 +129,7 @@ int masked_streams(int k);
 
 // Grow-only device workspaces (reused across calls, freed at PLA_Finalize);
 // `zero`: clear the memory when it is (re)allocated.
-constexpr int PLA_NUM_WS = 57;
+constexpr int PLA_NUM_WS = 58;
 // slots: 0-2 factorizations, 3-4 LU interchanges, 5-9 triangular solves and
 // residual, 10-23 mixed LU, 24-28 symmetric residual / inner products,
 // 29-33 GMRES, 34-42 mixed Cholesky, 43-44 application-interface staging, 45-48 complex
@@ -143,7 +143,8 @@ enum {
   WS_UTIL = 49, WS_UTIL_NORM,
   WS_ZOPA = 51, WS_ZOPB, WS_ZTRI,  // complex BLAS3 operands (pla_zblas.cpp)
   WS_CH_FULL = 54,                  // full symmetric FP64 copy of A (PLA_Chol_mixed refinement)
-  WS_CH_LINV, WS_CH_OUT             // inverse of L11 and the solved panel (PLA_Chol_mixed)
+  WS_CH_LINV, WS_CH_OUT,            // inverse of L11 and the solved panel (PLA_Chol_mixed)
+  WS_LEAF_FB = 57                   // scratch of a panel leaf factored by one workgroup after its exchange gave up
 };
 void *workspace(size_t bytes, int slot, bool zero = false);
 void release_workspaces();
@@ -178,6 +179,8 @@ struct leaf_ws {
   size_t slot_bytes = 0;
   unsigned long long *slots = nullptr;  // tagged exchange words
   int *err = nullptr;                   // timeout flag
+  unsigned long long *state = nullptr;  // committed / abandoned word of the current leaf launch,
+                                        // then the count of leaves factored by the fallback
 };
 leaf_ws &leaf_workspace(size_t slot_bytes);
 // First of `count` fresh exchange tags for a leaf launch on stream s.

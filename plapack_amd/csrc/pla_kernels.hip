@@ -26,6 +26,7 @@
 //
 // All matrices are column-major double with a 64-bit leading dimension.
 #include "pla_internal.hpp"
+#include "pla_leaf_fallback.hpp"
 
 #include <climits>
 #include <cstdint>
@@ -506,6 +507,8 @@ struct leaf_args {
   int *pairs;               // PAIRS_STRIDE ints, positions relative to the panel's first row
   unsigned long long *slots;  // [2][G][CAND_WORDS] then [2][G][ROW_WORDS]
   int *err;
+  unsigned long long *state;  // the launch's committed / abandoned word
+  char *fscr;                 // workgroup 0's scratch if the launch is abandoned
   unsigned tag0, spin_limit;
 };
 
@@ -533,15 +536,21 @@ __device__ inline double dbl_from(unsigned lo, unsigned hi) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Bounded wait: every lane counts the same iterations; the error flag is read
-// by lane 0 and the decision is wave-uniform, so the wave leaves together.
-// Returns true when the wait is abandoned (after setting *err).
+// Bounded wait: every lane counts the same iterations, and the state word is
+// read by lane 0, so the decision is wave-uniform.  Returns true when the
+// launch is abandoned: this wait gave up first, or another participant's did.
+// A wait that gives up after another participant committed the launch keeps
+// waiting: the records it waits for exist.
 __device__ inline bool wave_spin_fail(unsigned &it, const leaf_args &a) {
   ++it;
-  const int bail = (it > a.spin_limit) || ((threadIdx.x & 63) == 0 && (it & 63) == 0 && ld_dev(a.err));
-  if (__ballot(bail)) {
-    if ((threadIdx.x & 63) == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+  const bool timed_out = it > a.spin_limit;
+  if (timed_out || (it & 63) == 1) {
+    int st = 0;
+    if ((threadIdx.x & 63) == 0)
+      st = timed_out ? leaffb::decide(a.state, a.tag0, 2) : leaffb::state(a.state, a.tag0);
+    st = __shfl(st, 0, 64);
+    if (st == 2) return true;
+    if (timed_out) it = 0;
   }
   __builtin_amdgcn_s_sleep(1);
   return false;
@@ -578,13 +587,14 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
   __shared__ double prow_s[LEAF_THREADS / 64][LEAF_W];  // each wave's copy of the winning row
   __shared__ int s_p[LEAF_W];
   __shared__ int s_dst[PAIRS_MAX], s_src[PAIRS_MAX];
-  __shared__ int s_cnt;
+  __shared__ int s_cnt, s_outcome;
+  bool dead = false;  // a wait of this wave found the launch abandoned
 
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int r0 = g * R;
   const int rows = max(0, min(a.mp - r0, R));
-  // a timed-out exchange earlier on this stream: leave the panel untouched
+  // a bounded wait that gave up earlier on this stream (the error word): leave the panel untouched
   // and record no interchanges
   if (ld_dev(a.err)) {
     if (g == 0 && tid == 0) a.pairs[0] = 0;
@@ -656,7 +666,10 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
           if (ci >= j && ci < a.mp) argmax3(gv, gi, gw, fabs(dbl_from((unsigned)w0, (unsigned)w1)), ci, t);
         }
         if (__all(ok)) break;
-        if (wave_spin_fail(it, a)) break;
+        if (wave_spin_fail(it, a)) {
+          dead = true;
+          break;
+        }
       }
       wave_argmax3(gv, gi, gw);
     }
@@ -675,7 +688,10 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
         }
         if (lane < w) prow_s[wave][lane] = v;
         if (__all(ok)) break;
-        if (wave_spin_fail(it, a)) break;
+        if (wave_spin_fail(it, a)) {
+          dead = true;
+          break;
+        }
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -704,7 +720,22 @@ __global__ __launch_bounds__(LEAF_THREADS) void lu_leaf_kernel(leaf_args a) {
       if (g == 0) a.piv[j] = p - j;
     }
   }
-  __syncthreads();
+  // the workgroup decides the launch: committed (every wave saw every record)
+  // or abandoned (some wait gave up first); the outcome is the same for all.
+  // An abandoned launch writes nothing, and workgroup 0 factors the leaf
+  // alone from memory.
+  {
+    const int any_dead = __syncthreads_or(dead);
+    if (tid == 0) s_outcome = leaffb::decide(a.state, a.tag0, any_dead ? 2 : 1);
+    __syncthreads();
+    if (s_outcome == 2) {
+      if (g == 0) {
+        if (tid == 0) atomicAdd(a.state + 1, 1ULL);  // fallbacks run, for the tests
+        leaffb::factor(a.P, a.Ppanel, a.lda, a.mp, a.w, a.pw, a.pc0, a.piv, a.pairs, a.fscr);
+      }
+      return;
+    }
+  }
 
   // rows to their final positions (a permutation: no two rows collide)
   for (int e = tid; e < rows * w; e += LEAF_THREADS) {
@@ -776,7 +807,7 @@ __device__ inline int hslot(int key) { return (int)(((unsigned)key * 2654435761u
 
 __global__ __launch_bounds__(256) void panel_compose_kernel(const int *__restrict__ pairs, int nleaves,
                                                             int *__restrict__ out, const int *__restrict__ err) {
-  // after a timed-out exchange the leaves' lists are not trustworthy: no interchanges
+  // after a bounded wait gave up (the error word) the leaves' lists are not trustworthy: no interchanges
   if (ld_dev(err)) {
     if (threadIdx.x == 0) out[0] = 0;
     return;
@@ -835,7 +866,7 @@ __global__ __launch_bounds__(256) void panel_compose_kernel(const int *__restric
 template <typename T>
 __global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long lda, int nleft, int c1, int c2,
                                                          const int *__restrict__ comp, const int *__restrict__ err) {
-  if (ld_dev(err)) return;  // a timed-out exchange: the list is not trustworthy
+  if (ld_dev(err)) return;  // a bounded wait gave up: the list is not trustworthy
   const int cnt = min(comp[0], COMP_MAX);
   const int total = nleft + (c2 - c1);
   const int lane = threadIdx.x & 63;
@@ -1530,6 +1561,9 @@ int k_lu_leaf(hipStream_t s, const lu_leaf_call &c) {
   a.pairs = c.pairs;
   a.slots = ws.slots;
   a.err = ws.err;
+  a.state = ws.state;
+  a.fscr = (char *)workspace(leaffb::scratch_bytes(c.mp), WS_LEAF_FB);
+  if (!a.fscr) return PLA_ERR_NO_MEMORY;
   a.tag0 = tag0;
   a.spin_limit = ctx().leaf_spin_limit;
   static bool lds_attr = false;  // allow more than 64 KiB of dynamic LDS (gfx950 has 160 KiB)

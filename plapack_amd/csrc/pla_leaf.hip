@@ -3,7 +3,7 @@
 // Replaces PLA_LU_right_mv (SOLVERS/LU/PLA_LU_mv.c:52-82) for a leaf of up to
 // W columns of the panel: per column PLA_Iamax (BLAS1/PLA_Iamax.c:37-124)
 // becomes a wave reduction plus an exchange of (value, position, row)
-// records between workgroups, Pivot_rows_mv (PLA_Apply_pivots_new.c:208-305)
+// records between waves, Pivot_rows_mv (PLA_Apply_pivots_new.c:208-305)
 // a relabelling of logical row positions, PLA_Local_inv_scal
 // (Local_BLAS/PLA_Local_BLAS1.c:258) one reciprocal and a multiply, and
 // PLA_Local_ger (Local_BLAS/PLA_Local_BLAS2.c:578-656) a rank-1 update.
@@ -18,8 +18,9 @@
 // wave, two waves per SIMD) it takes the place of one update workgroup on
 // each of its G / 4 CUs, not on G CUs as single-wave workgroups spread over
 // the chip would.  All waves of one leaf must be resident at once (they
-// exchange pivot candidates every column); a bounded wait turns a failure to
-// become resident into an error.
+// exchange pivot candidates every column); when a bounded wait gives up
+// first, the launch is abandoned and workgroup 0 factors the leaf alone
+// (pla_leaf_fallback.hpp).
 //
 // Per column j (rows are never moved; each carries its logical position,
 // initially its row number; positions < j hold the pivot rows chosen so far):
@@ -37,15 +38,16 @@
 //      published (its row must carry this step's update in every column).
 // Exchange words carry the column's 32-bit tag beside 32 bits of payload;
 // readers re-read until the tags match (no fences, no flags).  Records are
-// double buffered by column parity.  A bounded wait sets *err; every wave
-// still runs to the end, and kernels that consume the leaf's interchanges
-// skip them once *err is set.
+// double buffered by column parity.  A wave that finishes every column
+// commits the launch; a bounded wait that gives up before that abandons it,
+// and every wave then stops without writing the matrix.
 //
 // At the end rows are written to their final positions, the interchanges are
 // replayed into at most 2w (destination, source) row pairs, and the pairs are
 // applied to the panel's other columns, so the recursive panel needs no
 // separate row-interchange launches.
 #include "pla_internal.hpp"
+#include "pla_leaf_fallback.hpp"
 
 #include <climits>
 #include <cstdint>
@@ -71,6 +73,8 @@ struct rleaf_args {
   int *pairs;               // RL_PAIRS_STRIDE ints, positions relative to the leaf's first row + pc0
   unsigned long long *slots;  // per column parity: headers [3][RL_MAXG], rows [RL_MAXG][RL_ROWW]
   int *err;
+  unsigned long long *state;  // the launch's committed / abandoned word
+  char *fscr;                 // workgroup 0's scratch if the launch is abandoned
   unsigned tag0, spin_limit;
 };
 
@@ -119,17 +123,28 @@ __device__ inline void wave_argmax(double &bv, int &bp, int &bx) {
   }
 }
 
-// Bounded wait: the decision is wave-uniform.  Returns true when abandoned.
+// Bounded wait (the counter, and so every decision, is wave-uniform).
+// Returns true when the launch is abandoned: this wait gave up first, or
+// another wave's did.  A wait that gives up after another wave committed the
+// launch keeps waiting: the records it waits for exist.
 __device__ inline bool spin_fail(unsigned &it, const rleaf_args &a) {
   ++it;
-  const int bail = (it > a.spin_limit) || ((threadIdx.x & 63) == 0 && (it & 63) == 0 && ld_dev(a.err));
-  if (__ballot(bail)) {
-    if ((threadIdx.x & 63) == 0) __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
+  const bool timed_out = it > a.spin_limit;
+  if (timed_out || (it & 63) == 1) {
+    int st = 0;
+    if ((threadIdx.x & 63) == 0)
+      st = timed_out ? leaffb::decide(a.state, a.tag0, 2) : leaffb::state(a.state, a.tag0);
+    st = __shfl(st, 0, 64);
+    if (st == 2) return true;
+    if (timed_out) it = 0;
   }
   __builtin_amdgcn_s_sleep(1);
   return false;
 }
+
+template <int W, int RPL>
+__device__ inline void leaf_finish(const rleaf_args &a, double (&v)[RPL][W], const int (&pos)[RPL], int ph, int g,
+                                   int G, int lane);
 
 template <int W>
 __global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
@@ -139,7 +154,7 @@ __global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
   const int G = gridDim.x * RL_WG_WAVES, g = blockIdx.x * RL_WG_WAVES + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int w = a.w, mp = a.mp;
   const long lda = a.lda;
-  if (ld_dev(a.err)) {  // a timed-out exchange earlier on this stream: no interchanges
+  if (ld_dev(a.err)) {  // a bounded wait gave up earlier on this stream (the error word): no interchanges
     if (g == 0 && lane == 0) a.pairs[0] = 0;
     return;
   }
@@ -199,10 +214,11 @@ __global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
   };
 
   publish(std::integral_constant<int, 0>{});
-  int ph = 0;  // lane j: the pivot position chosen for column j
+  int ph = 0;         // lane j: the pivot position chosen for column j
+  bool dead = false;  // the launch was abandoned: stop, write nothing
   static_for<W>([&](auto J) {
     constexpr int j = decltype(J)::value;
-    if (j < w) {
+    if (j < w && !dead) {
       const unsigned tagv = a.tag0 + (unsigned)j;
       const unsigned long long *par = slots + (size_t)(j & 1) * RL_PAR;
       // every workgroup's (position, value)
@@ -228,10 +244,14 @@ __global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
             }
           }
           if (__all(ok)) break;
-          if (spin_fail(it, a)) break;
+          if (spin_fail(it, a)) {
+            dead = true;
+            break;
+          }
         }
         wave_argmax(gv, gp, gw);
       }
+      if (dead) return;
       // the winner's row (all-NaN column: no winner, no interchange); lane l in
       // [2j, 2W) holds row word l, so column c is in lanes 2c (low) and 2c + 1 (high)
       unsigned rw = 0u;
@@ -246,9 +266,13 @@ __global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
             rw = (unsigned)x;
           }
           if (__all(ok)) break;
-          if (spin_fail(it, a)) break;
+          if (spin_fail(it, a)) {
+            dead = true;
+            break;
+          }
         }
       }
+      if (dead) return;
       // wave-uniform element c of the pivot row
       auto prow = [&](int c) -> double {
         if (gw < 0) return __longlong_as_double(0x7FF8000000000000LL);
@@ -284,7 +308,29 @@ __global__ __launch_bounds__(256, 2) void lu_rleaf_kernel(rleaf_args a) {
     }
   });
 
-  // rows to their final positions (a permutation: no two rows collide)
+  // every wave decides the launch: committed (this wave saw every record) or
+  // abandoned (some wait gave up first); the outcome is the same for all
+  int outcome = 0;
+  if (lane == 0) outcome = leaffb::decide(a.state, a.tag0, dead ? 2 : 1);
+  outcome = __shfl(outcome, 0, 64);
+  if (outcome == 1) leaf_finish(a, v, pos, ph, g, G, lane);
+  if (blockIdx.x == 0) {  // workgroup 0 is always resident with itself
+    __syncthreads();
+    if (outcome == 2) {
+      if (threadIdx.x == 0) atomicAdd(a.state + 1, 1ULL);  // fallbacks run, for the tests
+      leaffb::factor(a.P, a.Ppanel, a.lda, a.mp, a.w, a.pw, a.pc0, a.piv, a.pairs, a.fscr);
+    }
+  }
+}
+
+// The committed launch's end: rows to their final positions (a permutation:
+// no two rows collide), the interchanges replayed into (dst, src) pairs, and
+// the pairs applied to the panel's other columns.
+template <int W, int RPL>
+__device__ inline void leaf_finish(const rleaf_args &a, double (&v)[RPL][W], const int (&pos)[RPL], int ph, int g,
+                                   int G, int lane) {
+  const int w = a.w;
+  const long lda = a.lda;
 #pragma unroll
   for (int c = 0; c < W; c++)
 #pragma unroll
@@ -402,6 +448,9 @@ int k_lu_rleaf(hipStream_t s, const lu_leaf_call &c, int W) {
   a.pairs = c.pairs;
   a.slots = ws.slots;
   a.err = ws.err;
+  a.state = ws.state;
+  a.fscr = (char *)workspace(leaffb::scratch_bytes(c.mp), WS_LEAF_FB);
+  if (!a.fscr) return PLA_ERR_NO_MEMORY;
   a.tag0 = tag0;
   a.spin_limit = ctx().leaf_spin_limit;
   switch (W) {

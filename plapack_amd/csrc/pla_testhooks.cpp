@@ -73,6 +73,18 @@ int pla_th_set_leaf_spin_limit(unsigned limit) {
   return PLA_SUCCESS;
 }
 
+// Number of panel leaves factored by one workgroup after their exchange gave
+// up, since the last call (the count is reset).
+int pla_th_leaf_fallbacks(unsigned long long *out) {
+  leaf_ws &w = leaf_workspace(0);
+  *out = 0;
+  if (!w.base) return PLA_SUCCESS;
+  PLA_HIP(hipStreamSynchronize(ctx().stream));
+  PLA_HIP(hipMemcpy(out, w.state + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  PLA_HIP(hipMemset(w.state + 1, 0, sizeof(unsigned long long)));
+  return PLA_SUCCESS;
+}
+
 // Poll limit of the triangular-solve block waits (0 restores the default).
 int pla_th_set_trsv_spin_limit(unsigned limit) {
   ctx().trsv_spin_limit = limit ? limit : 2000000u;

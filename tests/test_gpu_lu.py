@@ -284,20 +284,52 @@ def test_panel_param_errors(pla_session):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_lu_exchange_timeout_reports_error(pla_session, ora, kind):
+def test_lu_exchange_timeout_falls_back(pla_session, ora, kind):
     """A panel exchange that gives up (forced from the host with a one-poll
-    wait limit) makes PLA_LU return PLA_ERR_DEVICE; every later kernel of the
-    factorization skips the interchanges, and the next call is correct."""
+    wait limit) abandons the leaf launch; workgroup 0 then factors the leaf
+    alone from the untouched panel, and PLA_LU completes with the same
+    pivots and factors, bit for bit, as with the exchange (every leaf uses
+    the same arithmetic in the same order).  Some leaves commit anyway (all
+    records in time), so both ends of a launch are exercised; the fallback
+    count proves the fallback ran.  The mixed LU uses the same leaves."""
     hooks = ctypes.CDLL(os.path.join(os.path.dirname(pla.LIB_PATH), "libpla_testhooks.so"))
     hooks.pla_th_set_leaf_spin_limit.argtypes = [ctypes.c_uint]
+    hooks.pla_th_leaf_fallbacks.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
     s = pla_session
-    n = 4096
+    n = 3000
+    fb = ctypes.c_ulonglong()
     pla.call("PLA_Set_device_param", pla.PLA_PARAM_LEAF_KIND, kind)
-    hooks.pla_th_set_leaf_spin_limit(1)
     try:
-        with pytest.raises(pla.PLAError) as e:
-            _lu_gpu(s, n=n, seed=0, nb=256)
-        assert e.value.code == -1006 and "timed out" in str(e.value)
+        lu_ref, piv_ref = _lu_gpu(s, n=n, seed=7, nb=256)
+        hooks.pla_th_leaf_fallbacks(ctypes.byref(fb))
+        assert fb.value == 0
+        hooks.pla_th_set_leaf_spin_limit(1)
+        lu, piv = _lu_gpu(s, n=n, seed=7, nb=256)
+        hooks.pla_th_leaf_fallbacks(ctypes.byref(fb))
+        assert fb.value > 0
+        assert np.array_equal(piv, piv_ref)
+        assert np.array_equal(lu, lu_ref)
+        # a ragged panel and narrow leaves, against the oracle
+        a = ora.uniform(1100, 900, 8)
+        lu2, piv2 = _lu_gpu(s, a=a, nb=96)
+        lu_o, piv_o = ora.lu_blocked(a, nb=96)
+        assert np.array_equal(piv2, piv_o)
+        assert np.abs(lu2 - lu_o).max() <= 1e-7 * np.abs(a).max()
+        # the mixed LU (FP64 panels on the same leaves) reaches FP64 accuracy
+        b = np.random.default_rng(8).uniform(-1, 1, (2000, 2))
+        am = ora.uniform(2000, 2000, 9)
+        A, B = pla.from_numpy(s, am), pla.from_numpy(s, b)
+        P, X = s.mvector(2000, dtype=pla.MPI_INT), s.matrix(2000, 2)
+        it, be = ctypes.c_int(), ctypes.c_double()
+        pla.call("PLA_LU_mixed", A, P, B, X, pla.PLA_MIXED_FP16X3, 1e-14, 30, ctypes.byref(it), ctypes.byref(be))
+        x = pla.get_contents(X)
+        pla.free(A, B, P, X)
+        r = b - am @ x
+        eta = (np.abs(r).max(axis=0) / (np.abs(am).sum(axis=1).max() * np.abs(x).max(axis=0)
+                                         + np.abs(b).max(axis=0))).max()
+        assert eta <= 2e-14
+        hooks.pla_th_leaf_fallbacks(ctypes.byref(fb))
+        assert fb.value > 0
     finally:
         hooks.pla_th_set_leaf_spin_limit(0)
         pla.call("PLA_Set_device_nb", 0)
