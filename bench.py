@@ -45,7 +45,13 @@ ROLE_LU, ROLE_CHOL, ROLE_MIXED, ROLE_FP8, ROLE_RESID, ROLE_TRSV, ROLE_SWAP = 1, 
 # HBM bytes per launch at the default sizes (rocprofv3 --pmc FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the scale factors measured on a copy of known size);
 # profiles/trailing_gemm_traffic_n65536.md and profiles/hbm_kernels_traffic.md
-TRAFFIC = {"hgemm": 9.33e9, "dgemm": 5.91e10}
+TRAFFIC = {"hgemm": 9.33e9, "dgemm": 5.91e10,
+           # n = 65536: the mixed and plain LU's HBM kernels
+           "resid_lu": 3.445e10, "trsv_f32": 8.706e9, "swap_f32": 2.088e9, "swap_f64": 5.619e9,
+           "trsv_f64": 1.883e10,
+           # n = 49152, FP8 Cholesky: per timed update (a triangle launch, and a
+           # rectangle launch every other time), residual, 8-column solve
+           "fp8_update": 4.05e9, "resid_fp8": 1.966e10, "trsv_fp8": 4.934e9}
 
 
 def cpu_baseline(n_lu, n_chol):
@@ -296,8 +302,10 @@ def run_cholesky(pla, s, n_plain, n_fp8, deltas, steps):
             pla.call("PLA_Chol_mixed", A, B, X, pla.PLA_MIXED_FP8, 1e-14, 50, ctypes.byref(it), ctypes.byref(be))
             pla.call("PLA_Device_synchronize")
             el = time.perf_counter() - t
-            rl = roofline("fp8_syrk_kernel (lower(W22) -= L21 L21^T, E4M3, FP32 accumulate)",
-                          *timing_query(pla, ROLE_FP8), FP8_MFMA_PEAK)
+            dflt = (n == 49152)
+            rl = roofline("fp8_update_kernel (lower(W22) -= L21 L21^T, E4M3 with E8M0 tile scales, FP32 accumulate)",
+                          *timing_query(pla, ROLE_FP8), FP8_MFMA_PEAK,
+                          traffic=TRAFFIC["fp8_update"] if dflt else None)
             lr, msr, br = timing_query(pla, ROLE_RESID)
             lt, mst, bt = timing_query(pla, ROLE_TRSV)
             pla.call("PLA_Kernel_timing", 0)
@@ -309,10 +317,10 @@ def run_cholesky(pla, s, n_plain, n_fp8, deltas, steps):
                    "ms": round(el * 1e3, 1), "backward_error": be.value, "backward_error_recomputed": eta,
                    "corrections": it.value, "gmres_steps": steps_g.value, "gmres_corrections": gcorr.value,
                    "shift": shift.value, "roofline": rl,
-                   "hbm_rooflines": [hbm_roofline("sresid_n/t_kernel (r = b - A x, A symmetric, lower triangle)",
-                                                  lr, msr, br),
-                                     hbm_roofline("trsv_sf/nt_kernel<float> (FP32 Cholesky factor solves)", lt, mst,
-                                                  bt)]}
+                   "hbm_rooflines": [hbm_roofline("resid_partial_kernel<8> (r = b - A x on the full symmetric FP64 copy)",
+                                                  lr, msr, br, TRAFFIC["resid_fp8"] if dflt else None),
+                                     hbm_roofline("trsv_nt_kernel<float, 8> (solves with L and the mirrored L^T)", lt,
+                                                  mst, bt, TRAFFIC["trsv_fp8"] if dflt else None)]}
             res["runs"].append(run)
             print(f"cholesky fp8 delta={delta}: {run}", file=sys.stderr, flush=True)
         pla.free(Abase, A, B, X)
@@ -421,8 +429,9 @@ def main():
             piv = pla.get_contents(P)[:, 0]
             swap_bytes = interchange_bytes(piv, n, nb_plain.value, 8) * steps
             hbm = [hbm_roofline("panel_swap_kernel<double> (row interchanges outside the panels)", ls, mss,
-                                swap_bytes),
-                   hbm_roofline("trsv_nt_kernel<double> (unit-lower and upper solves of b)", lt, mst, bt)]
+                                swap_bytes, TRAFFIC["swap_f64"] if default_size else None),
+                   hbm_roofline("trsv_nt_kernel<double> (unit-lower and upper solves of b)", lt, mst, bt,
+                                TRAFFIC["trsv_f64"] if default_size else None)]
             pla.free(A)
             eta, xerr = backward_error()
             ms = el / steps * 1e3
@@ -482,10 +491,13 @@ def main():
             pla.call("PLA_Kernel_timing", 0)
             piv = pla.get_contents(P)[:, 0]
             swap_bytes = interchange_bytes(piv, n, nbv.value, 4) * args.steps
-            hbm = [hbm_roofline("resid_partial_kernel (r = b - A x in FP64)", lr, msr, br),
-                   hbm_roofline("trsv_nt_kernel<float> (FP32 L and U solves of the refinement)", lt, mst, bt),
+            dflt = default_size and args.prec == 3
+            hbm = [hbm_roofline("resid_partial_kernel (r = b - A x in FP64)", lr, msr, br,
+                                TRAFFIC["resid_lu"] if dflt else None),
+                   hbm_roofline("trsv_nt_kernel<float> (FP32 L and U solves of the refinement)", lt, mst, bt,
+                                TRAFFIC["trsv_f32"] if dflt else None),
                    hbm_roofline("panel_swap_kernel<float> (row interchanges outside the panels)", ls, mss,
-                                swap_bytes)]
+                                swap_bytes, TRAFFIC["swap_f32"] if dflt else None)]
             pla.call("PLA_Copy", X, x)
             eta, xerr = backward_error()   # recomputed independently (GEMV on A0)
             ms = el / args.steps * 1e3
