@@ -239,9 +239,12 @@ def run_cholesky(pla, s, n_plain, n_fp8, deltas, steps):
 
     if n_plain > 0:
         n = n_plain
+        # the block size, autotuned over {128, 256, 512} on a scratch matrix
+        # (outside the timed region)
+        nbv, tune_ms = ctypes.c_int(), (ctypes.c_double * 4)()
+        pla.call("PLA_Autotune_nb", pla.PLA_NB_CHOL, n, ctypes.byref(nbv), ctypes.cast(tune_ms, ctypes.c_void_p))
         A0 = spd(n, 1e-2)
         A = s.matrix(n, n)
-        nbv = ctypes.c_int()
         pla.call("PLA_Get_device_nb_for", n, pla.PLA_NB_CHOL, ctypes.byref(nbv))
         pla.call("PLA_Copy", A0, A)
         pla.call("PLA_Chol", pla.PLA_LOWER_TRIANGULAR, A)       # warmup
@@ -269,7 +272,9 @@ def run_cholesky(pla, s, n_plain, n_fp8, deltas, steps):
         eta = eta_cols(A0, b, x)
         pla.free(A0, A, ones, b, x, one, zero)
         ms = el / steps * 1e3
-        out["plain_fp64"] = {"n": n, "nb": nbv.value, "value": round(n ** 3 / 3.0 / (ms * 1e-3) / 1e12, 3),
+        out["plain_fp64"] = {"n": n, "nb": nbv.value,
+                             "nb_autotune_ms": {str(c): round(tune_ms[i], 1) for i, c in enumerate((128, 256, 512))},
+                             "value": round(n ** 3 / 3.0 / (ms * 1e-3) / 1e12, 3),
                              "unit": "TFLOP/s", "frac_fp64_peak": round(n ** 3 / 3.0 / (ms * 1e-3) / 1e12 / FP64_MFMA_PEAK, 4),
                              "ms_per_factorization": round(ms, 2), "steps": steps, "roofline": rl,
                              "backward_error_inf": eta, "hpl_scaled_residual": eta / (n * 2.0 ** -52),

@@ -587,6 +587,17 @@ int PLA_Get_device_nb(int n, int *nb);
 #define PLA_NB_CHOL 1
 #define PLA_NB_LU_MIXED 2
 int PLA_Get_device_nb_for(int n, int factorization, int *nb);
+/* EXTENSION -- the nb_alg knob (TEMPLATE/PLA_Environ.c:88-127) tuned on the
+   device: PLA_LU or PLA_Chol (factorization PLA_NB_LU / PLA_NB_CHOL) runs on a
+   scratch n x n matrix (the drand48 input of TEST/create_problem.c; for
+   Cholesky plus n I) once per candidate block size 128, 256, 512 (and 1024
+   for PLA_LU from n = 16384); the fastest becomes the block size
+   PLA_Get_device_nb_for and the factorizations use for order n, for the
+   process lifetime (PLA_Set_device_nb and pla_Environ_set_nb_alg still take
+   precedence).  ms (NULL allowed) receives the four candidates' times in
+   milliseconds, 0 where not tried.  Needs 8 n^2 bytes of device memory for
+   the duration of the call; the caller's objects are not touched. */
+int PLA_Autotune_nb(int factorization, int n, int *best_nb, double *ms);
 /* Per-launch hipEvent timing of the trailing-update kernels on the library
    stream (the reference keeps per-operation timers in PLA_TIMINGS,
    OBJ/PLA_Timings.c:17-125).  role 1: PLA_LU's A22 -= A21 A12 GEMM;

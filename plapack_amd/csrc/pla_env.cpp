@@ -5,6 +5,9 @@
 // TEMPLATE/PLA_Environ.c, OBJ/PLA_Diagnostics.c.
 #include "pla_internal.hpp"
 
+#include <map>
+#include <utility>
+
 #include <dlfcn.h>
 
 #include <cstdarg>
@@ -280,10 +283,18 @@ void release_workspaces() {
   g_leaf = leaf_ws();
 }
 
+// Block sizes measured by PLA_Autotune_nb, per (factorization, n), for the
+// process lifetime.
+static std::map<std::pair<int, int>, int> g_tuned_nb;
+
+void set_tuned_nb(int kind, int n, int nb) { g_tuned_nb[{kind, n}] = nb; }
+
 int choose_nb(int n, int kind) {
   const context &c = g_ctx;
   if (c.nb_device > 0) return c.nb_device;
   if (c.nb_user > 0) return c.nb_user;
+  const auto t = g_tuned_nb.find({kind, n});
+  if (t != g_tuned_nb.end()) return t->second;
   // Measured on MI355X.  Plain FP64 LU: n=32768 nb 256 / 384 / 512 -> 25.8 /
   // 27.1 / 27.9 TF/s; n=65536 nb 512 / 1024 -> 42.6 / 45.1 TF/s (the
   // trailing GEMM runs faster at K = 1024).  Mixed LU n=65536: nb 512 / 1024

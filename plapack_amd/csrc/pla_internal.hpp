@@ -374,6 +374,11 @@ struct chol_scratch {
 // of the first non-positive pivot.  scr == nullptr: scratch from the grow-only
 // workspaces, trailing updates timed when kernel timing is on; otherwise the
 // caller's scratch, no allocation and no timing.
+// Block size of a factorization chosen by PLA_Autotune_nb for order n.
+void set_tuned_nb(int kind, int n, int nb);
+int lu_factor(hipStream_t sMain, int m, int n, double *a, long lda, int *piv, int nb, bool swap_left);
+// a(i, i) += v for i < nd
+int k_shift_diag(hipStream_t s, int nd, double *a, long lda, double v);
 int chol_factor(hipStream_t sM, int n, double *a, long lda, int nb, int *info_out,
                 const chol_scratch *scr = nullptr);
 // Device address of the no-allocation Cholesky's static scratch on `device`

@@ -246,4 +246,74 @@ int PLA_General_solve(PLA_Obj A, PLA_Obj ipiv, PLA_Obj B) {
   return written(B);
 }
 
+// The block-size knob of TEMPLATE/PLA_Environ.c:88-127 (nb_alg), tuned on the
+// device: the factorization runs on a scratch n x n matrix (TEST/create_problem.c's
+// drand48 input; for Cholesky plus n I, TEST/test_chol.c:121-127) once per
+// candidate, and the fastest block size becomes this process's choice for
+// order n (after PLA_Set_device_nb and pla_Environ_set_nb_alg, which still
+// win).  ms (optional) receives the times of the candidates 128, 256, 512, 1024
+// (0 where not tried; 1024 for PLA_LU from n = 16384).  Nothing of the caller's
+// is read or written.
+int PLA_Autotune_nb(int factorization, int n, int *best_nb, double *ms) {
+  if (!best_nb) return fail(PLA_ERR_INVALID_PARAM, "PLA_Autotune_nb: best_nb is NULL");
+  if (factorization != PLA_NB_LU && factorization != PLA_NB_CHOL)
+    return fail(PLA_ERR_INVALID_PARAM, "PLA_Autotune_nb: factorization %d (PLA_NB_LU or PLA_NB_CHOL)",
+                factorization);
+  if (n < 1) return fail(PLA_ERR_DIMENSIONS, "PLA_Autotune_nb: n = %d", n);
+  int rc = ensure_device();
+  if (rc) return rc;
+  const int cand[4] = {128, 256, 512, 1024};
+  const int ncand = (factorization == PLA_NB_LU && n >= 16384) ? 4 : 3;
+  double *a = nullptr;
+  int *piv = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t s = ctx().stream;
+  auto cleanup = [&]() {
+    if (a) (void)hipFree(a);
+    if (piv) (void)hipFree(piv);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  if (hipMalloc(&a, sizeof(double) * (size_t)n * n) != hipSuccess ||
+      hipMalloc(&piv, sizeof(int) * (size_t)n) != hipSuccess) {
+    cleanup();
+    return fail(PLA_ERR_NO_MEMORY, "PLA_Autotune_nb: scratch %d x %d", n, n);
+  }
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    return fail(PLA_ERR_DEVICE, "PLA_Autotune_nb: events");
+  }
+  int best = 0;
+  float best_ms = 0.0f;
+  for (int c = 0; c < 4; c++) {
+    if (ms) ms[c] = 0.0;
+    if (c >= ncand) continue;
+    const int nb = cand[c];
+    // srand48(0): state (0 << 16) | 0x330E
+    if ((rc = k_fill_drand48(s, n, n, a, n, 0x330EULL, 0))) break;
+    if (factorization == PLA_NB_CHOL && (rc = k_shift_diag(s, n, a, n, (double)n))) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(PLA_ERR_DEVICE, "PLA_Autotune_nb: event"); break; }
+    int info = 0;
+    if (factorization == PLA_NB_LU) rc = lu_factor(s, n, n, a, n, piv, nb, true);
+    else rc = chol_factor(s, n, a, n, nb, &info);
+    if (rc) break;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(PLA_ERR_DEVICE, "PLA_Autotune_nb: event");
+      break;
+    }
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    if (ms) ms[c] = t;
+    if (!best || t < best_ms) {
+      best = nb;
+      best_ms = t;
+    }
+  }
+  cleanup();
+  if (rc) return rc;
+  set_tuned_nb(factorization, n, best);
+  *best_nb = best;
+  return PLA_SUCCESS;
+}
+
 }  // extern "C"

@@ -248,6 +248,14 @@ int symv_common(const char *who, bool herm, int uplo, PLA_Obj alpha, PLA_Obj A, 
 }
 
 }  // namespace
+
+int k_shift_diag(hipStream_t s, int nd, double *a, long lda, double v) {
+  if (nd <= 0) return PLA_SUCCESS;
+  hipLaunchKernelGGL(shift_diag_kernel<double>, dim3((nd + 255) / 256), dim3(256), 0, s, nd, a, lda, v);
+  PLA_HIP(hipGetLastError());
+  return PLA_SUCCESS;
+}
+
 }  // namespace pla
 
 using namespace pla;

@@ -189,6 +189,7 @@ _SIGS = {
     "PLA_Set_device_param": [_i, _i],
     "PLA_Get_device_nb": [_i, _pi],
     "PLA_Get_device_nb_for": [_i, _i, _pi],
+    "PLA_Autotune_nb": [_i, _i, _pi, _p],
     "PLA_Kernel_timing": [_i],
     "PLA_Kernel_timing_query": [_i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_d),
                                 ctypes.POINTER(_d)],

@@ -125,6 +125,11 @@ def _shift(A, delta):
 
 def test_plain_chol_n32768(pla_session):
     s, n = pla_session, 32768
+    # the configuration's block size is autotuned over {128, 256, 512}
+    best, ms = ctypes.c_int(), (ctypes.c_double * 4)()
+    pla.call("PLA_Autotune_nb", pla.PLA_NB_CHOL, n, ctypes.byref(best), ctypes.cast(ms, ctypes.c_void_p))
+    _log(f"PLA_Autotune_nb PLA_Chol n={n}: nb 128/256/512 {ms[0]:.0f}/{ms[1]:.0f}/{ms[2]:.0f} ms -> {best.value}")
+    assert best.value in (128, 256, 512)
     A0 = _spd_full(s, n)
     _shift(A0, 1e-2)
     A = s.matrix(n, n)

@@ -403,3 +403,36 @@ def test_lu_block_size_schedule(pla_session, ora, m, n, nb, tail, switch):
     assert np.array_equal(piv, piv_o)
     assert np.abs(lu - lu_o).max() <= 1e-7 * np.abs(a).max()
     assert ora.lu_scaled_residual(a, lu, piv) < 16
+
+
+@pytest.mark.parametrize("fact,n", [(pla.PLA_NB_LU, 1777), (pla.PLA_NB_CHOL, 1901)])
+def test_autotune_nb(pla_session, ora, fact, n):
+    """PLA_Autotune_nb times the factorization for each candidate block size
+    on a scratch matrix, and the winner becomes the block size the device
+    uses for that order; the factorization at that order is then still the
+    oracle's (pivots, Cholesky values)."""
+    s = pla_session
+    best = ctypes.c_int()
+    ms = (ctypes.c_double * 4)()
+    pla.call("PLA_Autotune_nb", fact, n, ctypes.byref(best), ctypes.cast(ms, ctypes.c_void_p))
+    assert best.value in (128, 256, 512)
+    assert all(ms[i] > 0 for i in range(3)) and ms[3] == 0.0
+    assert ms[[128, 256, 512].index(best.value)] == min(ms[:3])
+    nb = ctypes.c_int()
+    pla.call("PLA_Get_device_nb_for", n, fact, ctypes.byref(nb))
+    assert nb.value == best.value
+    a = ora.uniform(n, n, 11)
+    if fact == pla.PLA_NB_LU:
+        lu, piv = _lu_gpu(s, a=a)
+        lu_o, piv_o = ora.lu_blocked(a, nb=best.value)
+        assert np.array_equal(piv, piv_o)
+    else:
+        spd = np.tril(a) + n * np.eye(n)
+        A = pla.from_numpy(s, spd)
+        assert pla.call("PLA_Chol", pla.PLA_LOWER_TRIANGULAR, A, ok=(0,)) == 0
+        L = np.tril(pla.get_contents(A))
+        pla.free(A)
+        sym = np.tril(spd) + np.tril(spd, -1).T
+        assert np.abs(L - np.linalg.cholesky(sym)).max() <= 1e-13 * np.abs(sym).max()
+    with pytest.raises(pla.PLAError):
+        pla.call("PLA_Autotune_nb", pla.PLA_NB_LU_MIXED, n, ctypes.byref(best), None)
