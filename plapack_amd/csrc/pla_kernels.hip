@@ -109,9 +109,11 @@ __device__ __forceinline__ void gemm_load_c(int m, int n, double beta, double sg
       for (int r = 0; r < 4; r++) {
         const int gi = i0 + wm * 64 + bi * 16 + l16;
         const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
-        double v = 0.0;
-        if (beta != 0.0 && gi < m && gj < n && tri_in<TRI>(gi, gj)) v = beta * C[(long)gj * ldc + gi];
-        acc[bi][bj][r] = sgn * v;
+        // read at the clamped position (always inside C) and select: an
+        // unguarded load does not wait for its data before the next is issued
+        const double c = C[(long)min(gj, n - 1) * ldc + min(gi, m - 1)];
+        const bool in = beta != 0.0 && gi < m && gj < n && tri_in<TRI>(gi, gj);
+        acc[bi][bj][r] = in ? sgn * beta * c : 0.0;
       }
 }
 
@@ -130,6 +132,59 @@ __device__ __forceinline__ void gemm_store_c(int m, int n, double sgn, double *_
         const int gj = j0 + wn * 64 + bj * 16 + lq + 4 * r;
         if (gi < m && gj < n && tri_in<TRI>(gi, gj)) C[(long)gj * ldc + gi] = sgn * acc[bi][bj][r];
       }
+}
+
+// The same for a tile wholly inside C.  The loads carry no per-element
+// condition, so all 64 are in flight together (a guarded load is followed
+// by its own wait); elements outside the TRI triangle are read but never
+// stored back.
+template <int TRI>
+__device__ __forceinline__ void gemm_load_c_full(double beta, double sgn, const double *__restrict__ C, long ldc,
+                                                 int i0, int j0, dbl4 (&acc)[4][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
+  if (beta == 0.0) {
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++) acc[bi][bj] = dbl4{0.0, 0.0, 0.0, 0.0};
+    return;
+  }
+  const double f = sgn * beta;
+  const double *cw = C + (long)(j0 + wn * 64 + lq) * ldc + i0 + wm * 64 + l16;
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[bi][bj][r] = f * cw[(long)(bj * 16 + 4 * r) * ldc + bi * 16];
+}
+
+template <int TRI>
+__device__ __forceinline__ void gemm_store_c_full(double sgn, double *__restrict__ C, long ldc, int i0, int j0,
+                                                  const dbl4 (&acc)[4][4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lq = lane >> 4;
+  double *cw = C + (long)(j0 + wn * 64 + lq) * ldc + i0 + wm * 64 + l16;
+  // a tile of the triangle's interior needs no per-element test
+  const bool whole = TRI == 0 || (TRI == 1 ? i0 >= j0 + GBN - 1 : j0 >= i0 + GBM - 1);
+  if (whole) {
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) cw[(long)(bj * 16 + 4 * r) * ldc + bi * 16] = sgn * acc[bi][bj][r];
+    return;
+  }
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (tri_in<TRI>(i0 + wm * 64 + bi * 16 + l16, j0 + wn * 64 + bj * 16 + lq + 4 * r))
+          cw[(long)(bj * 16 + 4 * r) * ldc + bi * 16] = sgn * acc[bi][bj][r];
 }
 
 // Interior tile, alpha = sgn = +-1.  Operand tiles are 128 (index) x 16
@@ -208,7 +263,7 @@ __device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int 
     if (nkf > 0) load(0);
     else load_tail(0);
   }
-  gemm_load_c<TRI>(m, n, beta, sgn, C, ldc, i0, j0, acc);
+  gemm_load_c_full<TRI>(beta, sgn, C, ldc, i0, j0, acc);
   if (nk > 0) store(0);
   __syncthreads();
   int kt = 0;
@@ -251,7 +306,7 @@ __device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int 
     kt++;
   }
   if (nk > 0) gemm_mfma_step(sm, kt & 1, wm, wn, l16, lq, acc);
-  gemm_store_c<TRI>(m, n, sgn, C, ldc, i0, j0, acc);
+  gemm_store_c_full<TRI>(sgn, C, ldc, i0, j0, acc);
 }
 
 // Any tile, any alpha: element loads with bounds checks, A scaled by alpha.
@@ -274,9 +329,9 @@ __device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, i
       if (!TA) { i = idx & 127; kk = idx >> 7; }
       else { kk = idx & 15; i = idx >> 4; }
       const int gi = i0 + i, gk = kb + kk;
-      double v = 0.0;
-      if (gi < m && gk < k) v = TA ? A[(long)gi * lda + gk] : A[(long)gk * lda + gi];
-      ra[e] = v;
+      const int ci = min(gi, m - 1), ck = min(gk, k - 1);
+      const double v = TA ? A[(long)ci * lda + ck] : A[(long)ck * lda + ci];
+      ra[e] = (gi < m && gk < k) ? v : 0.0;
     }
 #pragma unroll
     for (int e = 0; e < 8; e++) {
@@ -285,9 +340,9 @@ __device__ __forceinline__ void gemm_tile_checked(gemm_smem &sm, int m, int n, i
       if (!TB) { kk = idx & 15; j = idx >> 4; }
       else { j = idx & 127; kk = idx >> 7; }
       const int gj = j0 + j, gk = kb + kk;
-      double v = 0.0;
-      if (gj < n && gk < k) v = TB ? B[(long)gk * ldb + gj] : B[(long)gj * ldb + gk];
-      rb[e] = v;
+      const int cj = min(gj, n - 1), ck = min(gk, k - 1);
+      const double v = TB ? B[(long)ck * ldb + cj] : B[(long)cj * ldb + ck];
+      rb[e] = (gj < n && gk < k) ? v : 0.0;
     }
   };
   auto store_tile = [&](int buf) {
@@ -414,20 +469,31 @@ __global__ __launch_bounds__(256) void dgemm_tall_kernel(int m, int n, int k, do
   __shared__ double Bs[TS_K][TS_N + 1];  // Bs[kk][j] = alpha op(B)(kk, j), zero outside k x n
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, lq = lane >> 4;
-  for (int e = tid; e < TS_K * TS_N; e += 256) {
+  // Every load below reads a clamped position (inside the operand: k, n >= 1)
+  // and selects zero outside it.  Unguarded loads are all in flight at once;
+  // a guarded load would wait for its data before the next one issued.
+#pragma unroll
+  for (int e0 = 0; e0 < TS_K * TS_N; e0 += 256) {
+    const int e = e0 + tid;
     const int kk = e & 63, j = e >> 6;  // consecutive threads: consecutive depth (untransposed B is depth-contiguous)
-    double v = 0.0;
-    if (kk < k && j < n) v = alpha * (TB ? B[(long)kk * ldb + j] : B[(long)j * ldb + kk]);
-    Bs[kk][j] = v;
+    const int ck = min(kk, k - 1), cj = min(j, n - 1);
+    const double v = TB ? B[(long)ck * ldb + cj] : B[(long)cj * ldb + ck];
+    Bs[kk][j] = (kk < k && j < n) ? alpha * v : 0.0;
   }
   const int row = blockIdx.x * 64 + wave * 16 + l16;
   const bool rok = row < m;
+  const int crow = min(row, m - 1);
   const int nkq = (k + 3) >> 2, njt = (n + 15) >> 4;
   double a[TS_K / 4];
 #pragma unroll
   for (int q = 0; q < TS_K / 4; q++) {
     const int kk = 4 * q + lq;
-    a[q] = (q < nkq && rok && kk < k) ? A[(long)kk * lda + row] : 0.0;
+    if (q < nkq) {  // uniform
+      const double v = A[(long)min(kk, k - 1) * lda + crow];
+      a[q] = (rok && kk < k) ? v : 0.0;
+    } else {
+      a[q] = 0.0;
+    }
   }
   dbl4 acc[TS_N / 16];
 #pragma unroll
@@ -435,7 +501,12 @@ __global__ __launch_bounds__(256) void dgemm_tall_kernel(int m, int n, int k, do
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int col = 16 * t + lq + 4 * r;
-      acc[t][r] = (beta != 0.0 && rok && col < n) ? beta * C[(long)col * ldc + row] : 0.0;
+      if (beta != 0.0 && t < njt) {  // uniform
+        const double c = C[(long)min(col, n - 1) * ldc + crow];
+        acc[t][r] = (rok && col < n) ? beta * c : 0.0;
+      } else {
+        acc[t][r] = 0.0;
+      }
     }
   __syncthreads();
 #pragma unroll
@@ -868,23 +939,33 @@ __global__ __launch_bounds__(256) void panel_swap_kernel(T *__restrict__ A, long
                                                          const int *__restrict__ comp, const int *__restrict__ err) {
   if (ld_dev(err)) return;  // a bounded wait gave up: the list is not trustworthy
   const int cnt = min(comp[0], COMP_MAX);
+  if (cnt <= 0) return;
   const int total = nleft + (c2 - c1);
   const int lane = threadIdx.x & 63;
   constexpr int PER = COMP_MAX / 64;
+  const int chunks = (cnt + 63) >> 6;  // uniform: chunks of 64 pairs in use
+  // The list is the same for every column: each lane keeps its (dst, src)
+  // entries in registers.  Lanes past the list's end repeat its last entry,
+  // so every load is unguarded (a guarded load waits for its data before
+  // the next one issues) and only the stores test the position.
+  int dst[PER], src[PER];
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    const int p = min(lane + 64 * i, cnt - 1);
+    dst[i] = comp[1 + 2 * p];
+    src[i] = comp[2 + 2 * p];
+  }
   for (int q = blockIdx.x * 4 + (threadIdx.x >> 6); q < total; q += gridDim.x * 4) {
     const int col = q < nleft ? q : c1 + (q - nleft);
     T *cp = A + (long)col * lda;
     T v[PER];
+    // every source is read before any destination is written
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int p = lane + 64 * i;
-      if (p < cnt) v[i] = cp[comp[2 + 2 * p]];
-    }
+    for (int i = 0; i < PER; i++)
+      if (i < chunks) v[i] = cp[src[i]];
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int p = lane + 64 * i;
-      if (p < cnt) cp[comp[1 + 2 * p]] = v[i];
-    }
+    for (int i = 0; i < PER; i++)
+      if (i < chunks && lane + 64 * i < cnt) cp[dst[i]] = v[i];
   }
 }
 

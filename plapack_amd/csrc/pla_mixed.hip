@@ -125,20 +125,23 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
   // Thread tid moves 16-byte chunks tid and tid + 256 of each 8 KB tile; the
   // chunk's LDS slot is its position in the tile.
   V ra[NS][2], rb[NS][2];
-  const V zero = V{};
   auto load_tile = [&](int kt) {
 #pragma unroll
     for (int e = 0; e < 2; e++) {
       const int c = tid + 256 * e, row = c >> 2;
-      const int gi = i0 + row, gj = j0 + row;
-      const long ao = (long)kt * sa + (long)i0 * 32 + c * 8;
-      const long bo = (long)kt * sb + (boff + j0) * 32 + c * 8;
+      // Rows (columns) past the edge read the last one instead: they feed
+      // only rows (columns) of the product that are never stored, and the
+      // loads carry no condition, so they stay in flight during the MFMAs
+      // (a guarded or selected load is waited for where it is issued).
+      const int gi = min(i0 + row, m - 1), gj = min(j0 + row, n - 1);
+      const long ao = (long)kt * sa + (long)gi * 32 + (c & 3) * 8;
+      const long bo = (long)kt * sb + (boff + gj) * 32 + (c & 3) * 8;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         const H *Ap = s ? Al : Ah;
         const H *Bp = s ? Bl : Bh;
-        ra[s][e] = gi < m ? *(const V *)(Ap + ao) : zero;
-        rb[s][e] = gj < n ? *(const V *)(Bp + bo) : zero;
+        ra[s][e] = *(const V *)(Ap + ao);
+        rb[s][e] = *(const V *)(Bp + bo);
       }
     }
   };
@@ -187,6 +190,22 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
     __syncthreads();
   }
 
+  if (vec_ok && i0 + HBM <= m && j0 + HBN <= n) {
+    // a whole tile: all 16 loads of C in flight together
+    f4v cv[4][4];
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+        cv[bi][bj] = *(const f4v *)(C + (long)(j0 + wn * 64 + bj * 16 + l16) * ldc + i0 + wm * 64 + bi * 16 + lq * 4);
+#pragma unroll
+    for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+      for (int bj = 0; bj < 4; bj++)
+        *(f4v *)(C + (long)(j0 + wn * 64 + bj * 16 + l16) * ldc + i0 + wm * 64 + bi * 16 + lq * 4) =
+            cv[bi][bj] - acc[bi][bj];
+    return;
+  }
 #pragma unroll
   for (int bi = 0; bi < 4; bi++)
 #pragma unroll
