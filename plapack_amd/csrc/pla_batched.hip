@@ -173,8 +173,10 @@ __device__ int factor_system(const bsys &d, const batched_args &a, LT *L, LT *P,
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               const int il = (rtb + 4 * q) * 16 + l16, jl = c * 16 + M::row(lane, r);
-              if (il < R && jl < w && il >= jl)
-                P[jl * RS + il] = (LT)ld_l2(Ad + (long)(k0 + jl) * ld + k0 + il) - acc[q][c][r];
+              // read at the clamped position (inside the block column), so
+              // the loads carry no condition and are all in flight together
+              const double av = ld_l2(Ad + (long)(k0 + min(jl, w - 1)) * ld + k0 + min(il, R - 1));
+              if (il < R && jl < w && il >= jl) P[jl * RS + il] = (LT)av - acc[q][c][r];
             }
         }
     }

@@ -33,10 +33,18 @@ __global__ __launch_bounds__(256) void fp8_quantize_kernel(int R, int b, const d
   const int r0 = t * QT, rows = min(QT, R - r0);
   const int c0 = kt * QT, cols = max(0, min(QT, b - c0));
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // Thread tid reads row min(tid, rows - 1) of the tile (rows >= 1, cols >= 1:
+  // the grid covers ceil(b / QT) column tiles); clamped rows and columns
+  // repeat an element already counted, so every load is unguarded and eight
+  // are in flight at a time (a guarded load is waited for where it issues).
+  const double *pt = P + (long)c0 * ldp + r0 + min(tid, rows - 1);
   double m = 0.0;
-  for (int e = tid; e < rows * cols; e += 256) {
-    const int j = e / rows, i = e - j * rows;
-    m = fmax(m, fabs(P[(long)(c0 + j) * ldp + r0 + i]));
+  for (int j = 0; j < cols; j += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = pt[(long)min(j + u, cols - 1) * ldp];
+#pragma unroll
+    for (int u = 0; u < 8; u++) m = fmax(m, fabs(x[u]));
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
@@ -55,10 +63,14 @@ __global__ __launch_bounds__(256) void fp8_quantize_kernel(int R, int b, const d
   if (tid < rows) {
     const int r = r0 + tid;
     int *dst = (int *)(Lq + (size_t)r * kq + c0);
+#pragma unroll 4
     for (int k = 0; k < QT; k += 4) {
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) x[u] = pt[(long)min(k + u, cols - 1) * ldp];
       float v[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) v[u] = (k + u < cols) ? (float)(P[(long)(c0 + k + u) * ldp + r] * sc) : 0.0f;
+      for (int u = 0; u < 4; u++) v[u] = (k + u < cols) ? (float)(x[u] * sc) : 0.0f;
       int w = 0;
       w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], w, false);
       w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);

@@ -142,9 +142,18 @@ __global__ __launch_bounds__(256) void mdot_partial_kernel(int n, int nrhs, cons
   for (int q = 0; q < NR; q++) s[q] = 0.0;
   const int r1 = min(n, (c + 1) * DCH);
   for (int r = c * DCH + threadIdx.x; r < r1; r += 256) {
+    // all loads first, then the products: a load whose use sits in the same
+    // guarded block is waited for before the next one issues
+    double v[NR], w[NR];
 #pragma unroll
     for (int q = 0; q < NR; q++)
-      if (q < nrhs) s[q] = fma(Vi[(long)q * ldv + r], W[(long)q * ldw + r], s[q]);
+      if (q < nrhs) {
+        v[q] = Vi[(long)q * ldv + r];
+        w[q] = W[(long)q * ldw + r];
+      }
+#pragma unroll
+    for (int q = 0; q < NR; q++)
+      if (q < nrhs) s[q] = fma(v[q], w[q], s[q]);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -178,6 +187,7 @@ __global__ __launch_bounds__(256) void maxpy_kernel(int n, int nrhs, int k, cons
   const int q = blockIdx.y;
   if (r >= n) return;
   double s = 0.0;
+#pragma unroll 8
   for (int i = 0; i < k; i++) s = fma(h[i * nrhs + q], V[(size_t)i * vstride + (long)q * ldv + r], s);
   W[(long)q * ldw + r] += alpha * s;
 }

@@ -237,9 +237,13 @@ __global__ __launch_bounds__(256) void cvt2d_kernel(int m, int n, const S *__res
     const S *a = A + (long)j * lda;
     D *b = B + (long)j * ldb;
     for (int i0 = (blockIdx.x * 256 + threadIdx.x) * 4; i0 < m; i0 += gridDim.x * 1024) {
+      // unguarded loads (clamped to the column's last row): all four in flight
+      S x[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) x[r] = a[min(i0 + r, m - 1)];
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (i0 + r < m) b[i0 + r] = (D)a[i0 + r];
+        if (i0 + r < m) b[i0 + r] = (D)x[r];
     }
   }
 }
@@ -266,12 +270,14 @@ template <typename H, bool SPLIT>
 __global__ __launch_bounds__(256) void panel_out_kernel(int rows, int b, int kpad, const double *__restrict__ P,
                                                         long ldp, H *__restrict__ Lh, H *__restrict__ Ll, long sl) {
   const long per_slice = (long)rows * 32, total = per_slice * (kpad / 32);
+#pragma unroll 4
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
     const int t = (int)(e / per_slice);
     const long w = e - (long)t * per_slice;
     const int i = (int)(w >> 5), pos = (int)(w & 31);
     const int c = t * 32 + ((((pos >> 3) ^ ((i >> 1) & 3))) << 3) + (pos & 7);
-    const float v = c < b ? (float)P[(long)c * ldp + i] : 0.f;
+    const double x = P[(long)min(c, b - 1) * ldp + i];  // unguarded (b >= 1)
+    const float v = c < b ? (float)x : 0.f;
     to_low<H, SPLIT>(v, Lh, Ll, (long)t * sl + w);
   }
 }
@@ -284,13 +290,12 @@ __global__ __launch_bounds__(256) void upanel_out_kernel(int b, int kpad, int nr
                                                          long ldu, float *__restrict__ W, long ldw,
                                                          H *__restrict__ Uh, H *__restrict__ Ul, long su, long col0) {
   const long total = (long)nr * kpad;
+#pragma unroll 4
   for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
     const int j = (int)(e / kpad), r = (int)(e - (long)j * kpad);
-    float v = 0.f;
-    if (r < b) {
-      v = (float)U[(long)j * ldu + r];
-      W[(long)j * ldw + r] = v;
-    }
+    const double x = U[(long)j * ldu + min(r, b - 1)];  // unguarded (b >= 1)
+    const float v = r < b ? (float)x : 0.f;
+    if (r < b) W[(long)j * ldw + r] = v;
     to_low<H, SPLIT>(v, Uh, Ul, kslice_off(col0 + j, r, su));
   }
 }
@@ -801,7 +806,7 @@ static bool is_split(int prec) { return prec == PLA_MIXED_FP16X3 || prec == PLA_
 int k_panel_out(hipStream_t s, int prec, int rows, int b, int kpad, const double *P, long ldp, void *Lh, void *Ll,
                 long sl) {
   if (rows <= 0) return PLA_SUCCESS;
-  if (kpad % 32 || kpad < b || sl < (long)rows * 32) return fail(PLA_ERR_INVALID_PARAM, "panel_out: layout");
+  if (b < 1 || kpad % 32 || kpad < b || sl < (long)rows * 32) return fail(PLA_ERR_INVALID_PARAM, "panel_out: layout");
   dim3 grid(grid1((long)rows * kpad)), blk(256);
 #define PLA_PO(H, SP) \
   hipLaunchKernelGGL((panel_out_kernel<H, SP>), grid, blk, 0, s, rows, b, kpad, P, ldp, (H *)Lh, (H *)Ll, sl)
@@ -818,7 +823,7 @@ int k_panel_out(hipStream_t s, int prec, int rows, int b, int kpad, const double
 int k_upanel_out(hipStream_t s, int prec, int b, int kpad, int nr, const double *U, long ldu, float *W, long ldw,
                  void *Uh, void *Ul, long su, long col0) {
   if (nr <= 0) return PLA_SUCCESS;
-  if (kpad % 32 || kpad < b || su < (col0 + nr) * 32) return fail(PLA_ERR_INVALID_PARAM, "upanel_out: layout");
+  if (b < 1 || kpad % 32 || kpad < b || su < (col0 + nr) * 32) return fail(PLA_ERR_INVALID_PARAM, "upanel_out: layout");
   dim3 grid(grid1((long)nr * kpad)), blk(256);
 #define PLA_UO(H, SP)                                                                                           \
   hipLaunchKernelGGL((upanel_out_kernel<H, SP>), grid, blk, 0, s, b, kpad, nr, U, ldu, W, ldw, (H *)Uh, (H *)Ul, \
