@@ -128,18 +128,25 @@ __device__ int factor_system(const bsys &d, const batched_args &a, LT *L, LT *P,
         constexpr int U = 4;
         LT xa0[U][BQ], yb0[U][CT], xa1[U][BQ], yb1[U][CT];
         auto load = [&](int p0, LT (&xa)[U][BQ], LT (&yb)[U][CT]) {
+          // every load from a clamped position with no condition, the
+          // selects after them: a guarded or selected load is waited for
+          // where it issues
 #pragma unroll
           for (int u = 0; u < U; u++) {
             const LT *Lc = L + (long)(p0 + 4 * u + (lane >> 4)) * ld;
 #pragma unroll
-            for (int c = 0; c < CT; c++) {
-              const int j = k0 + c * 16 + l16;
-              yb[u][c] = j < k0 + w ? Lc[j] : (LT)0;
-            }
+            for (int c = 0; c < CT; c++) yb[u][c] = Lc[min(k0 + c * 16 + l16, k0 + w - 1)];
+#pragma unroll
+            for (int q = 0; q < BQ; q++) xa[u][q] = Lc[min(k0 + (rtb + 4 * q) * 16 + l16, n - 1)];
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int c = 0; c < CT; c++) yb[u][c] = c * 16 + l16 < w ? yb[u][c] : (LT)0;
 #pragma unroll
             for (int q = 0; q < BQ; q++) {
               const int i = k0 + (rtb + 4 * q) * 16 + l16;
-              xa[u][q] = (q < Q && i < n) ? Lc[i] : (LT)0;
+              xa[u][q] = (q < Q && i < n) ? xa[u][q] : (LT)0;
             }
           }
         };
@@ -168,15 +175,23 @@ __device__ int factor_system(const bsys &d, const batched_args &a, LT *L, LT *P,
 #pragma unroll
       for (int q = 0; q < BQ; q++)
         if (q < Q) {
+          // the tile's A entries read first, at clamped positions inside the
+          // block column (no condition, so all are in flight together), then
+          // the guarded stores
+          double av[CT][4];
 #pragma unroll
           for (int c = 0; c < CT; c++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               const int il = (rtb + 4 * q) * 16 + l16, jl = c * 16 + M::row(lane, r);
-              // read at the clamped position (inside the block column), so
-              // the loads carry no condition and are all in flight together
-              const double av = ld_l2(Ad + (long)(k0 + min(jl, w - 1)) * ld + k0 + min(il, R - 1));
-              if (il < R && jl < w && il >= jl) P[jl * RS + il] = (LT)av - acc[q][c][r];
+              av[c][r] = ld_l2(Ad + (long)(k0 + min(jl, w - 1)) * ld + k0 + min(il, R - 1));
+            }
+#pragma unroll
+          for (int c = 0; c < CT; c++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int il = (rtb + 4 * q) * 16 + l16, jl = c * 16 + M::row(lane, r);
+              if (il < R && jl < w && il >= jl) P[jl * RS + il] = (LT)av[c][r] - acc[q][c][r];
             }
         }
     }
@@ -208,7 +223,7 @@ __device__ int factor_system(const bsys &d, const batched_args &a, LT *L, LT *P,
 // y <- (L L^T)^{-1} y for nrhs right-hand sides held in LDS (y[r*RS + i]);
 // arithmetic in FP64 whatever L's storage type.
 template <typename LT>
-__device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
+__device__ __forceinline__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long ld = n;
   const int nwv = (nrhs + 3) >> 2;  // waves busy with the 16-row diagonal solves
@@ -220,9 +235,11 @@ __device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
       const bool act = r < nrhs && i < jb;
       double v = act ? y[r * RS + j0 + i] : 0.0;
       double lrow[16];  // row i of the diagonal block, loaded before the dependent chain
+      const int ic = min(i, jb - 1);
 #pragma unroll
-      for (int jj = 0; jj < 16; jj++)
-        lrow[jj] = (act && jj <= i) ? (double)L[(long)(j0 + jj) * ld + j0 + i] : 1.0;
+      for (int jj = 0; jj < 16; jj++) lrow[jj] = (double)L[(long)(j0 + min(jj, ic)) * ld + j0 + ic];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) lrow[jj] = (act && jj <= i) ? lrow[jj] : 1.0;
 #pragma unroll
       for (int jj = 0; jj < 16; jj++) {
         if (jj < jb) {
@@ -238,8 +255,15 @@ __device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
       double s[MAX_RHS];
 #pragma unroll
       for (int r = 0; r < MAX_RHS; r++) s[r] = 0.0;
-      for (int jj = 0; jj < jb; jj++) {
-        const double l = (double)L[(long)(j0 + jj) * ld + i];
+      // the block's 16 entries of row i loaded together (clamped), then
+      // used in column order
+      LT lv[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) lv[jj] = L[(long)(j0 + min(jj, jb - 1)) * ld + i];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) {
+        if (jj >= jb) break;
+        const double l = (double)lv[jj];
 #pragma unroll
         for (int r = 0; r < MAX_RHS; r++)
           if (r < nrhs) s[r] += l * y[r * RS + j0 + jj];
@@ -258,9 +282,11 @@ __device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
       const bool act = r < nrhs && i < jb;
       double v = act ? y[r * RS + j0 + i] : 0.0;
       double lcol[16];  // column i of the diagonal block (row i of its transpose)
+      const int ic = min(i, jb - 1);
 #pragma unroll
-      for (int jj = 0; jj < 16; jj++)
-        lcol[jj] = (act && jj >= i && jj < jb) ? (double)L[(long)(j0 + i) * ld + j0 + jj] : 1.0;
+      for (int jj = 0; jj < 16; jj++) lcol[jj] = (double)L[(long)(j0 + ic) * ld + j0 + max(ic, min(jj, jb - 1))];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) lcol[jj] = (act && jj >= i && jj < jb) ? lcol[jj] : 1.0;
 #pragma unroll
       for (int jj = 15; jj >= 0; jj--) {
         if (jj < jb) {
@@ -277,8 +303,13 @@ __device__ void solve_system(const LT *L, int n, int nrhs, double *y, int RS) {
 #pragma unroll
       for (int r = 0; r < MAX_RHS; r++) s[r] = 0.0;
       const LT *Lr = L + (long)i * ld + j0;  // L[j0 + jj][i], jj < jb
-      for (int jj = 0; jj < jb; jj++) {
-        const double l = (double)Lr[jj];
+      LT lv[16];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) lv[jj] = Lr[min(jj, jb - 1)];
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) {
+        if (jj >= jb) break;
+        const double l = (double)lv[jj];
 #pragma unroll
         for (int r = 0; r < MAX_RHS; r++)
           if (r < nrhs) s[r] += l * y[r * RS + j0 + jj];
@@ -306,7 +337,7 @@ __device__ inline double wave_max(double v) {
 // on the first call also rowsum[i] = sum_j |A[i][j]|.  Returns the normwise
 // backward error max_r ||r||_inf / (||A||_inf ||x_r||_inf + ||b_r||_inf) in
 // every thread.
-__device__ double residual_system(const double *Ad, const double *Bd, int n, int nrhs, const double *x,
+__device__ __forceinline__ double residual_system(const double *Ad, const double *Bd, int n, int nrhs, const double *x,
                                   double *rsd, double *rowsum, bool first, double *anorm, int RS,
                                   double *red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -326,18 +357,28 @@ __device__ double residual_system(const double *Ad, const double *Bd, int n, int
       dot[r] = 0.0;
     }
     double asum = 0.0;
-    for (int i = j + lane; i < n; i += 64) {
-      const double aij = Acol[i];
-      if (i > j) {
+    // four rows' loads in flight per lane (clamped to the column), each
+    // row's products applied in row order as before
+    for (int i0 = j + lane; i0 < n; i0 += 256) {
+      double a4[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) a4[u] = Acol[min(i0 + 64 * u, n - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int i = i0 + 64 * u;
+        if (i >= n) break;
+        const double aij = a4[u];
+        if (i > j) {
+#pragma unroll
+          for (int r = 0; r < MAX_RHS; r++)
+            if (r < nrhs) atomicAdd(&rsd[r * RS + i], -aij * xj[r]);
+          if (first) atomicAdd(&rowsum[i], fabs(aij));
+        }
 #pragma unroll
         for (int r = 0; r < MAX_RHS; r++)
-          if (r < nrhs) atomicAdd(&rsd[r * RS + i], -aij * xj[r]);
-        if (first) atomicAdd(&rowsum[i], fabs(aij));
+          if (r < nrhs) dot[r] += aij * x[r * RS + i];
+        asum += fabs(aij);
       }
-#pragma unroll
-      for (int r = 0; r < MAX_RHS; r++)
-        if (r < nrhs) dot[r] += aij * x[r * RS + i];
-      asum += fabs(aij);
     }
 #pragma unroll
     for (int r = 0; r < MAX_RHS; r++)

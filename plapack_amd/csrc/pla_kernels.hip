@@ -484,29 +484,24 @@ __global__ __launch_bounds__(256) void dgemm_tall_kernel(int m, int n, int k, do
   const bool rok = row < m;
   const int crow = min(row, m - 1);
   const int nkq = (k + 3) >> 2, njt = (n + 15) >> 4;
+  // all loads first, with no condition at all (the selects come after, so
+  // no load waits for its data before the next one issues)
   double a[TS_K / 4];
 #pragma unroll
-  for (int q = 0; q < TS_K / 4; q++) {
-    const int kk = 4 * q + lq;
-    if (q < nkq) {  // uniform
-      const double v = A[(long)min(kk, k - 1) * lda + crow];
-      a[q] = (rok && kk < k) ? v : 0.0;
-    } else {
-      a[q] = 0.0;
-    }
-  }
+  for (int q = 0; q < TS_K / 4; q++) a[q] = A[(long)min(4 * q + lq, k - 1) * lda + crow];
   dbl4 acc[TS_N / 16];
+#pragma unroll
+  for (int t = 0; t < TS_N / 16; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[t][r] = C[(long)min(16 * t + lq + 4 * r, n - 1) * ldc + crow];
+#pragma unroll
+  for (int q = 0; q < TS_K / 4; q++) a[q] = (rok && 4 * q + lq < k) ? a[q] : 0.0;
 #pragma unroll
   for (int t = 0; t < TS_N / 16; t++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int col = 16 * t + lq + 4 * r;
-      if (beta != 0.0 && t < njt) {  // uniform
-        const double c = C[(long)min(col, n - 1) * ldc + crow];
-        acc[t][r] = (rok && col < n) ? beta * c : 0.0;
-      } else {
-        acc[t][r] = 0.0;
-      }
+      acc[t][r] = (beta != 0.0 && rok && col < n) ? beta * acc[t][r] : 0.0;
     }
   __syncthreads();
 #pragma unroll

@@ -82,11 +82,16 @@ __global__ __launch_bounds__(256) void sresid_t_kernel(int n, int nrhs, const do
 #pragma unroll
     for (int q = 0; q < NR; q++) s[q] = 0.0;
     const double *col = A + (long)i * lda;
+    // every load unguarded (clamped into the chunk), then selected: a
+    // guarded load would be waited for before the next one issued
+    double av[SCH / 64];
+#pragma unroll
+    for (int t = 0; t < SCH / 64; t++) av[t] = col[min(j0 + t * 64 + lane, j1 - 1)];
 #pragma unroll
     for (int t = 0; t < SCH / 64; t++) {
       const int j = j0 + t * 64 + lane;
       if (j < j1 && j > i) {
-        const double a = ABS ? fabs(col[j]) : col[j];
+        const double a = ABS ? fabs(av[t]) : av[t];
 #pragma unroll
         for (int q = 0; q < NR; q++) s[q] = fma(a, xs[j - j0][q], s[q]);
       }

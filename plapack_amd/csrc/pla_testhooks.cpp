@@ -85,9 +85,10 @@ int pla_th_leaf_fallbacks(unsigned long long *out) {
   return PLA_SUCCESS;
 }
 
-// Poll limit of the triangular-solve block waits (0 restores the default).
-int pla_th_set_trsv_spin_limit(unsigned limit) {
-  ctx().trsv_spin_limit = limit ? limit : 2000000u;
+// on: every triangular-solve block that has to wait gives up at its first
+// wait (poll limit 0), whatever the timing; off: the default limit.
+int pla_th_trsv_give_up_at_first_wait(int on) {
+  ctx().trsv_spin_limit = on ? 0u : 2000000u;
   return PLA_SUCCESS;
 }
 

@@ -342,13 +342,12 @@ def test_lu_exchange_timeout_falls_back(pla_session, ora, kind):
 @pytest.mark.parametrize("entry", ["PLA_Trsv", "PLA_General_solve", "PLA_Pos_def_solve"])
 def test_triangular_solve_timeout_reports_error(pla_session, ora, entry):
     """A triangular-solve block wait that gives up (forced from the host with
-    a one-poll limit: a block gives up at the first poll that finds a
-    preceding block's solution unpublished, which along a chain of 128 blocks
-    happens to the later ones) makes the entry point return PLA_ERR_DEVICE
-    instead of a silently wrong solution; the error word is cleared, so the
-    next solve and the next LU are correct."""
+    a poll limit of zero: every block that has to wait for a preceding block
+    gives up at its first wait, whatever the timing) makes the entry point
+    return PLA_ERR_DEVICE instead of a silently wrong solution; the error word
+    is cleared, so the next solve and the next LU are correct."""
     hooks = ctypes.CDLL(os.path.join(os.path.dirname(pla.LIB_PATH), "libpla_testhooks.so"))
-    hooks.pla_th_set_trsv_spin_limit.argtypes = [ctypes.c_uint]
+    hooks.pla_th_trsv_give_up_at_first_wait.argtypes = [ctypes.c_int]
     s = pla_session
     n = 8192
     a = ora.uniform(n, n, 6)
@@ -372,13 +371,13 @@ def test_triangular_solve_timeout_reports_error(pla_session, ora, entry):
         finally:
             pla.free(*objs)
 
-    hooks.pla_th_set_trsv_spin_limit(1)
+    hooks.pla_th_trsv_give_up_at_first_wait(1)
     try:
         with pytest.raises(pla.PLAError) as e:
             run()
         assert e.value.code == pla.PLA_ERR_DEVICE and "timed out" in str(e.value)
     finally:
-        hooks.pla_th_set_trsv_spin_limit(0)
+        hooks.pla_th_trsv_give_up_at_first_wait(0)
     x, ref = run()
     assert np.abs(x - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
     lu, piv = _lu_gpu(s, n=1000, seed=3, nb=128)
