@@ -135,9 +135,12 @@ __global__ __launch_bounds__(256, 2) void fp8_update_kernel(fp8_upd_args a) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int e = tid + 256 * u, r = e >> 3, c = e & 7;
-      const int4 z = {0, 0, 0, 0};
-      ra[u] = (i0 + r < a.M) ? *(const int4 *)(a.Lr + (size_t)(i0 + r) * a.kq + ks * KC + c * 16) : z;
-      rb[u] = (j0 + r < a.N) ? *(const int4 *)(a.Lc + (size_t)(j0 + r) * a.kq + ks * KC + c * 16) : z;
+      // rows past M (N) read the last row: they feed only products that are
+      // never stored, and an unconditional load is a plain global load (a
+      // select between the row and a zero made it a flat load, waited for
+      // where it issued)
+      ra[u] = *(const int4 *)(a.Lr + (size_t)min(i0 + r, a.M - 1) * a.kq + ks * KC + c * 16);
+      rb[u] = *(const int4 *)(a.Lc + (size_t)min(j0 + r, a.N - 1) * a.kq + ks * KC + c * 16);
     }
   };
   auto sstore = [&](int buf) {
@@ -201,8 +204,7 @@ __global__ __launch_bounds__(256, 2) void fp8_update_kernel(fp8_upd_args a) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
           const int i = i0 + wm * 64 + (2 * h + b2) * 16 + l16, j = j0 + wn * 64 + bj * 16 + lq * 4 + r;
-          const bool in = i < a.M && j < a.N && (!TRI || j <= i);
-          cv[b2][bj][r] = in ? C[(long)j * a.ldc + i] : 0.0f;
+          cv[b2][bj][r] = C[(long)min(j, a.N - 1) * a.ldc + min(i, a.M - 1)];  // clamped, unguarded
         }
 #pragma unroll
     for (int b2 = 0; b2 < 2; b2++)
@@ -222,10 +224,17 @@ __global__ __launch_bounds__(256, 2) void fp8_update_kernel(fp8_upd_args a) {
 __global__ __launch_bounds__(256) void cvt_lower_shift_kernel(int n, const double *__restrict__ A, long lda,
                                                               double shift, float *__restrict__ W, long ldw) {
   const int j = blockIdx.y;
-  for (int i = j + blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-    double v = A[(long)j * lda + i];
-    if (i == j) v += shift;
-    W[(long)j * ldw + i] = (float)v;
+  const long stride = (long)gridDim.x * 256;
+  // four rows per thread and step, loads first (clamped to the column)
+  for (long i0 = j + blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = A[(long)j * lda + min(i0 + u * stride, (long)n - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const long i = i0 + u * stride;
+      if (i < n) W[(long)j * ldw + i] = (float)(i == j ? v[u] + shift : v[u]);
+    }
   }
 }
 

@@ -218,21 +218,41 @@ __device__ __forceinline__ void gemm_tile_fast(gemm_smem &sm, int m, int n, int 
 #pragma unroll
     for (int e = 0; e < 4; e++) rb[e] = *(const dbl2u *)pb(kb, e);
   };
-  // last, partial 16-deep step: depths >= k read as zero
+  // last, partial 16-deep step: depths >= k read as zero.  The loads read
+  // clamped depths (no condition, so they are all in flight together) and
+  // the selects come after them.
   auto load_tail = [&](int kb) {
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      const double *p = pa(kb, e);
-      const int d0 = TA ? kb + ak : kb + ak + 4 * e, d1 = TA ? d0 + 1 : d0;
-      ra[e].x = d0 < k ? p[0] : 0.0;
-      ra[e].y = d1 < k ? p[1] : 0.0;
+      if (TA) {
+        const double *row = A + (long)(i0 + 32 * e + ax) * lda;
+        ra[e].x = row[min(kb + ak, k - 1)];
+        ra[e].y = row[min(kb + ak + 1, k - 1)];
+      } else {
+        ra[e] = *(const dbl2u *)(A + (long)min(kb + ak + 4 * e, k - 1) * lda + i0 + ax);
+      }
     }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      const double *p = pb(kb, e);
+      if (TB) {
+        rb[e] = *(const dbl2u *)(B + (long)min(kb + bk + 4 * e, k - 1) * ldb + j0 + bx);
+      } else {
+        const double *col = B + (long)(j0 + 32 * e + bx) * ldb;
+        rb[e].x = col[min(kb + bk, k - 1)];
+        rb[e].y = col[min(kb + bk + 1, k - 1)];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int d0 = TA ? kb + ak : kb + ak + 4 * e, d1 = TA ? d0 + 1 : d0;
+      ra[e].x = d0 < k ? ra[e].x : 0.0;
+      ra[e].y = d1 < k ? ra[e].y : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
       const int d0 = TB ? kb + bk + 4 * e : kb + bk, d1 = TB ? d0 : d0 + 1;
-      rb[e].x = d0 < k ? p[0] : 0.0;
-      rb[e].y = d1 < k ? p[1] : 0.0;
+      rb[e].x = d0 < k ? rb[e].x : 0.0;
+      rb[e].y = d1 < k ? rb[e].y : 0.0;
     }
   };
   auto store = [&](int buf) {
