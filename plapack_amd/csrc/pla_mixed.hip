@@ -465,10 +465,6 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
     }
     unsigned it = 0;
     for (;;) {
-      if (spin_limit == 0) {  // a zero poll limit gives up at the first wait (uniform)
-        failed = true;
-        break;
-      }
       int ok = 1;
       for (int e = tid; e < cs * nrhs; e += 256) {
         const int c = e / nrhs, q = e - c * nrhs;
@@ -477,7 +473,8 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
         if ((lo & 0xFFFFFFFF00000000ULL) != tag || (hi & 0xFFFFFFFF00000000ULL) != tag) ok = 0;
         xb[c][q] = __longlong_as_double((long long)(((hi & 0xFFFFFFFFULL) << 32) | (lo & 0xFFFFFFFFULL)));
       }
-      if (__syncthreads_and(ok)) break;
+      // (a zero poll limit gives up at the first wait, data or not)
+      if (__syncthreads_and(ok) && spin_limit != 0) break;
       ++it;
       const int bail = (it >= spin_limit) || (tid == 0 && (it & 63) == 0 &&
                                                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -589,10 +586,6 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
     }
     unsigned it = 0;
     for (;;) {
-      if (spin_limit == 0) {  // a zero poll limit gives up at the first wait (uniform)
-        failed = true;
-        break;
-      }
       int ok = 1;
       for (int e = tid; e < TB * NR; e += 256) {
         const int c = e / NR, q = e % NR;
@@ -605,7 +598,8 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
         }
         x[c][q] = v;
       }
-      if (__syncthreads_and(ok)) break;
+      // (a zero poll limit gives up at the first wait, data or not)
+      if (__syncthreads_and(ok) && spin_limit != 0) break;
       ++it;
       const int bail = (it >= spin_limit) || (tid == 0 && (it & 63) == 0 &&
                                                __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -685,16 +679,15 @@ __global__ __launch_bounds__(256) void resid_partial_kernel(int n, int nrhs, con
   for (int q = 0; q < NR; q++) s[q] = 0.0;
   const double *ap = A + (long)j0 * lda + i;
   int j = j0;
-  // eight columns' loads in flight per step; the products are added in
-  // column order whatever the step
-  for (; j + 8 <= j1; j += 8, ap += 8 * lda) {
-    double a[8];
+  for (; j + 4 <= j1; j += 4, ap += 4 * lda) {
+    const double a0 = ap[0], a1 = ap[lda], a2 = ap[2 * lda], a3 = ap[3 * lda];
 #pragma unroll
-    for (int u = 0; u < 8; u++) a[u] = ap[u * lda];
-#pragma unroll
-    for (int q = 0; q < NR; q++)
-#pragma unroll
-      for (int u = 0; u < 8; u++) s[q] = fma(a[u], xs[j - j0 + u][q], s[q]);
+    for (int q = 0; q < NR; q++) {
+      s[q] = fma(a0, xs[j - j0][q], s[q]);
+      s[q] = fma(a1, xs[j - j0 + 1][q], s[q]);
+      s[q] = fma(a2, xs[j - j0 + 2][q], s[q]);
+      s[q] = fma(a3, xs[j - j0 + 3][q], s[q]);
+    }
   }
   for (; j < j1; j++, ap += lda) {
     const double a0 = ap[0];
