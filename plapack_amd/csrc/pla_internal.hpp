@@ -317,9 +317,10 @@ int trsm_leaf_size();
 // ---- mixed precision and FP64-accurate solves (pla_mixed.hip) ----
 int k_panel_swap_f32(hipStream_t s, float *A, long lda, int nleft, int c1, int c2, const int *comp);
 // C (FP32) -= A B with A and B in the packed k-slice layout (pla_mixed.hip):
-// A rows 0..m-1 (slice stride sa), B columns boff..boff+n-1 (slice stride sb)
+// A rows 0..m-1 (slice stride sa), B columns boff..boff+n-1 (slice stride sb);
+// P (optional): also the updated m x n block of C in FP64 (leading dimension ldp)
 int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long sa, const void *Bh,
-            const void *Bl, long sb, long boff, float *C, long ldc);
+            const void *Bl, long sb, long boff, float *C, long ldc, double *P = nullptr, long ldp = 0);
 int k_cvt_f64_f32(hipStream_t s, int m, int n, const double *A, long lda, float *W, long ldw);
 int k_cvt_f32_f64(hipStream_t s, int m, int n, const float *W, long ldw, double *P, long ldp);
 int k_panel_out(hipStream_t s, int prec, int rows, int b, int kpad, const double *P, long ldp, void *Lh, void *Ll,

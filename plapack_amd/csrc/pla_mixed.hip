@@ -94,7 +94,8 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
                                                        const H *__restrict__ Al, long sa,
                                                        const H *__restrict__ Bh, const H *__restrict__ Bl,
                                                        long sb, long boff, float *__restrict__ C, long ldc,
-                                                       int tiles_m, int tiles_n, int vec_ok) {
+                                                       double *__restrict__ P, long ldp, int tiles_m,
+                                                       int tiles_n, int vec_ok) {
   typedef typename hvec<H>::type V;
   constexpr int NS = SPLIT ? 2 : 1;
   __shared__ V As[2][NS][HBM * 4];
@@ -201,9 +202,20 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
 #pragma unroll
     for (int bi = 0; bi < 4; bi++)
 #pragma unroll
-      for (int bj = 0; bj < 4; bj++)
-        *(f4v *)(C + (long)(j0 + wn * 64 + bj * 16 + l16) * ldc + i0 + wm * 64 + bi * 16 + lq * 4) =
-            cv[bi][bj] - acc[bi][bj];
+      for (int bj = 0; bj < 4; bj++) {
+        cv[bi][bj] -= acc[bi][bj];
+        *(f4v *)(C + (long)(j0 + wn * 64 + bj * 16 + l16) * ldc + i0 + wm * 64 + bi * 16 + lq * 4) = cv[bi][bj];
+      }
+    if (P) {  // the FP64 copy of the updated block (exact)
+#pragma unroll
+      for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) {
+          double *pp = P + (long)(j0 + wn * 64 + bj * 16 + l16) * ldp + i0 + wm * 64 + bi * 16 + lq * 4;
+#pragma unroll
+          for (int r = 0; r < 4; r++) pp[r] = (double)cv[bi][bj][r];
+        }
+    }
     return;
   }
 #pragma unroll
@@ -218,10 +230,18 @@ __global__ __launch_bounds__(256, 2) void hgemm_kernel(int m, int n, int k, cons
         f4v c4 = *(f4v *)cp;
         c4 -= acc[bi][bj];
         *(f4v *)cp = c4;
+        if (P) {
+#pragma unroll
+          for (int r = 0; r < 4; r++) P[(long)col * ldp + row0 + r] = (double)c4[r];
+        }
       } else {
 #pragma unroll
         for (int r = 0; r < 4; r++)
-          if (row0 + r < m) cp[r] -= acc[bi][bj][r];
+          if (row0 + r < m) {
+            const float v = cp[r] - acc[bi][bj][r];
+            cp[r] = v;
+            if (P) P[(long)col * ldp + row0 + r] = (double)v;
+          }
       }
     }
 }
@@ -765,18 +785,19 @@ namespace pla {
 // ---- launchers ---------------------------------------------------------------
 
 int k_hgemm(hipStream_t s, int prec, int m, int n, int k, const void *Ah, const void *Al, long sa, const void *Bh,
-            const void *Bl, long sb, long boff, float *C, long ldc) {
+            const void *Bl, long sb, long boff, float *C, long ldc, double *P, long ldp) {
   if (m <= 0 || n <= 0 || k <= 0) return PLA_SUCCESS;
   if (k % HKS || sa % 256 || sb % 256 || boff % 8)
     return fail(PLA_ERR_INVALID_PARAM, "hgemm: k %% 32, slice strides %% 256, column offset %% 8");
   if (sa < (long)m * 32 || sb < (boff + n) * 32) return fail(PLA_ERR_INVALID_PARAM, "hgemm: slice too small");
+  if (P && ldp < m) return fail(PLA_ERR_INVALID_PARAM, "hgemm: FP64 copy leading dimension %ld < %d", ldp, m);
   const int tm = (m + HBM - 1) / HBM, tn = (n + HBN - 1) / HBN;
   const int nblk = ((tm * tn + 7) / 8) * 8;
   const int vec_ok = (ldc % 4 == 0) && (((uintptr_t)C & 15) == 0);
   dim3 grid(nblk), blk(256);
 #define PLA_HG(H, SP)                                                                                             \
   hipLaunchKernelGGL((hgemm_kernel<H, SP>), grid, blk, 0, s, m, n, k, (const H *)Ah, (const H *)Al, sa,           \
-                     (const H *)Bh, (const H *)Bl, sb, boff, C, ldc, tm, tn, vec_ok)
+                     (const H *)Bh, (const H *)Bl, sb, boff, C, ldc, P, ldp, tm, tn, vec_ok)
   switch (prec) {
     case PLA_MIXED_FP16: PLA_HG(_Float16, false); break;
     case PLA_MIXED_FP16X3: PLA_HG(_Float16, true); break;

@@ -62,12 +62,13 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
   auto steps = [&]() -> int {
     // panel at column k, on the look-ahead stream
     auto P64_of = [&](int par) { return P64 + (size_t)par * n * nb; };
-    auto panel = [&](int k, int b, int par, int step) -> int {
+    // (have_p64: the update of the panel's columns already wrote its FP64 copy)
+    auto panel = [&](int k, int b, int par, int step, bool have_p64) -> int {
       const int mp = n - k;
       float *Wk = W + (long)k * ldw + k;
       double *P64 = P64_of(par);
       int r;
-      if ((r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
+      if (!have_p64 && (r = k_cvt_f32_f64(sP, mp, b, Wk, ldw, P64, mp))) return r;
       if ((r = lu_panel(sP, P64, mp, b, mp, piv + k, pairs + par * pstride, &nleaves[par], gcap))) return r;
       if ((r = k_panel_compose(sP, pairs + par * pstride, nleaves[par], comp + (size_t)step * cstride)))
         return r;
@@ -77,7 +78,7 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
     };
     {
       const int b = width_at(0);
-      if ((rc = panel(0, b, 0, 0))) return rc;
+      if ((rc = panel(0, b, 0, 0, false))) return rc;
       PLA_HIP(hipEventRecord(ev_panel, sP));
     }
     int step = 0;
@@ -109,7 +110,12 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
         if ((r = k_upanel_out(sM, prec, b, kpad, nc, U64c, b, W12c, ldw, Uh, split ? Ul : nullptr, slice, c0)))
           return r;
         timing_begin(sM, 3);
-        r = k_hgemm(sM, prec, mr, nc, kpad, L_h, L_l, slice, Uh, split ? Ul : nullptr, slice, c0, W12c + b, ldw);
+        // the next panel's columns (c0 = 0, bn > 0): the update also writes
+        // the next panel's FP64 copy (rows kn.., columns kn..kn+bn, ld mr),
+        // so the panel stream starts factoring without a conversion pass
+        double *P_next = (c0 == 0 && bn > 0) ? P64_of(par ^ 1) : nullptr;
+        r = k_hgemm(sM, prec, mr, nc, kpad, L_h, L_l, slice, Uh, split ? Ul : nullptr, slice, c0, W12c + b, ldw,
+                    P_next, mr);
         timing_end(sM, 3, 2.0 * mr * (double)nc * b);
         return r;
       };
@@ -121,7 +127,7 @@ int lu_factor_mixed(hipStream_t sMain, int prec, int n, float *W, long ldw, int 
       }
       if (bn > 0) {
         PLA_HIP(hipStreamWaitEvent(sP, ev_ready, 0));
-        if ((rc = panel(kn, bn, par ^ 1, step + 1))) return rc;
+        if ((rc = panel(kn, bn, par ^ 1, step + 1, true))) return rc;
         PLA_HIP(hipEventRecord(ev_panel, sP));
       }
       if ((rc = right_part(n1, nr - n1))) return rc;
