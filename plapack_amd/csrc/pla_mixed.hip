@@ -401,10 +401,14 @@ __device__ inline bool wait_ready(const unsigned long long *__restrict__ w, unsi
   return *s_flag == 0;
 }
 
-// After the solution words of block blk are stored by every thread: release
-// them, then the ready word.
+// After the solution words of block blk are stored by every thread: the
+// ready word.  The words are device-scope atomic stores (they do not stay in
+// this XCD's L2) and each carries the launch's tag, which readers check, so
+// waiting for the block's own stores to complete orders the ready word after
+// them; a device-scope fence here would also write back and invalidate the
+// whole L2 of the XCD at every block.
 __device__ inline void publish_ready(unsigned long long *__restrict__ w, unsigned long long want) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
   __syncthreads();
   if (threadIdx.x == 0) st_dev64(w, want);
 }
