@@ -401,14 +401,13 @@ __device__ inline bool wait_ready(const unsigned long long *__restrict__ w, unsi
   return *s_flag == 0;
 }
 
-// After the solution words of block blk are stored by every thread: the
-// ready word.  The words are device-scope atomic stores (they do not stay in
-// this XCD's L2) and each carries the launch's tag, which readers check, so
-// waiting for the block's own stores to complete orders the ready word after
-// them; a device-scope fence here would also write back and invalidate the
-// whole L2 of the XCD at every block.
+// After the solution words of block blk are issued by every thread: the
+// ready word.  It may become visible before some of the words (no fence:
+// a device-scope fence here would write back and invalidate the XCD's whole
+// L2 at every block, and waiting for the stores would put their round trip
+// on the chain); readers check every word's tag and re-read, so it only
+// tells them when to start reading.
 __device__ inline void publish_ready(unsigned long long *__restrict__ w, unsigned long long want) {
-  __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
   __syncthreads();
   if (threadIdx.x == 0) st_dev64(w, want);
 }
@@ -416,11 +415,12 @@ __device__ inline void publish_ready(unsigned long long *__restrict__ w, unsigne
 // Solve M X = X in place (n x nrhs, nrhs <= NR <= 16).  xs: exchange words
 // [nblk][TB][MAXRHS][2], then one ready word per block; tag unique per launch.
 // A block's solution words carry the tag; readers re-read them until every
-// tag matches.  From NR = 4 on a ready word (tag | 1) also follows the words
-// behind a fence, and readers wait for it before reading the words, so the
-// blocks that wait do not stream the words through the memory system (at
-// n = 49152: 8 columns 8.0 -> 5.5 ms, 16 columns 15.3 -> 10.1 ms; with 1 or
-// 2 columns the fence and the extra round trip cost more than they save).
+// tag matches.  From NR = 4 on a ready word (tag | 1) also follows the words,
+// and a block waiting for any but its immediate predecessor waits for that
+// word before reading the words, so the blocks that wait do not stream the
+// words through the memory system (at n = 49152: 8 columns 8.0 -> 5.5 ms, 16
+// columns 15.3 -> 10.1 ms; with 1 or 2 columns the extra round trip cost
+// more than it saved).
 //
 // Thread t owns row t%64 of its block and the columns c = t/64 (mod 4) of
 // every tile; its partial sums stay in registers across tiles and the four
@@ -483,7 +483,10 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
       tile[i][c] = (double)treg[k];
     }
     if (s + 1 < b) load_tile(s + 1);  // next tile in flight during this step
-    if (READY && !wait_ready(rdy + iblk, tag | 1, spin_limit, err, &s_flag)) {
+    // the ready word only while the awaited block is not the chain's head:
+    // the immediate predecessor's words are polled directly (one block at a
+    // time does that), which saves the ready word's round trip on the chain
+    if (READY && s + 1 < b && !wait_ready(rdy + iblk, tag | 1, spin_limit, err, &s_flag)) {
       failed = true;
       break;
     }
@@ -604,7 +607,10 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
     const int cs = min(TB, n - iblk * TB);
     const unsigned long long *src = xs + (size_t)iblk * TB * MAXRHS * 2;
     double (*x)[NR] = xb[s & 1];
-    if (READY && !wait_ready(rdy + iblk, tag | 1, spin_limit, err, &s_flag)) {
+    // the ready word only while the awaited block is not the chain's head:
+    // the immediate predecessor's words are polled directly (one block at a
+    // time does that), which saves the ready word's round trip on the chain
+    if (READY && s + 1 < b && !wait_ready(rdy + iblk, tag | 1, spin_limit, err, &s_flag)) {
       failed = true;
       break;
     }
