@@ -45,13 +45,13 @@ ROLE_LU, ROLE_CHOL, ROLE_MIXED, ROLE_FP8, ROLE_RESID, ROLE_TRSV, ROLE_SWAP = 1, 
 # HBM bytes per launch at the default sizes (rocprofv3 --pmc FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the scale factors measured on a copy of known size);
 # profiles/trailing_gemm_traffic_n65536.md and profiles/hbm_kernels_traffic.md
-TRAFFIC = {"hgemm": 9.33e9, "dgemm": 5.91e10,
+TRAFFIC = {"hgemm": 9.40e9, "dgemm": 3.17e10,
            # n = 65536: the mixed and plain LU's HBM kernels
-           "resid_lu": 3.445e10, "trsv_f32": 8.706e9, "swap_f32": 2.088e9, "swap_f64": 5.619e9,
+           "resid_lu": 3.445e10, "trsv_f32": 8.706e9, "swap_f32": 1.711e9, "swap_f64": 4.298e9,
            "trsv_f64": 1.883e10,
            # n = 49152, FP8 Cholesky: per timed update (a triangle launch, and a
            # rectangle launch every other time), residual, 8-column solve
-           "fp8_update": 4.05e9, "resid_fp8": 1.966e10, "trsv_fp8": 4.934e9}
+           "fp8_update": 4.05e9, "resid_fp8": 1.966e10, "trsv_fp8": 4.946e9}
 
 
 def cpu_baseline(n_lu, n_chol):
