@@ -84,6 +84,7 @@ struct context {
   int nb_switch = 0;
   unsigned leaf_spin_limit = 400000u;  // bounded wait of a panel-leaf exchange (polls)
   unsigned trsv_spin_limit = 2000000u; // bounded wait of a triangular-solve block (polls; 0: none)
+  int trsv_group_cols = 4;  // untransposed solves: columns per independent chain (0: all in one)
   // FP8 Cholesky (PLA_Chol_mixed): block size (256 or 512), whether its panel
   // runs on the CU-masked Cholesky stream pair, and the first diagonal shift
   // tried after an unshifted factorization fails, as 2^fp8_shift_exp times

@@ -24,12 +24,13 @@
 //
 // Triangular solves (trsv_sf_kernel, also used by PLA_Trsv / the solves of
 // the plain path for up to 16 right-hand sides).  One workgroup per 64-row
-// block in solve order.  Each workgroup accumulates the contributions of the
-// earlier blocks as their solutions appear, then multiplies by the inverted
-// diagonal block.  Workgroups publish their solution blocks as tagged 64-bit
-// words (tag | 32-bit half), so readers need no flags or fences.  A
-// workgroup waits only on workgroups with smaller ids, which were
-// dispatched before it.
+// block in solve order (untransposed solves: per 64-row block and group of
+// at most 4 columns, each group its own chain).  Each workgroup accumulates
+// the contributions of the earlier blocks as their solutions appear, then
+// multiplies by the inverted diagonal block.  Workgroups publish their
+// solution blocks as tagged 64-bit words (tag | 32-bit half), so readers
+// need no flags or fences.  A workgroup waits only on workgroups with
+// smaller ids, which were dispatched before it.
 #include "pla_internal.hpp"
 
 #include <climits>
@@ -412,6 +413,38 @@ __device__ inline void publish_ready(unsigned long long *__restrict__ w, unsigne
   if (threadIdx.x == 0) st_dev64(w, want);
 }
 
+// One poll of a published solution block: rows 0..cs-1, columns q0..q0+nq-1
+// of the words at src into x[c][q] (zero outside).  Every thread issues all
+// of its word loads before it looks at any (a loop that tested each pair
+// before loading the next waited for one round trip per pair).  Returns
+// whether every tag this thread read matched.
+template <int NR>
+__device__ __forceinline__ int poll_words(const unsigned long long *__restrict__ src, int cs, int q0, int nq,
+                                          unsigned long long tag, double (*x)[NR]) {
+  constexpr int PW = (TB * NR + 255) / 256;
+  const int tid = threadIdx.x;
+  if (TB * NR < 256 && tid >= TB * NR) return 1;  // whole waves (TB * NR is a multiple of 64)
+  unsigned long long lo[PW], hi[PW];
+#pragma unroll
+  for (int j = 0; j < PW; j++) {
+    const int e = tid + 256 * j, c = e / NR, q = e % NR;
+    // outside the block: the first word, always in bounds, read and ignored
+    const size_t w = (c < cs && q < nq) ? ((size_t)c * MAXRHS + q0 + q) * 2 : 0;
+    lo[j] = ld_dev64(src + w);
+    hi[j] = ld_dev64(src + w + 1);
+  }
+  int ok = 1;
+#pragma unroll
+  for (int j = 0; j < PW; j++) {
+    const int e = tid + 256 * j, c = e / NR, q = e % NR;
+    const bool in = c < cs && q < nq;
+    if (in && ((lo[j] & 0xFFFFFFFF00000000ULL) != tag || (hi[j] & 0xFFFFFFFF00000000ULL) != tag)) ok = 0;
+    const double v = __longlong_as_double((long long)(((hi[j] & 0xFFFFFFFFULL) << 32) | (lo[j] & 0xFFFFFFFFULL)));
+    x[c][q] = in ? v : 0.0;
+  }
+  return ok;
+}
+
 // Solve M X = X in place (n x nrhs, nrhs <= NR <= 16).  xs: exchange words
 // [nblk][TB][MAXRHS][2], then one ready word per block; tag unique per launch.
 // A block's solution words carry the tag; readers re-read them until every
@@ -492,14 +525,7 @@ __global__ __launch_bounds__(256) void trsv_sf_kernel(int n, int nrhs, const T *
     }
     unsigned it = 0;
     for (;;) {
-      int ok = 1;
-      for (int e = tid; e < cs * nrhs; e += 256) {
-        const int c = e / nrhs, q = e - c * nrhs;
-        const unsigned long long lo = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2);
-        const unsigned long long hi = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2 + 1);
-        if ((lo & 0xFFFFFFFF00000000ULL) != tag || (hi & 0xFFFFFFFF00000000ULL) != tag) ok = 0;
-        xb[c][q] = __longlong_as_double((long long)(((hi & 0xFFFFFFFFULL) << 32) | (lo & 0xFFFFFFFFULL)));
-      }
+      const int ok = poll_words<NR>(src, cs, 0, nrhs, tag, xb);
       // (a zero poll limit gives up at the first wait, data or not)
       if (__syncthreads_and(ok) && spin_limit != 0) break;
       ++it;
@@ -564,8 +590,22 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
   __shared__ int s_flag;
   constexpr bool READY = NR >= 4;  // ready words: measured faster from 4 columns on
   const int nblk = (n + TB - 1) / TB;
-  unsigned long long *const rdy = xs + (size_t)nblk * TB * MAXRHS * 2;
-  const int b = blockIdx.x;
+  // Column groups: columns q0 .. q0 + NR - 1 form an independent chain.  In
+  // every run of 8 G blocks (G groups) the first 8 are row blocks 8p .. 8p+7
+  // of group 0, the next 8 the same row blocks of group 1, and so on: each
+  // chain is dispatched in its own order, and the G blocks of one row block
+  // land on the same XCD (blocks are spread over the 8 XCDs round-robin), so
+  // the factor tiles they share are read through that XCD's L2.
+  const int G = (nrhs + NR - 1) / NR;
+  int b = blockIdx.x, grp = 0;
+  if (G > 1) {
+    const int r = (int)(blockIdx.x % (8u * G));
+    b = (int)(blockIdx.x / (8u * G)) * 8 + (r & 7);
+    grp = r >> 3;
+    if (b >= nblk) return;  // padding of the last run (waited on by no block)
+  }
+  const int q0 = grp * NR, nq = min(NR, nrhs - q0);
+  unsigned long long *const rdy = xs + (size_t)nblk * TB * MAXRHS * 2 + (size_t)grp * nblk;
   const int blk = LOWER ? b : nblk - 1 - b;
   const int r0 = blk * TB, bs = min(TB, n - r0);
   const int tid = threadIdx.x, row = tid & 63, cg = tid >> 6;
@@ -580,7 +620,7 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
 #pragma unroll
   for (int j = 0; j < RPT; j++) {
     const int e = tid + 256 * j, i = e / NR, q = e % NR;
-    rhs[j] = (e < TB * NR && i < bs && q < nrhs) ? X[(long)q * ldx + r0 + i] : 0.0;
+    rhs[j] = (e < TB * NR && i < bs && q < nq) ? X[(long)(q0 + q) * ldx + r0 + i] : 0.0;
   }
   // wait for them here, while the block has nothing else to do
 #pragma unroll
@@ -616,18 +656,7 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
     }
     unsigned it = 0;
     for (;;) {
-      int ok = 1;
-      for (int e = tid; e < TB * NR; e += 256) {
-        const int c = e / NR, q = e % NR;
-        double v = 0.0;
-        if (c < cs && q < nrhs) {
-          const unsigned long long lo = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2);
-          const unsigned long long hi = ld_dev64(src + ((size_t)c * MAXRHS + q) * 2 + 1);
-          if ((lo & 0xFFFFFFFF00000000ULL) != tag || (hi & 0xFFFFFFFF00000000ULL) != tag) ok = 0;
-          v = __longlong_as_double((long long)(((hi & 0xFFFFFFFFULL) << 32) | (lo & 0xFFFFFFFFULL)));
-        }
-        x[c][q] = v;
-      }
+      const int ok = poll_words<NR>(src, cs, q0, nq, tag, x);
       // (a zero poll limit gives up at the first wait, data or not)
       if (__syncthreads_and(ok) && spin_limit != 0) break;
       ++it;
@@ -674,13 +703,13 @@ __global__ __launch_bounds__(256, (NR == 1 && sizeof(T) == 4) ? 4 : 1) void trsv
   for (int q = 0; q < NR; q++) red[cg][row][q] = y[q];
   __syncthreads();
   unsigned long long *dst = xs + (size_t)blk * TB * MAXRHS * 2;
-  for (int e = tid; e < bs * nrhs; e += 256) {
+  for (int e = tid; e < bs * nq; e += 256) {
     const int i = e % bs, q = e / bs;
     const double t = (red[0][i][q] + red[1][i][q]) + (red[2][i][q] + red[3][i][q]);
-    X[(long)q * ldx + r0 + i] = t;
+    X[(long)(q0 + q) * ldx + r0 + i] = t;
     const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
-    st_dev64(dst + ((size_t)i * MAXRHS + q) * 2, tag | (bits & 0xFFFFFFFFULL));
-    st_dev64(dst + ((size_t)i * MAXRHS + q) * 2 + 1, tag | (bits >> 32));
+    st_dev64(dst + ((size_t)i * MAXRHS + q0 + q) * 2, tag | (bits & 0xFFFFFFFFULL));
+    st_dev64(dst + ((size_t)i * MAXRHS + q0 + q) * 2 + 1, tag | (bits >> 32));
   }
   if (READY) publish_ready(rdy + blk, tag | 1);
 }
@@ -880,12 +909,17 @@ template <typename T, int NR>
 static int trsv_launch(hipStream_t s, bool lower, bool trans, int n, int nrhs, const T *A, long lda, double *X,
                        long ldx, const double *Dinv, unsigned long long *xs, unsigned tag, int *err) {
   const int nblk = (n + TB - 1) / TB;
+  // untransposed: ceil(nrhs / NR) column groups, each its own chain, in runs
+  // of 8 blocks per group (the kernel's block order)
+  const int G = (nrhs + NR - 1) / NR;
+  if (trans && G > 1) return fail(PLA_ERR_INVALID_PARAM, "trsv: column groups in a transposed solve");
+  const unsigned grid_nt = G > 1 ? 8u * (unsigned)G * (unsigned)((nblk + 7) / 8) : (unsigned)nblk;
 #define PLA_TSF(LO, TR)                                                                                      \
   hipLaunchKernelGGL((trsv_sf_kernel<T, LO, TR, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, \
                      ldx, xs, tag, ctx().trsv_spin_limit, err)
 #define PLA_TNT(LO)                                                                                          \
-  hipLaunchKernelGGL((trsv_nt_kernel<T, LO, NR>), dim3(nblk), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, ldx, \
-                     xs, tag, ctx().trsv_spin_limit, err)
+  hipLaunchKernelGGL((trsv_nt_kernel<T, LO, NR>), dim3(grid_nt), dim3(256), 0, s, n, nrhs, A, lda, Dinv, X, \
+                     ldx, xs, tag, ctx().trsv_spin_limit, err)
   if (lower && !trans) PLA_TNT(true);
   else if (lower && trans) PLA_TSF(true, true);
   else if (!lower && !trans) PLA_TNT(false);
@@ -902,8 +936,8 @@ static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n,
   if (n <= 0 || nrhs <= 0) return PLA_SUCCESS;
   if (nrhs > MAXRHS) return fail(PLA_ERR_INVALID_PARAM, "trsv: %d right-hand sides (max %d)", nrhs, MAXRHS);
   const int nblk = (n + TB - 1) / TB;
-  // solution words, then one ready word per block
-  const size_t xs_words = (size_t)nblk * TB * MAXRHS * 2 + nblk;
+  // solution words, then one ready word per block and column group
+  const size_t xs_words = (size_t)nblk * TB * MAXRHS * 2 + (size_t)nblk * MAXRHS;
   unsigned long long *xs = (unsigned long long *)workspace(sizeof(unsigned long long) * xs_words, 5, true);
   if (!xs) return PLA_ERR_NO_MEMORY;
   int *err = leaf_error_word();
@@ -930,10 +964,16 @@ static int trsv_sf_impl(hipStream_t s, bool lower, bool trans, bool unit, int n,
   const unsigned tag = g_trsv_tag++;
   int rc;
   timing_begin(s, 6);
-  if (nrhs <= 1) rc = trsv_launch<T, 1>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
-  else if (nrhs <= 2) rc = trsv_launch<T, 2>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
-  else if (nrhs <= 4) rc = trsv_launch<T, 4>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
-  else if (nrhs <= 8) rc = trsv_launch<T, 8>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  // Untransposed solves split the columns into groups of at most
+  // trsv_group_cols, each group an independent chain of blocks: a chain's
+  // link costs more the more columns it carries, the factor is read once per
+  // group.
+  const int gc = ctx().trsv_group_cols;
+  const int width = (!trans && gc > 0 && nrhs > gc) ? gc : nrhs;
+  if (width <= 1) rc = trsv_launch<T, 1>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else if (width <= 2) rc = trsv_launch<T, 2>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else if (width <= 4) rc = trsv_launch<T, 4>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
+  else if (width <= 8) rc = trsv_launch<T, 8>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
   else rc = trsv_launch<T, MAXRHS>(s, lower, trans, n, nrhs, A, lda, X, ldx, Dinv, xs, tag, err);
   // algorithmic bytes: the triangle once, the inverted diagonal blocks once, X read and written
   timing_end(s, 6, sizeof(T) * 0.5 * n * (n + 1.0) + 8.0 * TB * (double)n + 16.0 * nrhs * (double)n);

@@ -92,4 +92,12 @@ int pla_th_trsv_give_up_at_first_wait(int on) {
   return PLA_SUCCESS;
 }
 
+// Columns per independent chain of the untransposed triangular solves
+// (1, 2, 4, 8 or 16; 0: every column in one chain).
+int pla_th_set_trsv_group_cols(int cols) {
+  if (cols != 0 && cols != 1 && cols != 2 && cols != 4 && cols != 8 && cols != 16) return PLA_ERR_INVALID_PARAM;
+  ctx().trsv_group_cols = cols;
+  return PLA_SUCCESS;
+}
+
 }  // extern "C"

@@ -385,6 +385,44 @@ def test_triangular_solve_timeout_reports_error(pla_session, ora, entry):
     assert np.array_equal(piv, piv_o)
 
 
+@pytest.mark.parametrize("nrhs", [5, 8, 13, 16])
+def test_solve_column_groups_bit_identical(pla_session, ora, nrhs):
+    """The untransposed triangular solves split the right-hand sides into
+    independent chains of at most 4 columns (the default); 0 keeps every
+    column in one chain.  Each column's arithmetic is the same in every
+    grouping, so PLA_General_solve (unit lower, then upper) and
+    PLA_Pos_def_solve (lower, then lower transposed) give bit-identical
+    solutions for every group width, ragged last group and 64-row block
+    (n = 1500) included, and they solve the systems."""
+    hooks = ctypes.CDLL(os.path.join(os.path.dirname(pla.LIB_PATH), "libpla_testhooks.so"))
+    hooks.pla_th_set_trsv_group_cols.argtypes = [ctypes.c_int]
+    s = pla_session
+    n = 1500
+    a = ora.uniform(n, n, 8)
+    spd = a @ a.T / n + np.eye(n)
+    x = np.random.default_rng(nrhs).uniform(-1, 1, (n, nrhs))
+    outs = {}
+    try:
+        for gc in (0, 1, 2, 4, 8, 16):
+            assert hooks.pla_th_set_trsv_group_cols(gc) == pla.PLA_SUCCESS
+            A, B, P = pla.from_numpy(s, a), pla.from_numpy(s, a @ x), s.mvector(n, dtype=pla.MPI_INT)
+            pla.call("PLA_General_solve", A, P, B)
+            xl = pla.get_contents(B)
+            pla.free(A, B, P)
+            A, B = pla.from_numpy(s, spd), pla.from_numpy(s, spd @ x)
+            pla.call("PLA_Pos_def_solve", A, B)
+            outs[gc] = (xl, pla.get_contents(B))
+            pla.free(A, B)
+    finally:
+        hooks.pla_th_set_trsv_group_cols(4)
+    assert hooks.pla_th_set_trsv_group_cols(3) == pla.PLA_ERR_INVALID_PARAM
+    for gc, (xl, xc) in outs.items():
+        assert np.array_equal(xl, outs[0][0]), f"LU solve, {gc} columns per chain"
+        assert np.array_equal(xc, outs[0][1]), f"Cholesky solve, {gc} columns per chain"
+    assert np.abs(outs[4][0] - x).max() < 1e-9
+    assert np.abs(outs[4][1] - x).max() < 1e-9
+
+
 @pytest.mark.parametrize("m,n,nb,tail,switch", [(3000, 3000, 512, 128, 1500), (2500, 1100, 1024, 256, 2000),
                                                 (2000, 2000, 256, 64, 100000)])
 def test_lu_block_size_schedule(pla_session, ora, m, n, nb, tail, switch):
