@@ -51,7 +51,7 @@ TRAFFIC = {"hgemm": 9.40e9, "dgemm": 3.17e10,
            "trsv_f64": 1.883e10,
            # n = 49152, FP8 Cholesky: per timed update (a triangle launch, and a
            # rectangle launch every other time), residual, 8-column solve
-           "fp8_update": 4.05e9, "resid_fp8": 1.966e10, "trsv_fp8": None}
+           "fp8_update": 4.05e9, "resid_fp8": 1.966e10, "trsv_fp8": 7.845e9}
 
 
 def cpu_baseline(n_lu, n_chol):
