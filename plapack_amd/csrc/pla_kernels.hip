@@ -1145,54 +1145,62 @@ __global__ __launch_bounds__(256) void trsm_right_leaf_kernel(int nrows, int b, 
 }
 
 // ---------------------------------------------------------------------------
-// Unblocked lower Cholesky of a b x b diagonal block (b <= 64) in one
-// workgroup: PLA_dchol's left-looking order (dgemv update of column k, test
-// a(k,k) <= 0, sqrt, scale by the reciprocal).  On failure *info receives
-// offset + k + 1 (1-based global index) and the block is left partially
-// factored; the caller restores it from its backup.
+// v of lane `src` (a compile-time lane after unrolling), as a scalar
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, src), hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Unblocked lower Cholesky of a b x b diagonal block (b <= 64) in one wave:
+// PLA_dchol's left-looking order (dgemv update of column k, test
+// a(k,k) <= 0, sqrt, scale by the reciprocal).  Lane i holds row i in
+// registers; L(k, j) reaches every lane by readlane, so a column costs no
+// barrier and no LDS round trip (the LDS version took about 60 us per block
+// beside other work).  The sums and roundings are the same as with the
+// block in LDS.  On failure *info receives offset + k + 1 (1-based global
+// index) and the block is not written; the caller restores it from its
+// backup.
 __global__ __launch_bounds__(64) void potrf_leaf_kernel(int b, double *__restrict__ Ablk, long lda,
                                                         int offset, int *__restrict__ info) {
   if (*info) return;
-  __shared__ double S[TRSM_LEAF][TRSM_LEAF + 1];  // S[i][j] = A(i,j), lower part used
-  __shared__ int fail;
   const int i = threadIdx.x;
+  double a[TRSM_LEAF];  // a[j] = A(i, j), j <= i; L(i, j) once column j is done
   {
-    // every load of the row in flight at once, then into LDS
-    double row[TRSM_LEAF];
+    // every load of the row in flight at once (clamped: always in bounds)
 #pragma unroll
-    for (int j = 0; j < TRSM_LEAF; j++) row[j] = (j < b && i < b && i >= j) ? Ablk[(long)j * lda + i] : 0.0;
+    for (int j = 0; j < TRSM_LEAF; j++) a[j] = Ablk[(long)min(j, b - 1) * lda + min(i, b - 1)];
 #pragma unroll
-    for (int j = 0; j < TRSM_LEAF; j++)
-      if (j < b && i < b && i >= j) S[i][j] = row[j];
+    for (int j = 0; j < TRSM_LEAF; j++) a[j] = (j < b && i < b && i >= j) ? a[j] : 0.0;
   }
-  if (i == 0) fail = 0;
-  __syncthreads();
-  for (int kk = 0; kk < b; kk++) {
-    if (i < b && i >= kk) {
-      double y = S[i][kk];
-      for (int j = 0; j < kk; j++) y = fma(-S[kk][j], S[i][j], y);
-      S[i][kk] = y;
+  // no early exit inside the unrolled loop (a[] stays in registers): a
+  // failed pivot stops the updates through `fail`
+  int fail = 0;
+#pragma unroll
+  for (int kk = 0; kk < TRSM_LEAF; kk++) {
+    if (kk < b && !fail) {  // uniform
+      double y = a[kk];
+#pragma unroll
+      for (int j = 0; j < kk; j++) y = fma(-lane_value(a[j], kk), a[j], y);
+      if (i >= kk) a[kk] = y;
+      const double d = lane_value(a[kk], kk);
+      if (d <= 0.0) {
+        fail = kk + 1;
+      } else {
+        const double sq = sqrt(d);
+        const double invert = 1.0 / sq;
+        if (i == kk) a[kk] = sq;
+        else if (i > kk) a[kk] *= invert;
+      }
     }
-    __syncthreads();
-    const double d = S[kk][kk];
-    if (d <= 0.0) {
-      if (i == 0) fail = kk + 1;
-      break;
-    }
-    const double sq = sqrt(d);
-    const double invert = 1.0 / sq;
-    __syncthreads();
-    if (i == kk) S[kk][kk] = sq;
-    if (i < b && i > kk) S[i][kk] *= invert;
-    __syncthreads();
   }
-  __syncthreads();
   if (fail) {
     if (i == 0) atomicCAS(info, 0, offset + fail);
     return;
   }
-  for (int j = 0; j < b; j++)
-    if (i < b && i >= j) Ablk[(long)j * lda + i] = S[i][j];
+#pragma unroll
+  for (int j = 0; j < TRSM_LEAF; j++)
+    if (j < b && i < b && i >= j) Ablk[(long)j * lda + i] = a[j];
 }
 
 // ---------------------------------------------------------------------------
