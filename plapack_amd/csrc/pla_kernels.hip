@@ -1046,20 +1046,30 @@ __global__ __launch_bounds__(256) void trsm_leaf_kernel(int b, int ncols, const 
   // row `lane` of M = op(T), strictly on the solved side of the diagonal,
   // read straight from T (64 x 64 at most, cached): no LDS, so the leaf can
   // run beside trailing-update workgroups that hold all of a CU's LDS
+  // (clamped indices: every load is in bounds and issued before any is
+  // used; a guarded load was waited for where it issued)
+  const int lc = min(lane, b - 1);
   double mrow[TRSM_LEAF];
 #pragma unroll
   for (int k = 0; k < TRSM_LEAF; k++) {
-    const bool in = lane < b && k < b && (FORWARD ? k < lane : k > lane);
-    mrow[k] = in ? (TRANS ? T[(long)lane * ldt + k] : T[(long)k * ldt + lane]) : 0.0;
+    const int kc = min(k, b - 1);
+    mrow[k] = TRANS ? T[(long)lc * ldt + kc] : T[(long)kc * ldt + lc];
   }
-  const double diag = (!UNIT && lane < b) ? T[(long)lane * ldt + lane] : 1.0;
+  double diag = UNIT ? 1.0 : T[(long)lc * ldt + lc];
+#pragma unroll
+  for (int k = 0; k < TRSM_LEAF; k++) {
+    const bool in = lane < b && k < b && (FORWARD ? k < lane : k > lane);
+    mrow[k] = in ? mrow[k] : 0.0;
+  }
+  diag = lane < b ? diag : 1.0;
   const int nwaves = gridDim.x * 4;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   for (int c0 = wave * TRSM_NC; c0 < ncols; c0 += nwaves * TRSM_NC) {
     double x[TRSM_NC];
 #pragma unroll
-    for (int q = 0; q < TRSM_NC; q++)
-      x[q] = (c0 + q < ncols && lane < b) ? B[(long)(c0 + q) * ldb + lane] : 0.0;
+    for (int q = 0; q < TRSM_NC; q++) x[q] = B[(long)min(c0 + q, ncols - 1) * ldb + lc];
+#pragma unroll
+    for (int q = 0; q < TRSM_NC; q++) x[q] = (c0 + q < ncols && lane < b) ? x[q] : 0.0;
 #pragma unroll
     for (int s = 0; s < TRSM_LEAF; s++) {
       const int k = FORWARD ? s : TRSM_LEAF - 1 - s;
@@ -1102,9 +1112,10 @@ __global__ __launch_bounds__(256) void trsm_right_leaf_kernel(int nrows, int b, 
     double tv[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-      const int e = threadIdx.x + 256 * q;
+      // past b * b: the last element again (in bounds, not stored)
+      const int e = min((int)threadIdx.x + 256 * q, b * b - 1);
       const int col = e / b, row = e - col * b;  // T(row, col), read down the columns
-      tv[q] = e < b * b ? T[(long)col * ldt + row] : 0.0;
+      tv[q] = T[(long)col * ldt + row];
     }
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -1123,7 +1134,9 @@ __global__ __launch_bounds__(256) void trsm_right_leaf_kernel(int nrows, int b, 
   if (r >= nrows) return;
   double v[TRSM_LEAF];
 #pragma unroll
-  for (int c = 0; c < TRSM_LEAF; c++) v[c] = (c < b) ? X[(long)c * ldx + r] : 0.0;
+  for (int c = 0; c < TRSM_LEAF; c++) v[c] = X[(long)min(c, b - 1) * ldx + r];
+#pragma unroll
+  for (int c = 0; c < TRSM_LEAF; c++) v[c] = c < b ? v[c] : 0.0;
 #pragma unroll
   for (int q = 0; q < TRSM_LEAF; q++) {
     const int kk = FWD ? q : TRSM_LEAF - 1 - q;
