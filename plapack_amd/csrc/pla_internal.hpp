@@ -93,6 +93,7 @@ struct context {
   int fp8_masked = 0;
   int fp8_shift_exp = -6;
   int fp8_gmres_first = 1;  // measured n = 49152, delta 4e-2: 9 GMRES steps vs 12 classic corrections
+  int fp8_side_cvt = 1;     // FP8 panel: the conversions of A21 and X21 on a side stream (off the panel chain)
   // CU-masked stream pair of the LU look-ahead: the panel stream runs on
   // masked_cus CUs (masked_cus / 8 per XCD), the trailing stream on the rest
   hipStream_t stream_panel = nullptr, stream_trail = nullptr;
@@ -104,7 +105,7 @@ int ensure_device();                 // lazily initialise device + streams
 // PLA_Init and destroyed at PLA_Finalize (nullptr outside that window).
 enum {
   EV_LU_READY, EV_LU_PANEL, EV_MLU_READY, EV_MLU_PANEL, EV_ZLU_READY, EV_ZLU_PANEL, EV_CHOL_READY,
-  EV_CHOL_PANEL, EV_JOIN0, EV_JOIN1, EV_JOIN2, EV_COUNT
+  EV_CHOL_PANEL, EV_JOIN0, EV_JOIN1, EV_JOIN2, EV_FP8_SIDE, EV_FP8_A21, EV_FP8_X21, EV_FP8_OUT, EV_COUNT
 };
 hipEvent_t lib_event(int id);
 

@@ -34,8 +34,11 @@ double g_last_shift = 0.0;
 //
 // update stream, step k: wait(panel k) -> lower(C[:bn, :bn]), C[bn:, :bn]
 //                        -> signal -> lower(C[bn:, bn:])
-// panel stream:          wait(signal) -> FP64 copy -> potrf -> trsm -> FP32
-//                        copy back -> quantize -> signal(panel k+1)
+// panel stream:          wait(signal) -> FP64 copy of the diagonal block ->
+//                        potrf -> inverse -> [wait A21] GEMM -> quantize ->
+//                        signal(panel k+1)
+// side stream:           A21 to FP64 (beside potrf), X21 to FP32 (beside the
+//                        quantization)
 int chol_factor_fp8(hipStream_t sMain, int n, float *W, long ldw, int *info_out) {
   const int nb = (ctx().fp8_nb == 256 || ctx().fp8_nb == 1024) ? ctx().fp8_nb : 512;
   const int kq = fp8_row_bytes(nb), QT = fp8_tile_rows();
@@ -54,6 +57,20 @@ int chol_factor_fp8(hipStream_t sMain, int n, float *W, long ldw, int *info_out)
   hipStream_t sP, sM;
   int rc;
   if ((rc = chol_streams_begin(sMain, &sP, &sM, ctx().fp8_masked != 0))) return rc;
+  // The conversions of the rows below the diagonal block (A21 to FP64 before
+  // the panel GEMM, X21 back to FP32 after it) run on a side stream, beside the
+  // diagonal block's factorization and the quantization: neither is needed by
+  // the next step's update.  Ordering: the side stream starts a step after the
+  // panel stream has waited for the update of the step's columns; the panel
+  // GEMM waits for the A21 conversion, which (side-stream order) also follows
+  // the previous step's X21 conversion, so X21 is not rewritten while read; the
+  // main stream waits for the side stream at the end.
+  const bool side = ctx().fp8_side_cvt && sP != sM;
+  hipStream_t sX = side ? ctx().stream_tr : sP;
+  hipEvent_t ev_side = lib_event(EV_FP8_SIDE), ev_a21 = lib_event(EV_FP8_A21), ev_x21 = lib_event(EV_FP8_X21),
+             ev_out = lib_event(EV_FP8_OUT);
+  if (side && (!sX || !ev_side || !ev_a21 || !ev_x21 || !ev_out))
+    return fail(PLA_ERR_NOT_INITIALIZED, "PLAPACK not yet initialized");
   // the column block at diagonal position k, quantized into parity par.
   // L21 = A21 inv(L11)^T is one GEMM against the explicitly inverted L11
   // (inv(L11) = L11 \ I): the triangular solve's chain of 64-column leaves
@@ -64,14 +81,27 @@ int chol_factor_fp8(hipStream_t sMain, int n, float *W, long ldw, int *info_out)
     const long ldp = n - k;
     float *Wk = W + (long)k * ldw + k;
     int r2;
-    if ((r2 = k_cvt_f32_f64(sP, n - k, b, Wk, ldw, P64, ldp))) return r2;
+    if (side && R > 0) {
+      PLA_HIP(hipEventRecord(ev_side, sP));
+      PLA_HIP(hipStreamWaitEvent(sX, ev_side, 0));
+      if ((r2 = k_cvt_f32_f64(sX, R, b, Wk + b, ldw, P64 + b, ldp))) return r2;
+      PLA_HIP(hipEventRecord(ev_a21, sX));
+      if ((r2 = k_cvt_f32_f64(sP, b, b, Wk, ldw, P64, ldp))) return r2;
+    } else if ((r2 = k_cvt_f32_f64(sP, n - k, b, Wk, ldw, P64, ldp))) {
+      return r2;
+    }
     if ((r2 = potrf_block(sP, b, P64, ldp, k, info))) return r2;
     if ((r2 = k_cvt_f64_f32(sP, b, b, P64, ldp, Wk, ldw))) return r2;
     if (R <= 0) return PLA_SUCCESS;
     if ((r2 = k_eye(sP, b, Linv, b))) return r2;
     if ((r2 = trsm_left(sP, true, false, false, b, b, P64, ldp, Linv, b, info))) return r2;
+    if (side) PLA_HIP(hipStreamWaitEvent(sP, ev_a21, 0));
     if ((r2 = k_gemm(sP, 0, false, true, R, b, b, 1.0, P64 + b, ldp, Linv, b, 0.0, X21, R, info))) return r2;
-    if ((r2 = k_cvt_f64_f32(sP, R, b, X21, R, Wk + b, ldw))) return r2;
+    if (side) {
+      PLA_HIP(hipEventRecord(ev_x21, sP));
+      PLA_HIP(hipStreamWaitEvent(sX, ev_x21, 0));
+    }
+    if ((r2 = k_cvt_f64_f32(sX, R, b, X21, R, Wk + b, ldw))) return r2;
     return k_fp8_quantize(sP, R, b, X21, R, Lq + par * lq_stride, exps + par * ex_stride);
   };
   // update of the trailing block's columns c0 .. c0 + nc (lower part) by panel k
@@ -110,6 +140,10 @@ int chol_factor_fp8(hipStream_t sMain, int n, float *W, long ldw, int *info_out)
     return PLA_SUCCESS;
   };
   rc = steps();
+  if (side) {
+    PLA_HIP(hipEventRecord(ev_out, sX));
+    PLA_HIP(hipStreamWaitEvent(sMain, ev_out, 0));
+  }
   const int rc_end = lu_streams_end(sMain, sP, sM);
   if (rc) return rc;
   if (rc_end) return rc_end;

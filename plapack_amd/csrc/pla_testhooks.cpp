@@ -57,6 +57,14 @@ int pla_th_set_fp8_gmres_first(int on) {
   return PLA_SUCCESS;
 }
 
+// FP8 Cholesky panel: the FP32 <-> FP64 conversions of the rows below the
+// diagonal block on a side stream (1) or on the panel stream (0); the factor
+// is the same either way.
+int pla_th_set_fp8_side_cvt(int on) {
+  ctx().fp8_side_cvt = on != 0;
+  return PLA_SUCCESS;
+}
+
 int pla_th_sym_residual(int n, int nrhs, const double *A, long lda, const double *X, long ldx, const double *B,
                         long ldb, double *R, long ldr, double *rmax) {
   return sym_residual(ctx().stream, n, nrhs, A, lda, X, ldx, B, ldb, R, ldr, rmax);

@@ -278,3 +278,29 @@ def test_lu_mixed_falls_back_to_gmres(pla_session):
     assert be.value <= 1e-14, (it.value, be.value, steps.value, gcorr.value)
     assert gcorr.value >= 1 and steps.value >= 1
     assert _eta(a, x, b) <= 2e-14
+
+
+@pytest.mark.parametrize("n,delta,nb,masked", [(1700, 4e-2, 256, 0), (1700, 4e-2, 512, 1), (1000, 4e-4, 512, 0)])
+def test_chol_fp8_side_stream_conversions_identical(pla_session, th, n, delta, nb, masked):
+    """The FP8 panel's conversions of the rows below the diagonal block run
+    on a side stream ordered by events (A21 before the panel GEMM, X21 after
+    it, the main stream joined at the end): the solution is bit-identical to
+    the same conversions on the panel stream, with either stream arrangement
+    and through a shifted retry (delta = 4e-4)."""
+    th.pla_th_set_fp8_side_cvt.argtypes = [ctypes.c_int]
+    rng = np.random.default_rng(n + nb + masked)
+    a = _spd(rng, n, delta)
+    b = rng.uniform(-1, 1, (n, 2))
+    out = []
+    th.pla_th_set_fp8_options(nb, masked, 0)
+    try:
+        for side in (0, 1):
+            th.pla_th_set_fp8_side_cvt(side)
+            x, it, be, stats = _chol_mixed(pla_session, a, b)
+            out.append((x, it, be, stats))
+    finally:
+        th.pla_th_set_fp8_side_cvt(1)
+        th.pla_th_set_fp8_options(512, 0, 0)
+    (x0, it0, be0, s0), (x1, it1, be1, s1) = out
+    assert np.array_equal(x0, x1) and (it0, be0, s0) == (it1, be1, s1)
+    assert be1 <= 1e-14 and _eta(a, x1, b) <= 2e-14
